@@ -1,0 +1,352 @@
+"""Drop-in `tricubic` / `quadcubic` classes backed by gfx950 HIP kernels.
+
+Same constructor and query API as the reference ARBInterp v1.8
+(src/ARBInterp/ARBInterp.py): `tricubic(field, *args, **kwargs)` with the
+positional 'quiet' switch and `mode='vector'|'norm'|'both'`, then `.Query(q)`,
+`.rQuery(Q)`, `.sQuery(q)`, `.allCoeffs()`, and the public attributes
+(nPos, hx/hy/hz[/ht], x/y/z[/t]IntMin/Max, nc, queryInds, basePointInds, A,
+inputfield, eps).  Return shapes, NaN rows written into the caller's array,
+banners, `SystemExit` on badly shaped fields and the IndexError -> sQuery
+fallback follow the reference.
+
+Extensions (keyword arguments the reference does not have; its defaults are
+kept): `precision='fp64'` (default, plain FP64) or `'mixed'` (FP32 table and
+evaluation), `quiet=True` as a keyword, and `query_device(q)` for positions
+already in GPU memory (a torch tensor), which returns torch tensors.
+
+Every numeric result comes from libarbinterp.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as N
+
+__version__ = "1.8"
+
+_BANNERS = {
+    "scalar": "--- Scalar field, ignoring switches, interpolating for magnitude and gradient --- \n",
+    "vector": "--- Vector field, interpolating for vector components --- \n",
+    "norm": "--- Vector field, interpolating for magnitude and gradient --- \n",
+    "both": "--- Vector field, interpolating vector components plus magnitude and gradient --- \n",
+    "invalid": "--- Vector field, invalid option, defaulting to interpolating for vector components --- \n",
+    "default": "--- Vector field, no option selected, defaulting to interpolating for vector components --- \n",
+}
+
+_MODE_CODE = {"scalar": N.ARB_MODE_SCALAR, "vector": N.ARB_MODE_VECTOR, "norm": N.ARB_MODE_NORM,
+              "both": N.ARB_MODE_BOTH, "invalid": N.ARB_MODE_VECTOR_NAN,
+              "default": N.ARB_MODE_VECTOR_NAN}
+
+_AX = "xyzt"
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("arbinterp_amd needs a ROCm GPU (gfx950); none is visible")
+    return torch
+
+
+def _stream_handle(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class _Interp:
+    """Shared implementation of the 3D and 4D classes."""
+
+    NDIM = 3
+    __version__ = "1.8"
+
+    def __init__(self, field, *args, **kwargs):
+        d = self.NDIM
+        self.eps = 10 * np.finfo(float).eps            # ARBInterp.py:11
+        precision = kwargs.pop("precision", "fp64")
+        quiet = ("quiet" in args) or bool(kwargs.pop("quiet", False))
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        torch = _torch()
+        is_tensor = isinstance(field, torch.Tensor)
+        shape = tuple(field.shape)
+        if len(shape) != 2:
+            raise ValueError("field must be a 2-D array of rows")
+        ncols = shape[1]
+        # variant dispatch (ARBInterp.py:24-104, :643-723)
+        if ncols == d + 1:
+            kind = "scalar"
+        elif ncols == d + 3:
+            if "mode" in kwargs:
+                m = kwargs["mode"]
+                kind = m if m in ("vector", "norm", "both") else "invalid"
+            else:
+                kind = "default"
+        else:
+            raise SystemExit("--- Input not shaped as expected - should be N x %d or N x %d ---"
+                             % (d + 1, d + 3))
+        if not quiet:
+            print(_BANNERS[kind])
+        self._kind = kind
+        self._variant = {"scalar": 2, "vector": 1, "norm": 2, "both": 3}.get(kind, 1)
+        self._field_src = field
+        if is_tensor:
+            rows = field.detach().to(device="cuda", dtype=torch.float64).contiguous()
+        else:
+            rows = torch.from_numpy(np.ascontiguousarray(field, dtype=np.float64)).to("cuda")
+        L = N.lib()
+        h = ctypes.c_void_p()
+        prec = N.ARB_PREC_FP64 if precision == "fp64" else N.ARB_PREC_MIXED
+        rc = L.arb_field_create(ctypes.c_void_p(rows.data_ptr()), rows.shape[0], ncols,
+                                rows.shape[1], d, _MODE_CODE[kind], prec,
+                                _stream_handle(torch), ctypes.byref(h))
+        N.check(rc, "field")
+        self._h = h
+        info = N.FieldInfo()
+        N.check(L.arb_field_info_get(h, ctypes.byref(info)))
+        self._info = info
+        self.nc = int(info.nc)
+        n = [int(info.n[a]) for a in range(d)]
+        self._n = n
+        self.nPos = np.array([n[a] - 3 for a in range(d)])         # ARBInterp.py:545
+        for a in range(d):
+            setattr(self, "h" + _AX[a], np.float64(info.h[a]))
+            setattr(self, _AX[a] + "IntMin", np.float64(info.int_min[a]))
+            setattr(self, _AX[a] + "IntMax", np.float64(info.int_max[a]))
+        if d == 4:                                                  # ARBInterp.py:1288-1291
+            self.nPosx, self.nPosy, self.nPosz, self.nPost = n
+        self._lo = np.array([info.int_min[a] for a in range(d)])
+        self._hi = np.array([info.int_max[a] for a in range(d)])
+        self._hh = np.array([info.h[a] for a in range(d)])
+        self.queryInds = None
+
+    # -- reference attributes -------------------------------------------------
+    @property
+    def A(self):
+        """A = inv(B).D (ARBInterp.py:175, :878), built exactly on the host."""
+        K = 4 ** self.NDIM
+        out = np.zeros((K, K))
+        N.check(N.lib().arb_operator_matrix(self.NDIM, 3, out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    @property
+    def basePointInds(self):
+        """Flat node ids of each cell's base node (ARBInterp.py:558-565)."""
+        d = self.NDIM
+        stride = [int(np.prod(self._n[:a])) for a in range(d)]
+        base = np.zeros(1, dtype=np.int64) + sum(stride)
+        for a in range(d):
+            base = (base[None, :] + stride[a] * np.arange(self._n[a] - 3)[:, None]).reshape(-1)
+        return np.sort(base)
+
+    @property
+    def inputfield(self):
+        """The field rows in the reference's sorted order (ARBInterp.py:530-532)."""
+        f = self._field_src
+        if not isinstance(f, np.ndarray):
+            f = f.detach().cpu().numpy()
+        keys = tuple(f[:, a] for a in range(self.NDIM))   # lexsort: last key is primary
+        return f[np.lexsort(keys)]
+
+    @property
+    def alphamask(self):
+        m = np.zeros((self.nc + 1, 1))
+        m[-1] = 1
+        if self._info.table_built:
+            m[:] = 1
+        return m
+
+    def allCoeffs(self):
+        """Build the whole coefficient table on the GPU (ARBInterp.py:523-525)."""
+        torch = _torch()
+        N.check(N.lib().arb_build_coeffs(self._h, _stream_handle(torch)), "allCoeffs")
+        torch.cuda.current_stream().synchronize()
+        self._info.table_built = 1
+
+    def timings(self):
+        """Stage durations (ms) of the last query made with ARB_Q_TIMING:
+        [bucketing or whole direct kernel, scan, scatter, evaluation]."""
+        ms = (ctypes.c_float * 4)()
+        N.check(N.lib().arb_last_timings(self._h, ms))
+        return [float(x) for x in ms]
+
+    def set_tuning(self, block_cells=None, threads=0):
+        """Cells per bucket block (one entry per axis, None = default) and
+        threads per evaluation workgroup for the bucketed query path."""
+        arr = None
+        if block_cells is not None:
+            arr = (ctypes.c_int32 * self.NDIM)(*[int(b) for b in block_cells])
+        N.check(N.lib().arb_set_tuning(self._h, arr, int(threads)), "tuning")
+
+    def coefficient_table(self, cell0=0, ncells=None):
+        """Coefficients of cells [cell0, cell0+ncells) as a (ncells, ninterp, K)
+        array, interpolants ordered (x, y, z, n) as present; row nc is the
+        sentinel (the reference's alphax/.../alphan[:, cell], ARBInterp.py:42-74)."""
+        torch = _torch()
+        if ncells is None:
+            ncells = self.nc + 1 - cell0
+        K = 4 ** self.NDIM
+        dt = torch.float64 if self.precision == "fp64" else torch.float32
+        out = torch.empty((ncells, self._info.ninterp, K), dtype=dt, device="cuda")
+        N.check(N.lib().arb_table_copy(self._h, cell0, ncells, ctypes.c_void_p(out.data_ptr()),
+                                       _stream_handle(torch)), "table")
+        self._info.table_built = 1
+        return out.cpu().numpy()
+
+    # -- queries ---------------------------------------------------------------
+    def _alloc_out(self, torch, n, dtype):
+        d = self.NDIM
+        dev = "cuda"
+        comps = norms = grads = None
+        if self._info.has_comps:
+            comps = torch.empty((n, 3), dtype=dtype, device=dev)
+        if self._info.has_norm:
+            norms = torch.empty((n,), dtype=dtype, device=dev)
+            grads = torch.empty((n, d), dtype=dtype, device=dev)
+        return comps, norms, grads
+
+    def _run(self, torch, q_dev, n, ldq, comps, norms, grads, cells, flags):
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        rc = N.lib().arb_query(self._h, ptr(q_dev), n, ldq, ptr(comps), ptr(norms), ptr(grads),
+                               ptr(cells), flags, _stream_handle(torch))
+        self._info.table_built = 1
+        return rc
+
+    def query_device(self, q, nan_rows=False, out_f64=False, cells=False, flags=0, out=None):
+        """Evaluate positions held in GPU memory (torch tensor n x >= d, float64,
+        row-contiguous).  Returns (comps, norms, grads[, cells]) as torch tensors
+        (None where the mode has no such output), enqueued on the current
+        stream.  nan_rows=True writes NaN over out-of-range rows of `q` (F4).
+        `out` = a tuple returned by an earlier call of the same size reuses
+        those buffers (no allocation)."""
+        torch = _torch()
+        if q.dtype != torch.float64 or not q.is_cuda or q.stride(1) != 1:
+            raise ValueError("query_device needs a CUDA float64 tensor with contiguous rows")
+        n = q.shape[0]
+        dt = torch.float64 if (self.precision == "fp64" or out_f64) else torch.float32
+        if out is not None:
+            comps, norms, grads = out[:3]
+            cel = out[3] if (cells and len(out) > 3) else None
+        else:
+            comps, norms, grads = self._alloc_out(torch, n, dt)
+            cel = torch.empty((n,), dtype=torch.int64, device="cuda") if cells else None
+        fl = flags | (N.ARB_Q_NAN_ROWS if nan_rows else 0)
+        if out_f64:
+            fl |= N.ARB_Q_OUT_F64
+        rc = self._run(torch, q, n, q.stride(0), comps, norms, grads, cel, fl)
+        N.check(rc, "query")
+        return (comps, norms, grads, cel) if cells else (comps, norms, grads)
+
+    def rQuery(self, query):
+        """Batch query (rQuery1/2/3, ARBInterp.py:344-521 / :1064-1258).
+
+        Rows with a coordinate strictly outside [IntMin, IntMax] are set to NaN
+        in the caller's array (all columns); their outputs are NaN.  Sets
+        `queryInds`.  Raises IndexError when a row reduces past the sentinel
+        cell (the reference's alphamask lookup does)."""
+        d = self.NDIM
+        torch = _torch()
+        # F4: whole caller rows with a coordinate strictly outside the box
+        oob = np.zeros(len(query), dtype=bool)
+        for a in range(d):
+            col = query[:, a]
+            oob |= (col < self._lo[a]) | (col > self._hi[a])
+        if oob.any():
+            query[np.where(oob)[0]] = np.nan
+        n = len(query)
+        qh = np.ascontiguousarray(query[:, :d], dtype=np.float64)
+        q_dev = torch.from_numpy(qh).to("cuda", non_blocking=False)
+        out_dt = torch.float64
+        comps, norms, grads = self._alloc_out(torch, n, out_dt)
+        cells = torch.empty((n,), dtype=torch.int64, device="cuda")
+        flags = N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
+        rc = self._run(torch, q_dev, n, d, comps, norms, grads, cells, flags)
+        self.queryInds = cells.cpu().numpy()
+        if rc == N.ARB_EINDEX:
+            raise IndexError("index %d is out of bounds for axis 0 with size %d"
+                             % (int(self.queryInds.max()), self.nc + 1))
+        N.check(rc, "query")
+        out_c = comps.cpu().numpy() if comps is not None else None
+        out_n = norms.cpu().numpy().reshape(n, 1) if norms is not None else None
+        out_g = grads.cpu().numpy() if grads is not None else None
+        if self._variant == 1:
+            return out_c
+        if self._variant == 2:
+            return out_n, out_g
+        return out_c, out_n, out_g
+
+    def sQuery(self, query):
+        """Single-point query (sQuery1/2/3, ARBInterp.py:213-342 / :916-1062).
+        Out of the box -> bare NaN; a NaN coordinate or a cell index past the
+        sentinel raises IndexError; a 2-D argument makes the bounds test raise
+        ValueError, as in the reference."""
+        d = self.NDIM
+        for a in range(d):
+            if query[a] < self._lo[a] or query[a] > self._hi[a]:
+                return np.nan
+        pt = np.array([[float(query[a]) for a in range(d)]], dtype=np.float64)
+        if np.isnan(pt).any():
+            raise IndexError("index -9223372036854775808 is out of bounds for axis 0 with size %d"
+                             % (self.nc + 1))
+        torch = _torch()
+        q_dev = torch.from_numpy(pt).to("cuda")
+        comps, norms, grads = self._alloc_out(torch, 1, torch.float64)
+        cells = torch.empty((1,), dtype=torch.int64, device="cuda")
+        flags = N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
+        rc = self._run(torch, q_dev, 1, d, comps, norms, grads, cells, flags)
+        self.queryInd = int(cells.cpu().numpy()[0])
+        if rc == N.ARB_EINDEX:
+            raise IndexError("index %d is out of bounds for axis 0 with size %d"
+                             % (self.queryInd, self.nc + 1))
+        N.check(rc, "query")
+        if self._variant == 1:
+            return comps.cpu().numpy()[0]
+        nv = np.float64(norms.cpu().numpy()[0])
+        gv = grads.cpu().numpy()[0]
+        if self._variant == 2:
+            return nv, gv
+        return comps.cpu().numpy()[0], nv, gv
+
+    def Query(self, query):
+        """Query1/2/3 (ARBInterp.py:177-211 / :880-914): 2-D input with more
+        than one column -> batch, else single point; an IndexError in the batch
+        path is retried as a single-point query."""
+        try:
+            if query.shape[1] > 1:
+                return self._unpack(self.rQuery(query))
+            return self._unpack(self.sQuery(query))
+        except IndexError:
+            return self._unpack(self.sQuery(query))
+
+    def _unpack(self, r):
+        """Query2/Query3 unpack the result into 2 / 3 names, so a bare NaN
+        from an out-of-box single point raises TypeError (ARBInterp.py:195,210)."""
+        v = self._variant if hasattr(self, "_variant") else self.variant
+        if v == 1:
+            return r
+        if v == 2:
+            norm, grad = r
+            return norm, grad
+        comps, norm, grad = r
+        return comps, norm, grad
+
+    def close(self):
+        if getattr(self, "_h", None):
+            N.lib().arb_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class tricubic(_Interp):
+    """Tricubic interpolation of a 3D field (ARBInterp.py:8-621)."""
+    NDIM = 3
+
+
+class quadcubic(_Interp):
+    """Quadcubic interpolation of a 4D field (ARBInterp.py:627-1381)."""
+    NDIM = 4

@@ -1,0 +1,605 @@
+// Host runtime and C ABI of libarbinterp.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "arb_internal.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                 \
+    do {                                                                          \
+        hipError_t e_ = (x);                                                      \
+        if (e_ != hipSuccess)                                                     \
+            return fail(e_ == hipErrorOutOfMemory ? ARB_ENOMEM : ARB_EHIP,         \
+                        "%s failed: %s", #x, hipGetErrorString(e_));              \
+    } while (0)
+
+// Device buffer freed on scope exit unless released.
+struct DevBuf {
+    void* p = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+    void* release() { void* r = p; p = nullptr; return r; }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---------------------------------------------------------------------------
+// Operator matrices, built exactly on the host (makeAMatrix,
+// ARBInterp.py:107-175 / :726-878).
+//
+// B: rows (block, corner) -> monomial columns; inv(B) is the tensor product of
+// the 1D cubic Hermite map (p0, p1, p0', p1') -> (a0, a1, a2, a3), so its
+// entries are integers (|.| <= 27 in 3D, 81 in 4D) and are formed exactly.
+// D: tensor products of the central difference (-1/2, 0, 1/2) in cuboid units;
+// in 4D the d4/dxdydzdt block carries the reference's row shift
+// (ARBInterp.py:860: rows 241..255 take the stencils of corners 0..14, row 240
+// stays zero).  A = inv(B).D is a sum of integer x dyadic products, exact in
+// FP64, equal to np.matmul(np.linalg.inv(B), D).
+// ---------------------------------------------------------------------------
+namespace {
+
+// b-vector blocks: subsets of axes in order of size, lexicographic.
+std::vector<int> deriv_blocks(int d) {
+    std::vector<int> out;
+    for (int order = 0; order <= d; ++order)
+        for (int m = 0; m < (1 << d); ++m)
+            if (__builtin_popcount(m) == order) out.push_back(m);
+    // within one order the reference lists x<y<z<t combinations
+    // lexicographically: (x,y),(x,z),(x,t),(y,z),(y,t),(z,t); sort by the
+    // tuple of axis ids.
+    auto key = [&](int m) {
+        std::vector<int> ax;
+        for (int a = 0; a < d; ++a) if (m >> a & 1) ax.push_back(a);
+        return ax;
+    };
+    std::stable_sort(out.begin(), out.end(), [&](int a, int b) {
+        int pa = __builtin_popcount(a), pb = __builtin_popcount(b);
+        if (pa != pb) return pa < pb;
+        return key(a) < key(b);
+    });
+    return out;
+}
+
+void make_B(int d, std::vector<double>& B) {
+    const int K = 1 << (2 * d), nco = 1 << d;
+    B.assign((size_t)K * K, 0.0);
+    auto blocks = deriv_blocks(d);
+    for (int i = 0; i < K; ++i)
+        for (size_t b = 0; b < blocks.size(); ++b)
+            for (int k = 0; k < nco; ++k) {
+                double v = 1.0;
+                for (int a = 0; a < d; ++a) {
+                    int e = (i >> (2 * a)) & 3;
+                    double x = (k >> a) & 1;
+                    if (blocks[b] >> a & 1) v *= (e == 0) ? 0.0 : e * std::pow(x, e - 1);
+                    else v *= std::pow(x, e);
+                }
+                B[(size_t)(b * nco + k) * K + i] = v;
+            }
+}
+
+// 1D Hermite: coefficient e from (p0, p1, d0, d1)
+const double kH1[4][4] = {{1, 0, 0, 0}, {0, 0, 1, 0}, {-3, 3, -2, -1}, {2, -2, 1, 1}};
+
+void make_invB(int d, std::vector<double>& Bi) {
+    const int K = 1 << (2 * d), nco = 1 << d;
+    Bi.assign((size_t)K * K, 0.0);
+    auto blocks = deriv_blocks(d);
+    for (int i = 0; i < K; ++i)
+        for (size_t b = 0; b < blocks.size(); ++b)
+            for (int k = 0; k < nco; ++k) {
+                double v = 1.0;
+                for (int a = 0; a < d; ++a) {
+                    int e = (i >> (2 * a)) & 3;
+                    int col = ((blocks[b] >> a & 1) ? 2 : 0) + ((k >> a) & 1);
+                    v *= kH1[e][col];
+                }
+                Bi[(size_t)i * K + (b * nco + k)] = v;
+            }
+}
+
+void make_D(int d, std::vector<double>& D) {
+    const int K = 1 << (2 * d), nco = 1 << d;
+    D.assign((size_t)K * K, 0.0);
+    auto blocks = deriv_blocks(d);
+    std::vector<int> C(nco);
+    for (int k = 0; k < nco; ++k) {
+        int c = 0, p = 1;
+        for (int a = 0; a < d; ++a, p *= 4) c += (1 + ((k >> a) & 1)) * p;
+        C[k] = c;
+    }
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        std::vector<std::pair<int, double>> taps{{0, 1.0}};
+        for (int a = 0; a < d; ++a) {
+            if (!(blocks[b] >> a & 1)) continue;
+            std::vector<std::pair<int, double>> nt;
+            int st = 1 << (2 * a);
+            for (auto& t : taps)
+                for (int s = -1; s <= 1; s += 2) nt.push_back({t.first + s * st, t.second * 0.5 * s});
+            taps.swap(nt);
+        }
+        int r0 = (int)b * nco, nrow = nco, cshift = 0;
+        if (d == 4 && __builtin_popcount(blocks[b]) == 4) { r0 += 1; nrow -= 1; cshift = 0; }
+        for (int r = 0; r < nrow; ++r)
+            for (auto& t : taps) D[(size_t)(r0 + r) * K + (C[r + cshift] + t.first)] = t.second;
+    }
+}
+
+void matmul(int K, const std::vector<double>& X, const std::vector<double>& Y,
+            std::vector<double>& Z) {
+    Z.assign((size_t)K * K, 0.0);
+    for (int i = 0; i < K; ++i)
+        for (int r = 0; r < K; ++r) {
+            double x = X[(size_t)i * K + r];
+            if (x == 0.0) continue;
+            for (int j = 0; j < K; ++j) Z[(size_t)i * K + j] += x * Y[(size_t)r * K + j];
+        }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// field handle
+// ---------------------------------------------------------------------------
+struct arb_field {
+    ArbGeom g{};
+    int32_t mode = 0, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
+    int32_t table_built = 0;
+    int device = 0;
+    double* vals = nullptr;     // [ninterp][nnodes] FP64, canonical node order
+    void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
+    int64_t table_bytes = 0;
+    double* A = nullptr;        // K x K FP64
+    unsigned short* Ab = nullptr;  // inv(B) as BF16 in MFMA operand order (mixed)
+    int32_t* flag = nullptr;    // device status word
+    // bucketed query path
+    ArbBuckets B{};
+    int32_t user_block[ARB_MAXD] = {0, 0, 0, 0};
+    int32_t eval_threads = 256;
+    int32_t* ws_bkt = nullptr;  // per query: bucket (-1 = handled in place)
+    int32_t* ws_rank = nullptr; // per query: rank inside its bucket
+    int32_t* ws_perm = nullptr; // bucket-sorted query indices
+    int64_t ws_n = 0;
+    int32_t* ws_count = nullptr;
+    int32_t* ws_offs = nullptr;
+    int64_t ws_nb = 0;
+    hipEvent_t ev[5] = {};
+    int32_t ev_ready = 0;
+    float last_ms[4] = {0, 0, 0, 0};
+};
+
+// Choose the cell block of one bucket: the largest of a fixed ladder whose
+// coefficients (padded by 16 bytes per cell) fit in 48 KiB of LDS, unless the
+// caller set one.
+static void choose_buckets(arb_field* f) {
+    const int d = f->g.ndim;
+    const int K = 1 << (2 * d);
+    const int esz = f->precision == ARB_PREC_FP64 ? 8 : 4;
+    ArbBuckets& B = f->B;
+    B.cellvec = f->ninterp * K * esz / 16;
+    B.cellstride = B.cellvec + 1;
+    static const int ladder3[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
+    static const int ladder4[][4] = {{4, 4, 2, 2}, {4, 2, 2, 2}, {2, 2, 2, 2}, {2, 2, 2, 1},
+                                     {2, 2, 1, 1}, {2, 1, 1, 1}, {1, 1, 1, 1}};
+    const int max_cells = (48 * 1024) / (B.cellstride * 16);
+    int dims[4] = {1, 1, 1, 1};
+    if (f->user_block[0] > 0) {
+        for (int a = 0; a < d; ++a) dims[a] = f->user_block[a];
+    } else if (d == 3) {
+        for (auto& c : ladder3)
+            if (c[0] * c[1] * c[2] <= max_cells) { dims[0] = c[0]; dims[1] = c[1]; dims[2] = c[2]; break; }
+    } else {
+        for (auto& c : ladder4)
+            if (c[0] * c[1] * c[2] * c[3] <= max_cells) {
+                for (int a = 0; a < 4; ++a) dims[a] = c[a];
+                break;
+            }
+    }
+    B.ncell = 1;
+    B.nbt = 1;
+    for (int a = 0; a < ARB_MAXD; ++a) {
+        B.b[a] = a < d ? dims[a] : 1;
+        B.nb[a] = a < d ? (f->g.npos[a] + B.b[a] - 1) / B.b[a] : 1;
+        B.ncell *= B.b[a];
+        B.nbt *= B.nb[a];
+    }
+}
+
+static int ensure_workspace(arb_field* f, int64_t n) {
+    if (n > f->ws_n) {
+        if (f->ws_bkt) (void)hipFree(f->ws_bkt);
+        if (f->ws_rank) (void)hipFree(f->ws_rank);
+        if (f->ws_perm) (void)hipFree(f->ws_perm);
+        f->ws_bkt = f->ws_rank = f->ws_perm = nullptr;
+        f->ws_n = 0;
+        HIPCHK(hipMalloc(&f->ws_bkt, sizeof(int32_t) * n));
+        HIPCHK(hipMalloc(&f->ws_rank, sizeof(int32_t) * n));
+        HIPCHK(hipMalloc(&f->ws_perm, sizeof(int32_t) * n));
+        f->ws_n = n;
+    }
+    if (f->B.nbt + 1 > f->ws_nb) {
+        if (f->ws_count) (void)hipFree(f->ws_count);
+        if (f->ws_offs) (void)hipFree(f->ws_offs);
+        f->ws_count = f->ws_offs = nullptr;
+        f->ws_nb = 0;
+        HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * (f->B.nbt + 1)));
+        HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (f->B.nbt + 1)));
+        f->ws_nb = f->B.nbt + 1;
+    }
+    return ARB_OK;
+}
+
+static int interp_count(int mode, int* comps, int* norm) {
+    switch (mode) {
+        case ARB_MODE_SCALAR: *comps = 0; *norm = 1; return 1;
+        case ARB_MODE_VECTOR:
+        case ARB_MODE_VECTOR_NAN: *comps = 1; *norm = 0; return 3;
+        case ARB_MODE_NORM: *comps = 0; *norm = 1; return 1;
+        case ARB_MODE_BOTH: *comps = 1; *norm = 1; return 4;
+        default: return 0;
+    }
+}
+
+extern "C" {
+
+const char* arb_last_error(void) { return g_err.c_str(); }
+
+const char* arb_version(void) { return "arbinterp-mi355x 0.1 (gfx950)"; }
+
+int arb_operator_matrix(int32_t ndim, int32_t which, double* out) {
+    if ((ndim != 3 && ndim != 4) || !out || which < 0 || which > 3)
+        return fail(ARB_EINVAL, "arb_operator_matrix: bad argument");
+    const int K = 1 << (2 * ndim);
+    std::vector<double> M, Bi, D;
+    if (which == 0) make_B(ndim, M);
+    else if (which == 1) make_invB(ndim, M);
+    else if (which == 2) make_D(ndim, M);
+    else { make_invB(ndim, Bi); make_D(ndim, D); matmul(K, Bi, D, M); }
+    std::memcpy(out, M.data(), sizeof(double) * (size_t)K * K);
+    return ARB_OK;
+}
+
+int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64_t ld,
+                     int32_t ndim, int32_t mode, int32_t precision, void* stream,
+                     arb_field** out) {
+    if (!out) return fail(ARB_EINVAL, "arb_field_create: out is NULL");
+    *out = nullptr;
+    if (ndim != 3 && ndim != 4) return fail(ARB_EINVAL, "ndim must be 3 or 4 (got %d)", ndim);
+    if (ncols != ndim + 1 && ncols != ndim + 3)
+        return fail(ARB_ESHAPE, "--- Input not shaped as expected - should be N x %d or N x %d ---",
+                    ndim + 1, ndim + 3);
+    if (!rows_dev || nrows <= 0 || ld < ncols)
+        return fail(ARB_EINVAL, "arb_field_create: bad rows/nrows/ld");
+    if (precision != ARB_PREC_FP64 && precision != ARB_PREC_MIXED)
+        return fail(ARB_EINVAL, "unknown precision %d", precision);
+    if (ncols == ndim + 1) mode = ARB_MODE_SCALAR;
+    else if (mode == ARB_MODE_SCALAR || mode < 0 || mode > ARB_MODE_VECTOR_NAN)
+        return fail(ARB_EINVAL, "vector field needs mode vector/norm/both (got %d)", mode);
+    hipStream_t s = (hipStream_t)stream;
+
+    arb_field* f = new (std::nothrow) arb_field();
+    if (!f) return fail(ARB_ENOMEM, "host allocation failed");
+    struct Guard { arb_field* f; ~Guard() { if (f) arb_field_destroy(f); } } guard{f};
+    HIPCHK(hipGetDevice(&f->device));
+    f->mode = mode;
+    f->precision = precision;
+    f->ninterp = interp_count(mode, &f->has_comps, &f->has_norm);
+    HIPCHK(hipMalloc(&f->flag, sizeof(int32_t) * 4));
+    HIPCHK(hipMemsetAsync(f->flag, 0, sizeof(int32_t) * 4, s));
+
+    // 1. coordinates must be finite
+    HIPCHK(arb_launch_finite_check(rows_dev, nrows, ncols, ld, ndim, f->flag, s));
+    int32_t hflag = 0;
+    HIPCHK(hipMemcpyAsync(&hflag, f->flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hflag) return fail(ARB_EGRID, "field has a non-finite coordinate");
+
+    // 2. unique values per axis
+    int64_t cap = 1024;
+    while (cap < 2 * nrows) cap <<= 1;
+    DevBuf tab, uniq, cnt;
+    HIPCHK(tab.alloc(sizeof(unsigned long long) * cap));
+    HIPCHK(uniq.alloc(sizeof(double) * cap));
+    HIPCHK(cnt.alloc(sizeof(unsigned int)));
+    std::vector<std::vector<double>> axes(ndim);
+    for (int a = 0; a < ndim; ++a) {
+        HIPCHK(hipMemsetAsync(tab.p, 0xFF, sizeof(unsigned long long) * cap, s));
+        HIPCHK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned int), s));
+        HIPCHK(arb_launch_hash_insert(rows_dev, nrows, ld, a, tab.as<unsigned long long>(), cap, s));
+        HIPCHK(arb_launch_hash_compact(tab.as<unsigned long long>(), cap, uniq.as<double>(),
+                                       cnt.as<unsigned int>(), s));
+        unsigned int nu = 0;
+        HIPCHK(hipMemcpyAsync(&nu, cnt.p, sizeof nu, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        axes[a].resize(nu);
+        HIPCHK(hipMemcpyAsync(axes[a].data(), uniq.p, sizeof(double) * nu, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::sort(axes[a].begin(), axes[a].end());
+    }
+    ArbGeom& g = f->g;
+    g.ndim = ndim;
+    int64_t prod = 1;
+    for (int a = 0; a < ndim; ++a) {
+        g.n[a] = (int64_t)axes[a].size();
+        if (g.n[a] < 4)
+            return fail(ARB_EGRID, "axis %d has %lld distinct coordinates; at least 4 are needed",
+                        a, (long long)g.n[a]);
+        if (prod > (int64_t)1 << 40) return fail(ARB_EGRID, "grid too large");
+        prod *= g.n[a];
+    }
+    if (prod != nrows)
+        return fail(ARB_EGRID, "rows do not form a complete grid: %lld rows, axis lengths give %lld",
+                    (long long)nrows, (long long)prod);
+    int64_t st = 1;
+    g.nc = 1;
+    for (int a = 0; a < ndim; ++a) {
+        g.nstride[a] = st;
+        st *= g.n[a];
+        g.npos[a] = g.n[a] - 3;
+        g.nc *= g.npos[a];
+        g.h[a] = std::fabs(axes[a][0] - axes[a][1]);
+        g.lo[a] = axes[a][1];
+        g.hi[a] = axes[a][g.n[a] - 2];
+    }
+    g.nnodes = prod;
+
+    // 3. rank rows, scatter values into canonical order, check completeness
+    DevBuf axes_dev[ARB_MAXD], seen;
+    const double* axp[ARB_MAXD] = {nullptr, nullptr, nullptr, nullptr};
+    for (int a = 0; a < ndim; ++a) {
+        HIPCHK(axes_dev[a].alloc(sizeof(double) * g.n[a]));
+        HIPCHK(hipMemcpyAsync(axes_dev[a].p, axes[a].data(), sizeof(double) * g.n[a],
+                              hipMemcpyHostToDevice, s));
+        axp[a] = axes_dev[a].as<double>();
+    }
+    HIPCHK(seen.alloc(sizeof(int32_t) * g.nnodes));
+    HIPCHK(hipMemsetAsync(seen.p, 0, sizeof(int32_t) * g.nnodes, s));
+    HIPCHK(hipMalloc(&f->vals, sizeof(double) * g.nnodes * f->ninterp));
+    HIPCHK(arb_launch_rank_scatter(rows_dev, nrows, ld, axp, g, mode, ndim, f->vals, f->ninterp,
+                                   seen.as<int32_t>(), s));
+    HIPCHK(arb_launch_check_seen(seen.as<int32_t>(), g.nnodes, f->flag, s));
+    HIPCHK(hipMemcpyAsync(&hflag, f->flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (hflag) return fail(ARB_EGRID, "rows do not form a complete grid (duplicate points)");
+
+    // 4. operator matrix and table storage
+    const int K = 1 << (2 * ndim);
+    std::vector<double> Bi, D, A;
+    make_invB(ndim, Bi);
+    make_D(ndim, D);
+    matmul(K, Bi, D, A);
+    HIPCHK(hipMalloc(&f->A, sizeof(double) * K * K));
+    HIPCHK(hipMemcpyAsync(f->A, A.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, s));
+    std::vector<unsigned short> Ab;
+    if (precision == ARB_PREC_MIXED) {
+        // operand order [mt][ks][lane][j]: row mt*16 + (lane & 15), column
+        // ks*32 + 8*(lane >> 4) + j.  Entries are small integers: exact in BF16.
+        const int KS = K / 32, MT = K / 16;
+        Ab.resize((size_t)K * K);
+        for (int mt = 0; mt < MT; ++mt)
+            for (int ks = 0; ks < KS; ++ks)
+                for (int l = 0; l < 64; ++l)
+                    for (int j = 0; j < 8; ++j) {
+                        float v = (float)Bi[(size_t)(mt * 16 + (l & 15)) * K + ks * 32 + 8 * (l >> 4) + j];
+                        uint32_t u;
+                        std::memcpy(&u, &v, 4);
+                        Ab[(((size_t)mt * KS + ks) * 64 + l) * 8 + j] = (unsigned short)(u >> 16);
+                    }
+        HIPCHK(hipMalloc(&f->Ab, sizeof(unsigned short) * Ab.size()));
+        HIPCHK(hipMemcpyAsync(f->Ab, Ab.data(), sizeof(unsigned short) * Ab.size(),
+                              hipMemcpyHostToDevice, s));
+    }
+    const size_t esz = precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+    f->table_bytes = (int64_t)((g.nc + 1) * f->ninterp * K * esz);
+    HIPCHK(hipMalloc(&f->table, (size_t)f->table_bytes));
+    HIPCHK(hipStreamSynchronize(s));
+    guard.f = nullptr;
+    *out = f;
+    return ARB_OK;
+}
+
+int arb_field_info_get(const arb_field* f, arb_field_info* out) {
+    if (!f || !out) return fail(ARB_EINVAL, "arb_field_info_get: NULL argument");
+    std::memset(out, 0, sizeof *out);
+    out->ndim = f->g.ndim;
+    out->mode = f->mode;
+    out->precision = f->precision;
+    out->ninterp = f->ninterp;
+    out->has_comps = f->has_comps;
+    out->has_norm = f->has_norm;
+    out->table_built = f->table_built;
+    for (int a = 0; a < f->g.ndim; ++a) {
+        out->n[a] = f->g.n[a];
+        out->npos[a] = f->g.npos[a];
+        out->h[a] = f->g.h[a];
+        out->int_min[a] = f->g.lo[a];
+        out->int_max[a] = f->g.hi[a];
+    }
+    out->nc = f->g.nc;
+    out->nnodes = f->g.nnodes;
+    out->table_bytes = f->table_bytes;
+    return ARB_OK;
+}
+
+int arb_build_coeffs(arb_field* f, void* stream) {
+    if (!f) return fail(ARB_EINVAL, "arb_build_coeffs: NULL handle");
+    if (f->table_built) return ARB_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const int K = 1 << (2 * f->g.ndim);
+    const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
+    if (f->precision == ARB_PREC_FP64) {
+        HIPCHK(arb_launch_coef_fp64(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
+    } else {
+        HIPCHK(arb_launch_coef_bf16x3(f->g, f->vals, f->ninterp, f->Ab, (float*)f->table, s));
+    }
+    HIPCHK(arb_launch_sentinel(f->table, f->precision != ARB_PREC_FP64, f->g.nc, f->ninterp, K,
+                               sentinel_nan, s));
+    f->table_built = 1;
+    return ARB_OK;
+}
+
+int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
+              void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
+              void* stream) {
+    if (!f) return fail(ARB_EINVAL, "arb_query: NULL handle");
+    if (n < 0 || (n > 0 && (!q_dev || ldq < f->g.ndim)))
+        return fail(ARB_EINVAL, "arb_query: bad positions (n=%lld ldq=%lld)", (long long)n,
+                    (long long)ldq);
+    if (n > 0 && f->has_comps && !comps_dev) return fail(ARB_EINVAL, "arb_query: comps is NULL");
+    if (n > 0 && f->has_norm && (!norms_dev || !grads_dev))
+        return fail(ARB_EINVAL, "arb_query: norms/grads is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = arb_build_coeffs(f, stream);
+    if (rc) return rc;
+    if (n == 0) return ARB_OK;
+    if (flags & ARB_Q_CHECK) HIPCHK(hipMemsetAsync(f->flag + 1, 0, sizeof(int32_t), s));
+    ArbOut o{};
+    o.comps = comps_dev;
+    o.norms = norms_dev;
+    o.grads = grads_dev;
+    o.cells = cells_dev;
+    o.qmut = (flags & ARB_Q_NAN_ROWS) ? q_dev : nullptr;
+    o.ldq = ldq;
+    o.flag = f->flag + 1;
+    const int out64 = (flags & ARB_Q_OUT_F64) ? 1 : 0;
+    const bool timing = (flags & ARB_Q_TIMING) != 0;
+    if (timing && !f->ev_ready) {
+        for (auto& e : f->ev) HIPCHK(hipEventCreate(&e));
+        f->ev_ready = 1;
+    }
+    bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
+    if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
+    if (n > INT32_MAX - 1) sorted = false;
+    for (float& m : f->last_ms) m = 0.f;
+    if (!sorted) {
+        if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
+        HIPCHK(arb_launch_query_direct(f->g, f->precision, out64, f->table, f->ninterp,
+                                       f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
+        if (timing) {
+            HIPCHK(hipEventRecord(f->ev[4], s));
+            HIPCHK(hipEventSynchronize(f->ev[4]));
+            HIPCHK(hipEventElapsedTime(&f->last_ms[3], f->ev[0], f->ev[4]));
+        }
+    } else {
+        if (f->B.nbt == 0) choose_buckets(f);
+        rc = ensure_workspace(f, n);
+        if (rc) return rc;
+        if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
+        HIPCHK(arb_launch_fill_i32(f->ws_count, f->B.nbt, 0, s));
+        HIPCHK(arb_launch_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+                                 f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_bkt,
+                                 f->ws_rank, f->ws_count, s));
+        if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
+        HIPCHK(arb_launch_scan(f->ws_count, f->ws_offs, f->B.nbt, s));
+        if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
+        HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, n, s));
+        if (timing) HIPCHK(hipEventRecord(f->ev[3], s));
+        if (flags & ARB_Q_SORTED_GLOBAL)
+            HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+                                        f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
+                                        f->ws_perm, s));
+        else
+            HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+                                          f->has_comps, f->has_norm, q_dev, ldq, o, f->ws_offs,
+                                          f->ws_perm, f->eval_threads, s));
+        if (timing) {
+            HIPCHK(hipEventRecord(f->ev[4], s));
+            HIPCHK(hipEventSynchronize(f->ev[4]));
+            for (int k = 0; k < 4; ++k)
+                HIPCHK(hipEventElapsedTime(&f->last_ms[k], f->ev[k], f->ev[k + 1]));
+        }
+    }
+    if (flags & ARB_Q_CHECK) {
+        int32_t h = 0;
+        HIPCHK(hipMemcpyAsync(&h, f->flag + 1, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h) return fail(ARB_EINDEX, "a query reduced to a cell index past the sentinel");
+    }
+    return ARB_OK;
+}
+
+int arb_last_timings(const arb_field* f, float* ms4) {
+    if (!f || !ms4) return fail(ARB_EINVAL, "arb_last_timings: NULL argument");
+    for (int k = 0; k < 4; ++k) ms4[k] = f->last_ms[k];
+    return ARB_OK;
+}
+
+int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads) {
+    if (!f) return fail(ARB_EINVAL, "arb_set_tuning: NULL handle");
+    if (threads != 0 && (threads < 64 || threads > 256 || threads % 64))
+        return fail(ARB_EINVAL, "threads must be 64, 128, 192 or 256");
+    const int d = f->g.ndim;
+    if (block_cells) {
+        int64_t cells = 1;
+        for (int a = 0; a < d; ++a) {
+            if (block_cells[a] < 1 || block_cells[a] > 64)
+                return fail(ARB_EINVAL, "block edge must be 1..64 cells");
+            cells *= block_cells[a];
+        }
+        const int K = 1 << (2 * d);
+        const int esz = f->precision == ARB_PREC_FP64 ? 8 : 4;
+        const int64_t lds = cells * (f->ninterp * K * esz / 16 + 1) * 16;
+        if (lds > 64 * 1024) return fail(ARB_EINVAL, "block needs %lld bytes of LDS (max 65536)",
+                                         (long long)lds);
+        for (int a = 0; a < d; ++a) f->user_block[a] = block_cells[a];
+    } else {
+        for (int a = 0; a < ARB_MAXD; ++a) f->user_block[a] = 0;
+    }
+    f->eval_threads = threads ? threads : 256;
+    choose_buckets(f);
+    return ARB_OK;
+}
+
+int arb_table_copy(arb_field* f, int64_t cell0, int64_t ncells, void* dst_dev, void* stream) {
+    if (!f || !dst_dev || cell0 < 0 || ncells < 0 || cell0 + ncells > f->g.nc + 1)
+        return fail(ARB_EINVAL, "arb_table_copy: bad range");
+    int rc = arb_build_coeffs(f, stream);
+    if (rc) return rc;
+    const int K = 1 << (2 * f->g.ndim);
+    const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const size_t row = esz * (size_t)f->ninterp * K;
+    HIPCHK(hipMemcpyAsync(dst_dev, (char*)f->table + row * cell0, row * ncells,
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return ARB_OK;
+}
+
+void arb_field_destroy(arb_field* f) {
+    if (!f) return;
+    if (f->vals) (void)hipFree(f->vals);
+    if (f->table) (void)hipFree(f->table);
+    if (f->A) (void)hipFree(f->A);
+    if (f->Ab) (void)hipFree(f->Ab);
+    if (f->flag) (void)hipFree(f->flag);
+    if (f->ws_bkt) (void)hipFree(f->ws_bkt);
+    if (f->ws_rank) (void)hipFree(f->ws_rank);
+    if (f->ws_perm) (void)hipFree(f->ws_perm);
+    if (f->ws_count) (void)hipFree(f->ws_count);
+    if (f->ws_offs) (void)hipFree(f->ws_offs);
+    if (f->ev_ready)
+        for (auto& e : f->ev) (void)hipEventDestroy(e);
+    delete f;
+}
+
+}  // extern "C"

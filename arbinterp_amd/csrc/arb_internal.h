@@ -1,0 +1,220 @@
+// Internal declarations shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "../../include/arbinterp.h"
+
+#define ARB_MAXD 4
+
+// Grid constants as the kernels see them (by value).
+struct ArbGeom {
+    double lo[ARB_MAXD];      // IntMin per axis
+    double hi[ARB_MAXD];      // IntMax per axis
+    double h[ARB_MAXD];       // spacing |node0 - node1|
+    int64_t n[ARB_MAXD];      // nodes per axis
+    int64_t npos[ARB_MAXD];   // cells per axis = n - 3
+    int64_t nstride[ARB_MAXD];// node strides (x fastest)
+    int64_t nc;               // number of cells
+    int64_t nnodes;
+    int32_t ndim;
+};
+
+// Outputs of one query launch.
+struct ArbOut {
+    void* comps;      // n x 3
+    void* norms;      // n
+    void* grads;      // n x d
+    int64_t* cells;   // n (queryInds), nullable
+    double* qmut;     // positions to NaN-fill for out-of-range rows (nullable)
+    int64_t ldq;      // row stride of the positions (also the fill width)
+    int32_t* flag;    // set to 1 when a row reduced past the sentinel
+};
+
+// Cell blocks used to bucket queries: a bucket is a block of b[0] x b[1] x
+// b[2] [x b[3]] cells; its coefficients are staged in LDS together.
+struct ArbBuckets {
+    int32_t b[ARB_MAXD];     // block edge (cells) per axis
+    int64_t nb[ARB_MAXD];    // blocks per axis
+    int64_t nbt;             // number of buckets
+    int32_t cellvec;         // 16-byte vectors per cell of coefficients
+    int32_t cellstride;      // LDS stride per cell in 16-byte vectors
+    int32_t ncell;           // cells per block
+};
+
+// ---------------------------------------------------------------------------
+// Cell reduction (ARBInterp.py:350-373, 1069-1092): strict bounds test, then
+// iu = (q - IntMin)/h in IEEE FP64, ix = floor(iu), local = iu - ix, flat cell
+// index ix + iy*npx + iz*npx*npy [+ it*npx*npy*npz].  The reference forms the
+// flat index in FP64; every term is an integer below 2^53, so the int64 form
+// is identical.  Rows outside the box (or with a NaN coordinate) map to the
+// sentinel cell nc with NaN local coordinates.
+// Returns the unclamped flat index; *oob is set for rows strictly outside.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ int64_t arb_reduce(const double* __restrict__ q, const ArbGeom& g,
+                                              double* __restrict__ loc, bool* oob) {
+    bool out = false;
+    double x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        x[a] = q[a];
+        out |= (x[a] < g.lo[a]) | (x[a] > g.hi[a]);
+    }
+    *oob = out;
+    int64_t flat = 0, mul = 1;
+    bool bad = out;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double iu = out ? NAN : (x[a] - g.lo[a]) / g.h[a];
+        double fl = floor(iu);
+        loc[a] = iu - fl;
+        bad |= isnan(iu);
+        int64_t ia = bad ? 0 : (int64_t)fl;
+        flat += ia * mul;
+        mul *= g.npos[a];
+    }
+    return bad ? g.nc : flat;
+}
+
+// ---------------------------------------------------------------------------
+// Tensor Horner evaluation of p(u,v,w[,t]) = sum_i a_i u^ex v^ey w^ez [t^et],
+// i = ex + 4ey + 16ez [+ 64et].  Mathematically equal to the reference's
+// sum over the tile/repeat power basis (ARBInterp.py:380-397, 493-521,
+// 1097-1127, 1223-1258); the derivative of the basis along each axis is
+// carried alongside (Horner's rule for p and p').  All arithmetic is explicit
+// fma() so the result does not depend on compiler contraction.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T h4(T c0, T c1, T c2, T c3, T u) {
+    return fma(fma(fma(c3, u, c2), u, c1), u, c0);
+}
+// value and d/du of c0 + c1 u + c2 u^2 + c3 u^3
+template <typename T>
+__device__ __forceinline__ void h4d(T c0, T c1, T c2, T c3, T u, T& p, T& dp) {
+    T b = fma(c3, u, c2);
+    T d = fma(c3, u, b);
+    b = fma(b, u, c1);
+    d = fma(d, u, b);
+    p = fma(b, u, c0);
+    dp = d;
+}
+
+// value only, 3D; a: 64 coefficients
+template <typename T, typename C>
+__device__ __forceinline__ T arb_val3(const C* a, T u, T v, T w) {
+    T q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T p[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const C* c = a + 16 * k + 4 * j;
+            p[j] = h4<T>((T)c[0], (T)c[1], (T)c[2], (T)c[3], u);
+        }
+        q[k] = h4<T>(p[0], p[1], p[2], p[3], v);
+    }
+    return h4<T>(q[0], q[1], q[2], q[3], w);
+}
+
+// value and gradient in cuboid units, 3D
+template <typename T, typename C>
+__device__ __forceinline__ void arb_valgrad3(const C* a, T u, T v, T w, T& val, T* g) {
+    T q[4], qu[4], qv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        T p[4], pu[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const C* c = a + 16 * k + 4 * j;
+            h4d<T>((T)c[0], (T)c[1], (T)c[2], (T)c[3], u, p[j], pu[j]);
+        }
+        h4d<T>(p[0], p[1], p[2], p[3], v, q[k], qv[k]);
+        qu[k] = h4<T>(pu[0], pu[1], pu[2], pu[3], v);
+    }
+    T dw;
+    h4d<T>(q[0], q[1], q[2], q[3], w, val, dw);
+    g[0] = h4<T>(qu[0], qu[1], qu[2], qu[3], w);
+    g[1] = h4<T>(qv[0], qv[1], qv[2], qv[3], w);
+    g[2] = dw;
+}
+
+// 4D: a has 256 coefficients; reduce over x, y, z (3D blocks), then t.
+template <typename T, typename C>
+__device__ __forceinline__ T arb_val4(const C* a, T u, T v, T w, T t) {
+    T r[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) r[l] = arb_val3<T, C>(a + 64 * l, u, v, w);
+    return h4<T>(r[0], r[1], r[2], r[3], t);
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void arb_valgrad4(const C* a, T u, T v, T w, T t, T& val, T* g) {
+    T r[4], ru[4], rv[4], rw[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        T gg[3];
+        arb_valgrad3<T, C>(a + 64 * l, u, v, w, r[l], gg);
+        ru[l] = gg[0];
+        rv[l] = gg[1];
+        rw[l] = gg[2];
+    }
+    T dt;
+    h4d<T>(r[0], r[1], r[2], r[3], t, val, dt);
+    g[0] = h4<T>(ru[0], ru[1], ru[2], ru[3], t);
+    g[1] = h4<T>(rv[0], rv[1], rv[2], rv[3], t);
+    g[2] = h4<T>(rw[0], rw[1], rw[2], rw[3], t);
+    g[3] = dt;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (defined in the .hip kernel files).
+// ---------------------------------------------------------------------------
+struct ArbField;
+
+hipError_t arb_launch_finite_check(const double* rows, int64_t nrows, int32_t ncols, int64_t ld,
+                                   int32_t ndim, int32_t* flag, hipStream_t s);
+hipError_t arb_launch_hash_insert(const double* rows, int64_t nrows, int64_t ld, int32_t axis,
+                                  unsigned long long* tab, int64_t cap, hipStream_t s);
+hipError_t arb_launch_hash_compact(const unsigned long long* tab, int64_t cap, double* out,
+                                   unsigned int* count, hipStream_t s);
+hipError_t arb_launch_rank_scatter(const double* rows, int64_t nrows, int64_t ld,
+                                   const double* const* axes_dev, const ArbGeom& g,
+                                   int32_t mode, int32_t vcol0, double* vals, int32_t ninterp,
+                                   int32_t* seen, hipStream_t s);
+hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* flag, hipStream_t s);
+
+hipError_t arb_launch_coef_fp64(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                const double* A, double* table, hipStream_t s);
+hipError_t arb_launch_coef_mixed(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                 const double* A, float* table, hipStream_t s);
+hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                  const unsigned short* Ab, float* table, hipStream_t s);
+hipError_t arb_launch_sentinel(void* table, int32_t f32, int64_t nc, int32_t ninterp, int32_t K,
+                               int32_t nan, hipStream_t s);
+
+hipError_t arb_launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
+                             const void* table, int32_t ninterp, int32_t has_comps,
+                             int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                             const ArbOut& o, int32_t* bkt, int32_t* rank, int32_t* count,
+                             hipStream_t s);
+hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipStream_t s);
+hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
+                              int32_t* perm, int64_t n, hipStream_t s);
+hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                  int32_t out_f64, const void* table, int32_t ninterp,
+                                  int32_t has_comps, int32_t has_norm, const double* q,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
+hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                int32_t out_f64, const void* table, int32_t ninterp,
+                                int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                const int32_t* perm, hipStream_t s);
+hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f64,
+                                   const void* table, int32_t ninterp, int32_t has_comps,
+                                   int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                                   const ArbOut& o, hipStream_t s);

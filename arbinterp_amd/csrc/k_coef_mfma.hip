@@ -1,0 +1,192 @@
+// Coefficient table build on the matrix cores (gfx950).
+//
+// Mixed precision (ARB_PREC_MIXED): for every cell the b-vector of
+// finite-difference data (f, fx, fy, fz, fxy, fxz, fyz, fxyz at the 8 corners;
+// 16 blocks x 16 corners in 4D, ARBInterp.py:118-125 / :738-757) is formed in
+// FP64 from an LDS tile of node values, then alpha = inv(B) . b is one dense
+// contraction per tile of 16 cells:
+//     (K x K) x (K x 16),  K = 64 (3D) or 256 (4D)
+// on v_mfma_f32_16x16x32_bf16.  inv(B) is integer (|.| <= 81) and exact in
+// BF16; b is split as b = hi + mid + lo (three BF16 parts, 24 significant
+// bits), so alpha = inv(B).lo + inv(B).mid + inv(B).hi with FP32 accumulation,
+// smallest part first.  Before the split the cell's base value f(corner 0) is
+// subtracted from the 8 (16) corner values: inv(B) maps the constant data
+// (1 at every corner, 0 derivatives) to e_0, so alpha_0 gets it back exactly
+// and the GEMM only carries differences, which keeps the FP32 accumulation
+// error proportional to the field's variation across the stencil.
+//
+// Work split: one workgroup = 4 waves = a 16(x) x 4(y) tile of cells at one
+// z (and t); wave w owns cell row y0+w, lane l the cell x0 + (l & 15).  The
+// node tile (19 x 7 x 4 [x 4]) is loaded once per interpolant into LDS with
+// row-contiguous reads.
+#include "arb_internal.h"
+
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+constexpr int TXC = 16, TYC = 4;            // cells per tile
+constexpr int TX = TXC + 3, TY = TYC + 3;   // nodes per tile
+
+// b-vector block -> axis bitmask (bit a = derivative along axis a), in the
+// reference's block order (ARBInterp.py:118-125, :738-757).
+__constant__ unsigned char kBlockMask3[8] = {0, 1, 2, 4, 3, 5, 6, 7};
+__constant__ unsigned char kBlockMask4[16] = {0, 1, 2, 4, 8, 3, 5, 9, 6, 10, 12, 7, 11, 13, 14, 15};
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+    unsigned int u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;     // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ double bf16_to_double(unsigned short b) {
+    return (double)__uint_as_float(((unsigned int)b) << 16);
+}
+
+template <int D>
+struct TileGeom {
+    static constexpr int TZ = 4;
+    static constexpr int TT = (D == 4) ? 4 : 1;
+    static constexpr int NODES = TX * TY * TZ * TT;
+};
+
+// Finite-difference entry e of the b-vector (block e / nco, corner e % nco)
+// for the cell whose 4^D neighbourhood starts at tile offset (lx, ly, 0, 0).
+template <int D>
+__device__ __forceinline__ double fd_entry(const double* __restrict__ tile, int lx, int ly, int e) {
+    constexpr int nco = 1 << D;
+    const int blk = e / nco;
+    int corner = e % nco;
+    const int bm = (D == 3) ? kBlockMask3[blk] : kBlockMask4[blk];
+    if (D == 4 && blk == 15) {
+        // ARBInterp.py:860: row 240 is zero, row 240+k uses corner k-1
+        if (corner == 0) return 0.0;
+        corner -= 1;
+    }
+    const int nb = __builtin_popcount(bm);
+    const double scale = (nb == 0) ? 1.0 : (nb == 1) ? 0.5 : (nb == 2) ? 0.25 : (nb == 3) ? 0.125 : 0.0625;
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < (1 << D); ++t) {
+        if (t & ~bm) continue;
+        int o[4] = {0, 0, 0, 0};
+        int neg = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int c = 1 + ((corner >> a) & 1);
+            if (bm >> a & 1) {
+                const int s = (t >> a & 1) ? 1 : -1;
+                neg += (s < 0);
+                o[a] = c + s;
+            } else {
+                o[a] = c;
+            }
+        }
+        const int idx = (((o[3] * TileGeom<D>::TZ + o[2]) * TY + (ly + o[1])) * TX + (lx + o[0]));
+        const double v = tile[idx];
+        acc += (neg & 1) ? -v : v;
+    }
+    return acc * scale;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
+              const unsigned short* __restrict__ Ab, float* __restrict__ table) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    constexpr int KS = K / 32;      // MFMA k-steps
+    constexpr int MT = K / 16;      // MFMA row tiles
+    constexpr int nco = 1 << D;
+    __shared__ double tile[TileGeom<D>::NODES];
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int grp = lane >> 4;      // 16-lane group: k-slice of the operands
+    const int lx = lane & 15;
+    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
+    const int64_t cy0 = (int64_t)blockIdx.y * TYC;
+    int64_t cz = blockIdx.z, ct = 0;
+    if (D == 4) { ct = cz / g.npos[2]; cz = cz % g.npos[2]; }
+    const int64_t cx = cx0 + lx, cy = cy0 + w;
+    const bool valid = (cx < g.npos[0]) && (cy < g.npos[1]);
+    const int64_t cell = cx + g.npos[0] * (cy + g.npos[1] * (cz + (D == 4 ? g.npos[2] * ct : 0)));
+
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();
+        for (int i = threadIdx.x; i < TileGeom<D>::NODES; i += 256) {
+            const int x = i % TX;
+            const int y = (i / TX) % TY;
+            const int z = (i / (TX * TY)) % TileGeom<D>::TZ;
+            const int t = i / (TX * TY * TileGeom<D>::TZ);
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z, nt = ct + t;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1])
+                v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz + (D == 4 ? g.nstride[3] * nt : 0)];
+            tile[i] = v;
+        }
+        __syncthreads();
+
+        // b-vector slice of this lane: entries ks*32 + 8*grp + j, j = 0..7
+        const int lxc = valid ? lx : 0;
+        const int lyc = w;
+        const double f0 = fd_entry<D>(tile, lxc, lyc, 0);
+        us8 bp[3][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int e = ks * 32 + 8 * grp + j;
+                double b = fd_entry<D>(tile, lxc, lyc, e);
+                if (e < nco) b -= f0;                   // base value carried by alpha_0
+                const unsigned short hi = bf16_rne((float)b);
+                const double r1 = b - bf16_to_double(hi);
+                const unsigned short mid = bf16_rne((float)r1);
+                const double r2 = r1 - bf16_to_double(mid);
+                const unsigned short lo = bf16_rne((float)r2);
+                bp[0][ks][j] = lo;
+                bp[1][ks][j] = mid;
+                bp[2][ks][j] = hi;
+            }
+        }
+        float* out = table + (cell * ninterp + k) * K;
+#pragma unroll 1
+        for (int mt = 0; mt < MT; ++mt) {
+            f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int p = 0; p < 3; ++p) {
+#pragma unroll
+                for (int ks = 0; ks < KS; ++ks) {
+                    const us8 a = *reinterpret_cast<const us8*>(Ab + (((mt * KS + ks) * 64 + lane) * 8));
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                                  __builtin_bit_cast(bf8, bp[p][ks]),
+                                                                  acc, 0, 0, 0);
+                }
+            }
+            // lane holds alpha[mt*16 + 4*grp + r] of cell lx
+            if (mt == 0 && grp == 0) acc[0] = (float)((double)acc[0] + f0);
+            if (valid) *reinterpret_cast<f4*>(out + mt * 16 + 4 * grp) = acc;
+        }
+    }
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+// Ab: inv(B) as BF16 in operand order [mt][ks][lane][8] (host-prepared).
+hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                  const unsigned short* Ab, float* table, hipStream_t s) {
+    if (g.nc == 0) return hipSuccess;
+    dim3 block(256);
+    if (g.ndim == 3) {
+        dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]);
+        hipLaunchKernelGGL(k_coef_bf16x3<3>, grid, block, 0, s, g, vals, ninterp, Ab, table);
+    } else {
+        dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)(g.npos[2] * g.npos[3]));
+        hipLaunchKernelGGL(k_coef_bf16x3<4>, grid, block, 0, s, g, vals, ninterp, Ab, table);
+    }
+    return hipGetLastError();
+}

@@ -1,0 +1,161 @@
+// Grid detection: per-axis unique coordinates, rank of every row, scatter of
+// the value columns to canonical x-fastest node order.
+//
+// Replaces getFieldParams (ARBInterp.py:528-568, 4D :1264-1320), which sorts
+// the rows lexicographically with three (four) argsorts and counts the axis
+// lengths along the first node's axis lines.  For a complete grid the sorted
+// row order is unique, so ranking each coordinate in its axis' sorted unique
+// values gives the same canonical order without a full-row sort.
+//
+// Unique values are found with an open-addressing hash set per axis (64-bit
+// atomicCAS on the coordinate's bit pattern, -0.0 folded into +0.0 as the
+// reference's `==` does); the unique sets are small and are sorted on the host.
+#include "arb_internal.h"
+
+namespace {
+
+constexpr unsigned long long kEmpty = 0xFFFFFFFFFFFFFFFFull;  // a NaN pattern
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+__global__ void k_finite_check(const double* __restrict__ rows, int64_t nrows, int32_t ncols,
+                               int64_t ld, int32_t ndim, int32_t* flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    bool bad = false;
+    for (int a = 0; a < ndim; ++a) bad |= !isfinite(rows[i * ld + a]);
+    if (bad) flag[0] = 1;
+}
+
+__global__ void k_hash_insert(const double* __restrict__ rows, int64_t nrows, int64_t ld,
+                              int32_t axis, unsigned long long* tab, int64_t cap) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    double x = rows[i * ld + axis] + 0.0;   // -0.0 -> +0.0
+    unsigned long long key = (unsigned long long)__double_as_longlong(x);
+    unsigned long long mask = (unsigned long long)cap - 1;
+    unsigned long long slot = mix64(key) & mask;
+    // The table holds at most nrows distinct keys in cap >= 2*nrows slots, so
+    // the probe always reaches an empty slot or the key itself.
+    for (int64_t probe = 0; probe < cap; ++probe) {
+        unsigned long long old = atomicCAS(&tab[slot], kEmpty, key);
+        if (old == kEmpty || old == key) return;
+        slot = (slot + 1) & mask;
+    }
+}
+
+__global__ void k_hash_compact(const unsigned long long* __restrict__ tab, int64_t cap,
+                               double* out, unsigned int* count) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    unsigned long long k = tab[i];
+    if (k != kEmpty) {
+        unsigned int p = atomicAdd(count, 1u);
+        out[p] = __longlong_as_double((long long)k);
+    }
+}
+
+// Index of x in the ascending array ax[0..n-1] (exact match expected).
+__device__ __forceinline__ int64_t rank_of(const double* __restrict__ ax, int64_t n, double x) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (ax[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+struct AxisPtrs { const double* p[ARB_MAXD]; };
+
+// mode: ARB_MODE_*; vcol0 = first value column (= ndim).
+// Interpolant order in `vals`: scalar -> [f]; vector -> [Bx,By,Bz];
+// norm -> [|B|]; both -> [Bx,By,Bz,|B|].
+__global__ void k_rank_scatter(const double* __restrict__ rows, int64_t nrows, int64_t ld,
+                               AxisPtrs axes, ArbGeom g, int32_t mode, int32_t vcol0,
+                               double* __restrict__ vals, int32_t* __restrict__ seen) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const double* r = rows + i * ld;
+    int64_t node = 0;
+    for (int a = 0; a < g.ndim; ++a) {
+        int64_t k = rank_of(axes.p[a], g.n[a], r[a] + 0.0);
+        node += k * g.nstride[a];
+    }
+    atomicAdd(&seen[node], 1);
+    const int64_t nn = g.nnodes;
+    if (mode == ARB_MODE_SCALAR) {
+        vals[node] = r[vcol0];
+        return;
+    }
+    double bx = r[vcol0], by = r[vcol0 + 1], bz = r[vcol0 + 2];
+    // |B| as numpy.linalg.norm(axis=1) forms it: sqrt((x*x + y*y) + z*z);
+    // compiled without FP contraction, so each product is rounded.
+    double bn = sqrt((bx * bx + by * by) + bz * bz);
+    if (mode == ARB_MODE_NORM) {
+        vals[node] = bn;
+        return;
+    }
+    vals[node] = bx;
+    vals[nn + node] = by;
+    vals[2 * nn + node] = bz;
+    if (mode == ARB_MODE_BOTH) vals[3 * nn + node] = bn;
+}
+
+__global__ void k_check_seen(const int32_t* __restrict__ seen, int64_t nnodes, int32_t* flag) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    if (seen[i] != 1) flag[0] = 1;
+}
+
+inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+hipError_t arb_launch_finite_check(const double* rows, int64_t nrows, int32_t ncols, int64_t ld,
+                                   int32_t ndim, int32_t* flag, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finite_check, dim3(grid1(nrows, 256)), dim3(256), 0, s, rows, nrows,
+                       ncols, ld, ndim, flag);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_hash_insert(const double* rows, int64_t nrows, int64_t ld, int32_t axis,
+                                  unsigned long long* tab, int64_t cap, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hash_insert, dim3(grid1(nrows, 256)), dim3(256), 0, s, rows, nrows, ld,
+                       axis, tab, cap);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_hash_compact(const unsigned long long* tab, int64_t cap, double* out,
+                                   unsigned int* count, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash_compact, dim3(grid1(cap, 256)), dim3(256), 0, s, tab, cap, out,
+                       count);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_rank_scatter(const double* rows, int64_t nrows, int64_t ld,
+                                   const double* const* axes_dev, const ArbGeom& g,
+                                   int32_t mode, int32_t vcol0, double* vals, int32_t ninterp,
+                                   int32_t* seen, hipStream_t s) {
+    (void)ninterp;
+    AxisPtrs ap{};
+    for (int a = 0; a < g.ndim; ++a) ap.p[a] = axes_dev[a];
+    hipLaunchKernelGGL(k_rank_scatter, dim3(grid1(nrows, 256)), dim3(256), 0, s, rows, nrows, ld,
+                       ap, g, mode, vcol0, vals, seen);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* flag,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_check_seen, dim3(grid1(nnodes, 256)), dim3(256), 0, s, seen, nnodes,
+                       flag);
+    return hipGetLastError();
+}

@@ -1,0 +1,355 @@
+// Query evaluation kernels.
+//
+// Replaces rQuery1/2/3 (ARBInterp.py:344-521; 4D :1064-1258): bounds mask,
+// cell reduction, coefficient gather and the power-basis sums for values and
+// gradients.  The reference builds ~40 N x 64 temporaries per call; here each
+// query is one lane.
+#include "arb_internal.h"
+
+namespace {
+
+template <typename TO, typename T>
+__device__ __forceinline__ void put(void* p, int64_t idx, T v) {
+    reinterpret_cast<TO*>(p)[idx] = (TO)v;
+}
+
+// Evaluate the interpolants of one point from its cell's coefficients `a`
+// ([interp][K], in global memory or LDS) and write row i of the outputs.
+template <int D, typename T, typename TC, typename TO>
+__device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32_t has_norm,
+                                           const double* loc, const ArbGeom& g, const ArbOut& o,
+                                           int64_t i) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    T x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = (T)loc[d];
+    int k = 0;
+    if (has_comps) {
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {
+            T v;
+            if constexpr (D == 3) v = arb_val3<T, TC>(a + c * K, x[0], x[1], x[2]);
+            else v = arb_val4<T, TC>(a + c * K, x[0], x[1], x[2], x[3]);
+            put<TO>(o.comps, i * 3 + c, v);
+        }
+        k = 3;
+    }
+    if (has_norm) {
+        T v, gr[D];
+        if constexpr (D == 3) arb_valgrad3<T, TC>(a + k * K, x[0], x[1], x[2], v, gr);
+        else arb_valgrad4<T, TC>(a + k * K, x[0], x[1], x[2], x[3], v, gr);
+        put<TO>(o.norms, i, v);
+#pragma unroll
+        for (int d = 0; d < D; ++d) put<TO>(o.grads, i * D + d, gr[d] / (T)g.h[d]);
+    }
+}
+
+// Cell reduction of row i with the side outputs every path writes: queryInds,
+// the past-the-sentinel flag and the NaN row fill.  Returns the cell used for
+// the coefficients (clamped to the sentinel nc).
+template <int D>
+__device__ __forceinline__ int64_t reduce_row(const double* __restrict__ q, int64_t ldq,
+                                              const ArbGeom& g, const ArbOut& o, int64_t i,
+                                              double* loc) {
+    bool oob;
+    int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+    if (o.cells) o.cells[i] = flat;
+    if (flat > g.nc) {
+        o.flag[0] = 1;
+        flat = g.nc;
+    }
+    if (oob && o.qmut) {
+        for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
+    }
+    return flat;
+}
+
+// Lane-per-query kernel: coefficients read straight from the table.
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_query_direct(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+               int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[D];
+    const int64_t flat = reduce_row<D>(q, ldq, g, o, i, loc);
+    eval_point<D, T, TC, TO>(table + flat * (int64_t)ninterp * K, has_comps, has_norm, loc, g, o, i);
+}
+
+// ---------------------------------------------------------------------------
+// Cell-bucketed path.  Queries are grouped by the block of cells they fall in
+// (counting sort: atomic rank per bucket, one scan, one scatter of the query
+// index); one workgroup then stages its block's coefficients in LDS with
+// contiguous 16-byte loads -- each cell's coefficients leave HBM once per
+// call -- and evaluates the bucket's queries from LDS.  Output rows are
+// written at the caller's row index, so no unsort pass is needed.  Results do
+// not depend on the order inside a bucket (each row is computed alone), so
+// the atomic ranks leave the outputs bitwise deterministic.
+// ---------------------------------------------------------------------------
+
+// Bucket of an in-range flat cell index and the cell's index inside its block.
+template <int D>
+__device__ __forceinline__ int64_t bucket_of(int64_t flat, const ArbGeom& g, const ArbBuckets& B,
+                                             int* lc) {
+    int64_t rem = flat, bidx = 0, bmul = 1;
+    int l = 0, lmul = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const int64_t c = rem % g.npos[a];
+        rem /= g.npos[a];
+        const int64_t bi = c / B.b[a];
+        const int li = (int)(c - bi * B.b[a]);
+        bidx += bi * bmul;
+        bmul *= B.nb[a];
+        l += li * lmul;
+        lmul *= B.b[a];
+    }
+    *lc = l;
+    return bidx;
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Reduction + bucket rank.  Rows that reduce to the sentinel (out of range,
+// NaN, or past the sentinel) are evaluated here from the sentinel row.
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+         int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
+         int32_t* __restrict__ bkt, int32_t* __restrict__ rank, int32_t* __restrict__ count) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[D];
+    const int64_t flat = reduce_row<D>(q, ldq, g, o, i, loc);
+    if (flat >= g.nc) {
+        eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
+                                 o, i);
+        bkt[i] = -1;
+        return;
+    }
+    int lc;
+    const int64_t b = bucket_of<D>(flat, g, B, &lc);
+    bkt[i] = (int32_t)b;
+    rank[i] = atomicAdd(&count[b], 1);
+}
+
+// Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024.
+__global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count,
+                                               int32_t* __restrict__ offs, int64_t nb) {
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int64_t chunk = (nb + 1023) / 1024;
+    const int64_t s0 = t * chunk;
+    const int64_t s1 = (s0 + chunk < nb) ? s0 + chunk : nb;
+    int64_t sum = 0;
+    for (int64_t j = s0; j < s1; ++j) sum += count[j];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = part[t] - sum;        // exclusive prefix of this chunk
+    for (int64_t j = s0; j < s1; ++j) {
+        offs[j] = (int32_t)run;
+        run += count[j];
+    }
+    if (t == 1023) offs[nb] = (int32_t)part[1023];
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
+                          const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t b = bkt[i];
+    if (b >= 0) perm[offs[b] + rank[i]] = (int32_t)i;
+}
+
+// One workgroup per bucket: stage the block's coefficients in LDS, then
+// evaluate the bucket's queries from LDS.
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has_comps,
+              int32_t has_norm, const double* __restrict__ q, int64_t ldq, ArbOut o,
+              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
+    extern __shared__ uint4 smem[];
+    const int64_t b = blockIdx.x;
+    const int32_t start = offs[b], end = offs[b + 1];
+    if (start == end) return;                       // uniform over the workgroup
+    int64_t c0[D], ext[D], cstr[D];
+    {
+        int64_t rem = b, st = 1;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int64_t bi = rem % B.nb[a];
+            rem /= B.nb[a];
+            c0[a] = bi * B.b[a];
+            const int64_t left = g.npos[a] - c0[a];
+            ext[a] = left < B.b[a] ? left : B.b[a];
+            cstr[a] = st;
+            st *= g.npos[a];
+        }
+    }
+    const uint4* __restrict__ tv = reinterpret_cast<const uint4*>(table);
+    const int total = B.ncell * B.cellvec;
+    for (int v = threadIdx.x; v < total; v += blockDim.x) {
+        const int lc = v / B.cellvec;
+        const int j = v - lc * B.cellvec;
+        int rem = lc;
+        bool inside = true;
+        int64_t gcell = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            const int la = rem % B.b[a];
+            rem /= B.b[a];
+            inside &= la < ext[a];
+            gcell += (c0[a] + la) * cstr[a];
+        }
+        if (inside) smem[lc * B.cellstride + j] = tv[gcell * B.cellvec + j];
+    }
+    __syncthreads();
+    for (int32_t s = start + threadIdx.x; s < end; s += blockDim.x) {
+        const int64_t i = perm[s];
+        double loc[D];
+        bool oob;
+        const int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+        int lc;
+        const int64_t bb = bucket_of<D>(flat, g, B, &lc);
+        if (bb != b || flat >= g.nc) continue;      // cannot happen: same reduction as k_bucket
+        const TC* a = reinterpret_cast<const TC*>(smem + lc * B.cellstride);
+        eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, i);
+    }
+}
+
+// Sorted order, coefficients read from the table in global memory.
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_eval_perm(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+            int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
+            const int32_t* __restrict__ total, const int32_t* __restrict__ perm) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || s >= total[0]) return;
+    const int64_t i = perm[s];
+    double loc[D];
+    bool oob;
+    const int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+    if (flat >= g.nc) return;
+    eval_point<D, T, TC, TO>(table + flat * (int64_t)ninterp * K, has_comps, has_norm, loc, g, o, i);
+}
+
+inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Dispatch over (ndim, precision, output type).
+#define ARB_DISPATCH(D, PREC, OUTF64, LAUNCH)                                   \
+    do {                                                                        \
+        if (PREC == ARB_PREC_FP64) { LAUNCH(D, double, double, double); }       \
+        else if (OUTF64) { LAUNCH(D, float, float, double); }                   \
+        else { LAUNCH(D, float, float, float); }                                \
+    } while (0)
+
+template <int D>
+hipError_t launch_direct(const ArbGeom& g, int32_t prec, int32_t out_f64, const void* table,
+                         int32_t ninterp, int32_t has_comps, int32_t has_norm, const double* q,
+                         int64_t n, int64_t ldq, const ArbOut& o, hipStream_t s) {
+    dim3 grid(grid1(n, 256)), block(256);
+    if (prec == ARB_PREC_FP64) {
+        hipLaunchKernelGGL((k_query_direct<D, double, double, double>), grid, block, 0, s, g,
+                           (const double*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    } else if (out_f64) {
+        hipLaunchKernelGGL((k_query_direct<D, float, float, double>), grid, block, 0, s, g,
+                           (const float*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    } else {
+        hipLaunchKernelGGL((k_query_direct<D, float, float, float>), grid, block, 0, s, g,
+                           (const float*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f64,
+                                   const void* table, int32_t ninterp, int32_t has_comps,
+                                   int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                                   const ArbOut& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (g.ndim == 3)
+        return launch_direct<3>(g, prec, out_f64, table, ninterp, has_comps, has_norm, q, n, ldq,
+                                o, s);
+    return launch_direct<4>(g, prec, out_f64, table, ninterp, has_comps, has_norm, q, n, ldq, o,
+                            s);
+}
+
+hipError_t arb_launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid1(n, 256)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
+                             const void* table, int32_t ninterp, int32_t has_comps,
+                             int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                             const ArbOut& o, int32_t* bkt, int32_t* rank, int32_t* count,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define L_BUCKET(DD, T, TC, TO)                                                              \
+    hipLaunchKernelGGL((k_bucket<DD, T, TC, TO>), dim3(grid1(n, 256)), dim3(256), 0, s, g, B,   \
+                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, bkt, rank, \
+                       count)
+    if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_BUCKET);
+    else ARB_DISPATCH(4, prec, out_f64, L_BUCKET);
+#undef L_BUCKET
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, count, offs, nb);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
+                              int32_t* perm, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs, perm, n);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                  int32_t out_f64, const void* table, int32_t ninterp,
+                                  int32_t has_comps, int32_t has_norm, const double* q,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
+    (void)ninterp;
+    const size_t lds = (size_t)B.ncell * B.cellstride * 16;
+    dim3 grid((unsigned)B.nbt), block(block_threads);
+#define L_EVALB(DD, T, TC, TO)                                                              \
+    hipLaunchKernelGGL((k_eval_bucket<DD, T, TC, TO>), grid, block, lds, s, g, B,             \
+                       (const TC*)table, has_comps, has_norm, q, ldq, o, offs, perm)
+    if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
+    else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
+#undef L_EVALB
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                int32_t out_f64, const void* table, int32_t ninterp,
+                                int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                const int32_t* perm, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int32_t* total = offs + B.nbt;
+#define L_EVALP(DD, T, TC, TO)                                                              \
+    hipLaunchKernelGGL((k_eval_perm<DD, T, TC, TO>), dim3(grid1(n, 256)), dim3(256), 0, s, g,  \
+                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, total, perm)
+    if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALP);
+    else ARB_DISPATCH(4, prec, out_f64, L_EVALP);
+#undef L_EVALP
+    return hipGetLastError();
+}

@@ -1,0 +1,166 @@
+/*
+ * arbinterp.h -- C ABI of the MI355X-native Lekien-Marsden tricubic (3D) /
+ * quadcubic (4D) interpolation library (libarbinterp.so, gfx950).
+ *
+ * The reference (DurhamDecLab/ARBInterp v1.8, src/ARBInterp/ARBInterp.py) is a
+ * pure-Python class API with no FFI; each entry point below replaces one part
+ * of that API, cited as ARBInterp.py:line.  The Python drop-in
+ * (arbinterp_amd/ARBInterp.py) binds these through ctypes.
+ *
+ * Conventions
+ *   - Every pointer argument named *_dev is HIP device memory; `stream` is a
+ *     hipStream_t (NULL = the default stream).  Calls are stream-ordered; a
+ *     call returns once its work is enqueued unless stated otherwise.
+ *   - Return value: ARB_OK (0) or a negative ARB_E* code; arb_last_error()
+ *     gives a thread-local message for the last failing call.
+ *   - A handle is not thread-safe (neither is the reference object: Query
+ *     mutates queryInds and the coefficient cache, ARBInterp.py:368-377).
+ *   - All layouts are row-major.  Node/cell order is x fastest, then y, z (t):
+ *     node = ix + nx*(iy + ny*(iz [+ nz*it]))  (ARBInterp.py:530-532,1266-1269);
+ *     cell = cx + npx*(cy + npy*(cz [+ npz*ct])), npx = nx-3 (ARBInterp.py:368,1088).
+ *   - Coefficient index i = ex + 4ey + 16ez [+ 64et] for u^ex v^ey w^ez [tau^et]
+ *     (ARBInterp.py:111,730).
+ */
+#ifndef ARBINTERP_H
+#define ARBINTERP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define ARB_OK      0
+#define ARB_EINVAL (-1)   /* bad argument                                       */
+#define ARB_ESHAPE (-2)   /* ncols is not d+1 or d+3: the reference sys.exit()s
+                             (ARBInterp.py:103-104, :722-723)                     */
+#define ARB_ENOMEM (-3)   /* device allocation failed                            */
+#define ARB_EHIP   (-4)   /* a HIP runtime call failed                           */
+#define ARB_EGRID  (-5)   /* rows do not form a complete grid with >= 4 nodes per
+                             axis (the reference does not check, :528-568)       */
+#define ARB_EINDEX (-6)   /* a query reduced to a flat cell index past the sentinel
+                             column nc; the reference raises IndexError from
+                             alphamask[queryInds] (ARBInterp.py:376,431,489)     */
+
+/* ---- interpolation modes (ARBInterp.py:24-104, :643-723) --------------- */
+#define ARB_MODE_SCALAR  0  /* N x (d+1) field: one interpolant, value + gradient;
+                               sentinel column 0 (ARBInterp.py:24-32)           */
+#define ARB_MODE_VECTOR  1  /* N x (d+3), mode='vector': 3 components; sentinel
+                               column nc is 0, nc+1 is NaN (ARBInterp.py:35-48)  */
+#define ARB_MODE_NORM    2  /* mode='norm': |B| value + gradient; sentinel NaN
+                               (ARBInterp.py:49-58)                              */
+#define ARB_MODE_BOTH    3  /* mode='both': components, |B|, grad |B|; sentinel
+                               NaN (ARBInterp.py:59-74)                          */
+#define ARB_MODE_VECTOR_NAN 4 /* no mode / unknown mode: components, sentinel
+                               NaN (ARBInterp.py:75-102)                         */
+
+/* ---- precision --------------------------------------------------------- */
+#define ARB_PREC_FP64   0   /* plain: FP64 everywhere; A = inv(B).D GEMM on
+                               FP64 MFMA; FP64 table; FP64 evaluation            */
+#define ARB_PREC_MIXED  1   /* FP64 cell reduction and FD stencils; inv(B).b as a
+                               split BF16x3 MFMA GEMM with FP32 accumulation;
+                               FP32 table; FP32 evaluation                       */
+
+/* ---- query flags ------------------------------------------------------- */
+#define ARB_Q_NAN_ROWS  1   /* write NaN over every column of rows with a
+                               coordinate outside [IntMin, IntMax] (F4 of the
+                               reference: ARBInterp.py:350-355,1069-1076)      */
+#define ARB_Q_CHECK     2   /* synchronise the stream and return ARB_EINDEX if any
+                               row reduced past the sentinel                     */
+#define ARB_Q_OUT_F64   4   /* mixed precision: write FP64 outputs (default FP32) */
+#define ARB_Q_DIRECT    8   /* force the unsorted lane-per-query kernel          */
+#define ARB_Q_SORTED   16   /* force the cell-bucketed path (LDS-staged blocks)  */
+#define ARB_Q_SORTED_GLOBAL 32 /* bucketed order, coefficients read from HBM/L2  */
+#define ARB_Q_TIMING   64   /* record per-stage HIP events; the call then waits
+                               for its own work (see arb_last_timings)           */
+
+typedef struct arb_field arb_field;
+
+typedef struct arb_field_info {
+    int32_t ndim;          /* 3 or 4                                            */
+    int32_t mode;          /* ARB_MODE_*                                        */
+    int32_t precision;     /* ARB_PREC_*                                        */
+    int32_t ninterp;       /* interpolants in the table: 1, 3 or 4              */
+    int32_t has_comps;     /* query writes comps (N x 3)                        */
+    int32_t has_norm;      /* query writes norms (N) and grads (N x d)          */
+    int32_t table_built;   /* 1 once arb_build_coeffs has run                   */
+    int32_t coord_f32;     /* reserved, 0                                       */
+    int64_t n[4];          /* nodes per axis (nPos+3)                           */
+    int64_t npos[4];       /* cells per axis (nPos, ARBInterp.py:545)           */
+    double  h[4];          /* |node0 - node1| per axis (ARBInterp.py:547-549)  */
+    double  int_min[4];    /* 2nd node per axis (ARBInterp.py:551-553)          */
+    double  int_max[4];    /* 2nd-last node per axis (ARBInterp.py:554-556)    */
+    int64_t nc;            /* number of cells (ARBInterp.py:568)                */
+    int64_t nnodes;
+    int64_t table_bytes;   /* bytes of the coefficient table                    */
+} arb_field_info;
+
+/* Analyse a field and upload it (tricubic/quadcubic constructor minus the
+ * coefficient fill: ARBInterp.py:10-104 -> getFieldParams :528-568,
+ * makeAMatrix :107-175; 4D :629-723, :1264-1320, :726-878).
+ * rows_dev: nrows x ncols FP64, row stride ld (>= ncols) elements, any row
+ * order.  ndim 3|4; mode ARB_MODE_*; ncols must be ndim+1 (then mode is
+ * forced to ARB_MODE_SCALAR, as the reference ignores it) or ndim+3.
+ * Synchronises `stream` (the grid constants are needed on the host). */
+int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64_t ld,
+                     int32_t ndim, int32_t mode, int32_t precision,
+                     void* stream, arb_field** out);
+
+/* Grid constants (the reference's public attributes nPos, hx.., xIntMin..,
+ * nc: ARBInterp.py:545-568). */
+int arb_field_info_get(const arb_field* f, arb_field_info* out);
+
+/* Fill the whole coefficient table (allCoeffs, ARBInterp.py:523-525,
+ * :1260-1262, with calcCoefficients :570-603 / :1322-1355).  Idempotent. */
+int arb_build_coeffs(arb_field* f, void* stream);
+
+/* Batch query (Query/rQuery, ARBInterp.py:177-211,344-521 / :880-914,1064-1258).
+ * q_dev: n x ldq FP64 positions (first d columns used).  Outputs (device,
+ * nullable when the mode has no such output): comps n x 3, norms n, grads
+ * n x d; FP64 for ARB_PREC_FP64, FP32 for ARB_PREC_MIXED unless ARB_Q_OUT_F64.
+ * cells_dev (nullable): int64 flat cell index per row, the reference's
+ * queryInds (NaN -> nc).  Builds the table first if needed. */
+int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
+              void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
+              int32_t flags, void* stream);
+
+/* Stage durations (ms, HIP events on the call's stream) of the last arb_query
+ * made with ARB_Q_TIMING: [0] reduction + bucketing (whole kernel on the
+ * direct path), [1] bucket scan, [2] scatter, [3] evaluation.  No reference
+ * counterpart (the reference has no timers); used by the benchmark. */
+int arb_last_timings(const arb_field* f, float* ms4);
+
+/* Tuning of the bucketed query path (used by the autotuner): cells per block
+ * along each axis (ndim entries, NULL = library default) and threads per
+ * evaluation workgroup (64..256, 0 = default).  No reference counterpart. */
+int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads);
+
+/* Copy cells [cell0, cell0+ncells) of the coefficient table as stored
+ * ([cell][interp][K]; FP64 for ARB_PREC_FP64, FP32 for ARB_PREC_MIXED) to
+ * dst_dev.  Cell nc is the sentinel column.  The reference keeps the same
+ * numbers as alphax/alphay/alphaz/alphan[:, cell] (ARBInterp.py:42-74).
+ * Builds the table first if needed. */
+int arb_table_copy(arb_field* f, int64_t cell0, int64_t ncells, void* dst_dev, void* stream);
+
+/* Host-only: write the K x K operator matrix of makeAMatrix
+ * (ARBInterp.py:107-175 / :726-878) into out (row-major, K = 4^ndim).
+ * which: 0 = B (monomials and derivatives at the corners, = the reference's
+ * examples/B_Matrix_{3D,4D}.csv), 1 = inv(B) (exact integers), 2 = D (central
+ * differences in cuboid units, with the 4D row shift of ARBInterp.py:860),
+ * 3 = A = inv(B).D.  Needs no GPU. */
+int arb_operator_matrix(int32_t ndim, int32_t which, double* out);
+
+/* Release the handle and its device memory. */
+void arb_field_destroy(arb_field* f);
+
+/* Thread-local message for the last failing call on this thread. */
+const char* arb_last_error(void);
+
+/* Library version string, e.g. "arbinterp-mi355x 0.1 (gfx950)". */
+const char* arb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ARBINTERP_H */

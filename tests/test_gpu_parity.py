@@ -1,0 +1,239 @@
+"""GPU parity: the drop-in (libarbinterp.so kernels) against the reference's
+golden vectors and the CPU oracle.  Bar for the FP64 plain path: identical cell
+indices, NaN patterns, mutated query arrays and exceptions; values and
+gradients within 1e-12 under oracle.rel_err (|d| / max(|ref|, 1e-4 S))."""
+import numpy as np
+import pytest
+
+from arbinterp_amd import workloads as W
+from oracle import arbinterp_oracle as O
+from tests.parity import compare_outputs, run_query
+
+pytestmark = pytest.mark.gpu
+TOL64 = 1e-12
+TOLMIX = 1e-6      # normwise per query (oracle.rel_err_normwise)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    from arbinterp_amd import ARBInterp as G
+    return G
+
+
+def test_scalar3d_gaussians_golden(dev, golden):
+    g = golden("scalar3d_gaussians")
+    t = dev.tricubic(g["field"].copy(), "quiet")
+    o = O.tricubic(g["field"].copy(), "quiet")
+    res, qa = run_query(t, g["query"])
+    compare_outputs(res, qa, g, "", o, g["query"], tol=TOL64)
+    assert np.array_equal(t.queryInds, g["queryInds"])
+
+
+@pytest.mark.parametrize("mode", ["vector", "norm", "both", "bogus", "default"])
+def test_dipole3d_modes_golden(dev, golden, mode):
+    g = golden("dipole3d_modes")
+    kw = {} if mode == "default" else {"mode": mode}
+    t = dev.tricubic(g["field"].copy(), "quiet", **kw)
+    o = O.tricubic(g["field"].copy(), "quiet", **kw)
+    res, qa = run_query(t, g["query"])
+    compare_outputs(res, qa, g, mode + "_", o, g["query"], tol=TOL64)
+    assert np.array_equal(t.queryInds, g[mode + "_queryInds"])
+
+
+def test_moving4d_scalar_golden(dev, golden):
+    g = golden("moving4d_scalar")
+    t = dev.quadcubic(g["field"].copy(), "quiet")
+    o = O.quadcubic(g["field"].copy(), "quiet")
+    res, qa = run_query(t, g["query"])
+    compare_outputs(res, qa, g, "", o, g["query"], tol=TOL64)
+    assert np.array_equal(t.queryInds, g["queryInds"])
+
+
+@pytest.mark.parametrize("mode", ["vector", "norm", "both", "default"])
+def test_vector4d_modes_golden(dev, golden, mode):
+    g = golden("vector4d_modes")
+    kw = {} if mode == "default" else {"mode": mode}
+    t = dev.quadcubic(g["field"].copy(), "quiet", **kw)
+    o = O.quadcubic(g["field"].copy(), "quiet", **kw)
+    res, qa = run_query(t, g["query"])
+    compare_outputs(res, qa, g, mode + "_", o, g["query"], tol=TOL64)
+    assert np.array_equal(t.queryInds, g[mode + "_queryInds"])
+
+
+def test_coefficient_tables_golden(dev, golden):
+    g = golden("coeff_tables")
+    t = dev.tricubic(g["field3"].copy(), "quiet", mode="both")
+    tab = t.coefficient_table()                 # (nc+1, 4, 64)
+    o = O.tricubic(g["field3"].copy(), "quiet", mode="both")
+    assert np.array_equal(t.basePointInds, g["t3_basePointInds"])
+    c = g["t3_consts"]
+    assert [t.hx, t.hy, t.hz, t.xIntMin, t.yIntMin, t.zIntMin, t.xIntMax, t.yIntMax,
+            t.zIntMax, t.nc] == list(c)
+    off = O.neighbour_offsets(o.p.n, 3)
+    for k, name in enumerate(("alphax", "alphay", "alphaz", "alphan")):
+        ref = g["t3_" + name]                    # (64, nc+1)
+        got = tab[:, k, :].T
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        vals = o.interps["xyzn"[k]]
+        scale = np.abs(O.make_A(3)) @ np.abs(vals[o.p.base[None, :] + off[:, None]])
+        assert (np.abs(got[:, :-1] - ref[:, :-1]) <= 1e-15 * scale).all(), name
+    t4 = dev.quadcubic(g["field4"].copy(), "quiet")
+    o4 = O.quadcubic(g["field4"].copy(), "quiet")
+    tab4 = t4.coefficient_table()
+    ref = g["t4_alphan"]
+    got = tab4[:, 0, :].T
+    off4 = O.neighbour_offsets(o4.p.n, 4)
+    scale = np.abs(O.make_A(4)) @ np.abs(o4.interps["n"][o4.p.base[None, :] + off4[:, None]])
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert (np.abs(got[:, :-1] - ref[:, :-1]) <= 1e-15 * scale).all()
+
+
+def test_edge_battery_golden(dev, golden):
+    g = golden("edge_battery")
+    rows, _ = W.scalar3d_gaussians(seed=1)
+    vrows, _ = W.dipole3d(seed=2, n=(14, 12, 10), nq=10)
+    for i in range(int(g["ncases"])):
+        pre = "c%03d_" % i
+        fld = rows if bool(g[pre + "scalar"]) else vrows
+        mode = str(g[pre + "mode"])
+        kw = {} if mode == "none" else {"mode": mode}
+        o = O.tricubic(fld.copy(), "quiet", **kw)
+        t = dev.tricubic(fld.copy(), "quiet", **kw)
+        res, qa = run_query(t, g[pre + "batch"])
+        compare_outputs(res, qa, g, pre + "b_", o, g[pre + "batch"], tol=TOL64, label=pre + mode)
+        p = g[pre + "batch"][0]
+        t2 = dev.tricubic(fld.copy(), "quiet", **kw)
+        res1, qa1 = run_query(t2, p)
+        compare_outputs(res1, qa1, g, pre + "s_", o, p[None, :], tol=TOL64, label=pre + mode)
+
+
+@pytest.mark.parametrize("tag", ["f64f32", "f32f64"])
+def test_dtype_promotion_golden(dev, golden, tag):
+    g = golden("dtype_promotion")
+    t = dev.tricubic(g[tag + "_field"].copy(), "quiet")
+    o = O.tricubic(g[tag + "_field"].copy(), "quiet")
+    res, qa = run_query(t, g[tag + "_query"])
+    compare_outputs(res, qa, g, tag + "_", o, g[tag + "_query"], tol=TOL64)
+    assert np.array_equal(t.queryInds, g[tag + "_queryInds"])
+
+
+def test_mixed_precision_error_bound(dev):
+    rows, q = W.dipole3d(seed=2, n=(40, 36, 32), nq=50_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    o = O.tricubic(rows.copy(), "quiet", mode="both")
+    c, nv, gr = t.Query(q.copy())
+    co, no, go = o.Query(q.copy())
+    sc, sn, sg = o.condition_scale(q)
+    for got, ref, s, name in ((c, co, sc, "comps"), (nv, no, sn, "norm"), (gr, go, sg, "grad")):
+        mx, rms = O.rel_err_normwise(got, ref, s)
+        print("mixed %s: max %.3e rms %.3e" % (name, mx, rms))
+        assert mx <= TOLMIX, (name, mx, rms)
+
+
+def test_device_path_matches_host_path(dev):
+    import torch
+    rows, q = W.scalar3d_gaussians(seed=1)
+    t = dev.tricubic(rows.copy(), "quiet")
+    vh, gh = t.Query(q.copy())
+    qd = torch.from_numpy(np.concatenate([q, np.ones((len(q), 2))], axis=1)).cuda()
+    _, nd, gd = t.query_device(qd, nan_rows=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(np.nan_to_num(nd.cpu().numpy(), nan=3.0), np.nan_to_num(vh[:, 0], nan=3.0))
+    assert np.array_equal(np.nan_to_num(gd.cpu().numpy(), nan=3.0), np.nan_to_num(gh, nan=3.0))
+    qa = q.copy()
+    o = O.tricubic(rows.copy(), "quiet")
+    o.Query(qa)
+    oob = np.isnan(qa).any(axis=1) & ~np.isnan(q).any(axis=1)
+    qd_h = qd.cpu().numpy()
+    assert np.isnan(qd_h[oob]).all() and not np.isnan(qd_h[~oob]).any()
+
+
+def test_deterministic_repeat(dev):
+    rows, q = W.dipole3d(seed=2, n=(30, 28, 26), nq=20_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    a = t.Query(q.copy())
+    b = t.Query(q.copy())
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_full_size_dipole_sample(dev):
+    """128^3 vector field, mode 'both': 10^6 queries on the GPU; 20k of them
+    checked against the oracle; every row in range and finite."""
+    import torch
+    rows, q = W.dipole3d(seed=2, nq=1_000_000)
+    t = dev.tricubic(rows, "quiet", mode="both")
+    qd = torch.from_numpy(q).cuda()
+    c, nv, gr, cells = t.query_device(qd, cells=True)
+    torch.cuda.synchronize()
+    assert int(cells.max()) < t.nc and torch.isfinite(nv).all()
+    o = O.tricubic(rows, "quiet", mode="both")
+    idx = np.random.default_rng(0).choice(len(q), 20_000, replace=False)
+    co, no, go = o.Query(q[idx].copy())
+    assert np.array_equal(cells.cpu().numpy()[idx], o.queryInds)
+    sc, sn, sg = o.condition_scale(q[idx])
+    for got, ref, s in ((c.cpu().numpy()[idx], co, sc), (nv.cpu().numpy()[idx, None], no, sn),
+                        (gr.cpu().numpy()[idx], go, sg)):
+        mx, _ = O.rel_err(got, ref, s)
+        assert mx <= TOL64, mx
+
+
+def _nan_equal(a, b):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else a
+    b = b.cpu().numpy() if hasattr(b, "cpu") else b
+    return np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(a[~np.isnan(a)],
+                                                                       b[~np.isnan(b)])
+
+
+@pytest.mark.parametrize("case", ["3d_both_mixed", "3d_both_fp64", "3d_scalar", "4d_scalar",
+                                  "4d_both_mixed", "3d_vector_tuned"])
+def test_query_paths_bitwise_identical(dev, case):
+    """Direct, bucketed-LDS and bucketed-global paths give identical bits,
+    including rows outside the box, NaN rows, IntMax rows, partial blocks."""
+    import torch
+    from arbinterp_amd import _native as NN
+    rng = np.random.default_rng(5)
+    if case.startswith("3d"):
+        rows, q = W.dipole3d(seed=2, n=(23, 19, 14), nq=30_000)
+        if case == "3d_scalar":
+            rows = rows[:, :4].copy()
+    else:
+        rows, q = W.moving4d(seed=3, n=(11, 9, 8, 7), nq=30_000)
+        if case == "4d_both_mixed":
+            extra = np.stack([np.sin(rows[:, 0] * 3 + rows[:, 3]), np.cos(rows[:, 1] * 2),
+                              rows[:, 4]], axis=1)
+            rows = np.concatenate([rows[:, :4], extra], axis=1)
+    cls = dev.tricubic if case.startswith("3d") else dev.quadcubic
+    kw = {}
+    if "both" in case:
+        kw["mode"] = "both"
+    if "vector" in case:
+        kw["mode"] = "vector"
+    prec = "mixed" if "mixed" in case else "fp64"
+    t = cls(rows, "quiet", precision=prec, **kw)
+    if case == "3d_vector_tuned":
+        t.set_tuning((3, 2, 5), 128)
+    d = t.NDIM
+    lo = np.array([getattr(t, "xyzt"[a] + "IntMin") for a in range(d)])
+    hi = np.array([getattr(t, "xyzt"[a] + "IntMax") for a in range(d)])
+    extra = rng.uniform(lo - 0.2 * (hi - lo), hi + 0.2 * (hi - lo), size=(3000, d))
+    edge = np.tile((lo + hi) / 2, (6, 1))
+    for a in range(min(d, 3)):
+        edge[2 * a, a] = hi[a]
+        edge[2 * a + 1, a] = lo[a]
+    q = np.concatenate([q, extra, edge, np.full((3, d), np.nan)], axis=0)
+    q = q[rng.permutation(len(q))]
+    qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+    outs = {}
+    for name, fl in (("direct", NN.ARB_Q_DIRECT), ("lds", NN.ARB_Q_SORTED),
+                     ("global", NN.ARB_Q_SORTED_GLOBAL)):
+        outs[name] = t.query_device(qd.clone(), cells=True, flags=fl)
+    torch.cuda.synchronize()
+    for name in ("lds", "global"):
+        for a, b in zip(outs["direct"], outs[name]):
+            if a is None:
+                assert b is None
+                continue
+            assert _nan_equal(a, b), (case, name)
