@@ -119,6 +119,7 @@ class _Interp:
         self._hi = np.array([info.int_max[a] for a in range(d)])
         self._hh = np.array([info.h[a] for a in range(d)])
         self.queryInds = None
+        self.default_flags = 0      # query-path choice set by arbinterp_amd.autotune
 
     # -- reference attributes -------------------------------------------------
     @property
@@ -230,6 +231,8 @@ class _Interp:
         else:
             comps, norms, grads = self._alloc_out(torch, n, dt)
             cel = torch.empty((n,), dtype=torch.int64, device="cuda") if cells else None
+        if not flags & (N.ARB_Q_DIRECT | N.ARB_Q_SORTED | N.ARB_Q_SORTED_GLOBAL):
+            flags |= self.default_flags
         fl = flags | (N.ARB_Q_NAN_ROWS if nan_rows else 0)
         if out_f64:
             fl |= N.ARB_Q_OUT_F64
@@ -259,7 +262,8 @@ class _Interp:
         out_dt = torch.float64
         comps, norms, grads = self._alloc_out(torch, n, out_dt)
         cells = torch.empty((n,), dtype=torch.int64, device="cuda")
-        flags = N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
+        flags = (N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
+                 | self.default_flags)
         rc = self._run(torch, q_dev, n, d, comps, norms, grads, cells, flags)
         self.queryInds = cells.cpu().numpy()
         if rc == N.ARB_EINDEX:

@@ -60,6 +60,15 @@ SIGNATURES = [
     ("arb_build_coeffs", c_i32, [c_vp, c_vp]),
     ("arb_query", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     ("arb_table_copy", c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp]),
+    ("arb_field_values", c_i32, [c_vp, c_vp, c_vp]),
+    ("arb_field_set_values", c_i32, [c_vp, c_vp, c_vp]),
+    ("arb_query_backward", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    ("arb_batch_create", c_i32, [ctypes.POINTER(c_vp), c_i32, c_vp, ctypes.POINTER(c_vp)]),
+    ("arb_batch_query", c_i32, [c_vp, c_vp, ctypes.POINTER(c_i64), c_i64, c_vp, c_vp, c_vp,
+                                c_i32, c_vp]),
+    ("arb_batch_set_tuning", c_i32, [c_vp, c_i32, c_i32, c_i32]),
+    ("arb_batch_destroy", None, [c_vp]),
+    ("arb_push_rk4", c_i32, [c_vp, c_vp, c_vp, c_i64, ctypes.c_double, c_i64, c_i32, c_i32, c_vp]),
     ("arb_last_timings", c_i32, [c_vp, ctypes.POINTER(ctypes.c_float)]),
     ("arb_set_tuning", c_i32, [c_vp, ctypes.POINTER(c_i32), c_i32]),
     ("arb_operator_matrix", c_i32, [c_i32, c_i32, c_vp]),

@@ -21,7 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["arb_capi.hip", "k_grid.hip", "k_coef.hip", "k_coef_mfma.hip", "k_query.hip"]
+SOURCES = ["arb_capi.hip", "k_grid.hip", "k_coef.hip", "k_coef_mfma.hip", "k_query.hip",
+           "k_backward.hip", "k_batch.hip"]
 
 CFLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
@@ -51,7 +52,13 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     tmp = OUT + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp]
+         + objs + ["-L/opt/rocm/lib", "-lamdhip64"], verbose)
+    # the library must load (every symbol resolved) before it replaces the old one
+    r = subprocess.run([sys.executable, "-c", "import ctypes,sys; ctypes.CDLL(sys.argv[1])", tmp],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("built library does not load:\n" + r.stderr)
     os.replace(tmp, OUT)
     return OUT
 

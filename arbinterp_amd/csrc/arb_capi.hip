@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -181,10 +182,24 @@ struct arb_field {
     int32_t* ws_bkt = nullptr;  // per query: bucket (-1 = handled in place)
     int32_t* ws_rank = nullptr; // per query: rank inside its bucket
     int32_t* ws_perm = nullptr; // bucket-sorted query indices
+    double* ws_qs = nullptr;    // positions in bucket order (n x ndim)
     int64_t ws_n = 0;
     int32_t* ws_count = nullptr;
     int32_t* ws_offs = nullptr;
     int64_t ws_nb = 0;
+    // backward workspace
+    int32_t* bw_key = nullptr;     // per query: cell (-1 outside)
+    int32_t* bw_perm = nullptr;    // queries grouped per cell, index order
+    int64_t bw_n = 0;
+    int32_t* bw_count = nullptr;   // nc + 1
+    int32_t* bw_offs = nullptr;    // nc + 1
+    int32_t* bw_cursor = nullptr;  // nc + 1
+    double* bw_bbar = nullptr;     // nc x K
+    // particle push graph
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t push_exec = nullptr;
+    double push_key_d = 0.0;
+    int64_t push_key[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[5] = {};
     int32_t ev_ready = 0;
     float last_ms[4] = {0, 0, 0, 0};
@@ -219,9 +234,18 @@ static void choose_buckets(arb_field* f) {
     }
     B.ncell = 1;
     B.nbt = 1;
+    B.cvsh = -1;
+    for (int sh = 0; sh < 16; ++sh)
+        if ((1 << sh) == B.cellvec) B.cvsh = sh;
+    int lsh = 0;
     for (int a = 0; a < ARB_MAXD; ++a) {
         B.b[a] = a < d ? dims[a] : 1;
+        B.bsh[a] = 0;
+        while ((1 << B.bsh[a]) < B.b[a]) ++B.bsh[a];
+        B.lsh[a] = lsh;
+        lsh += B.bsh[a];
         B.nb[a] = a < d ? (f->g.npos[a] + B.b[a] - 1) / B.b[a] : 1;
+        B.nbs[a] = B.nbt;
         B.ncell *= B.b[a];
         B.nbt *= B.nb[a];
     }
@@ -237,6 +261,9 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         HIPCHK(hipMalloc(&f->ws_bkt, sizeof(int32_t) * n));
         HIPCHK(hipMalloc(&f->ws_rank, sizeof(int32_t) * n));
         HIPCHK(hipMalloc(&f->ws_perm, sizeof(int32_t) * n));
+        if (f->ws_qs) (void)hipFree(f->ws_qs);
+        f->ws_qs = nullptr;
+        HIPCHK(hipMalloc(&f->ws_qs, sizeof(double) * f->g.ndim * n));
         f->ws_n = n;
     }
     if (f->B.nbt + 1 > f->ws_nb) {
@@ -450,7 +477,10 @@ int arb_build_coeffs(arb_field* f, void* stream) {
     const int K = 1 << (2 * f->g.ndim);
     const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
     if (f->precision == ARB_PREC_FP64) {
-        HIPCHK(arb_launch_coef_fp64(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
+        if (f->g.ndim == 3)
+            HIPCHK(arb_launch_coef_f64mfma(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
+        else
+            HIPCHK(arb_launch_coef_fp64(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
     } else {
         HIPCHK(arb_launch_coef_bf16x3(f->g, f->vals, f->ninterp, f->Ab, (float*)f->table, s));
     }
@@ -514,7 +544,8 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
         HIPCHK(arb_launch_scan(f->ws_count, f->ws_offs, f->B.nbt, s));
         if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
-        HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, n, s));
+        HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, q_dev, ldq,
+                                  f->g.ndim, f->ws_qs, n, s));
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));
         if (flags & ARB_Q_SORTED_GLOBAL)
             HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
@@ -522,7 +553,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                         f->ws_perm, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
-                                          f->has_comps, f->has_norm, q_dev, ldq, o, f->ws_offs,
+                                          f->has_comps, f->has_norm, f->ws_qs, o, f->ws_offs,
                                           f->ws_perm, f->eval_threads, s));
         if (timing) {
             HIPCHK(hipEventRecord(f->ev[4], s));
@@ -540,6 +571,253 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     return ARB_OK;
 }
 
+int arb_field_values(const arb_field* f, double* dst_dev, void* stream) {
+    if (!f || !dst_dev) return fail(ARB_EINVAL, "arb_field_values: NULL argument");
+    HIPCHK(hipMemcpyAsync(dst_dev, f->vals, sizeof(double) * f->g.nnodes * f->ninterp,
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    return ARB_OK;
+}
+
+int arb_field_set_values(arb_field* f, const double* src_dev, void* stream) {
+    if (!f || !src_dev) return fail(ARB_EINVAL, "arb_field_set_values: NULL argument");
+    HIPCHK(hipMemcpyAsync(f->vals, src_dev, sizeof(double) * f->g.nnodes * f->ninterp,
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    f->table_built = 0;
+    return ARB_OK;
+}
+
+int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq,
+                       const double* gv_dev, const double* gg_dev, double* dq_dev,
+                       double* dval_dev, void* stream) {
+    if (!f) return fail(ARB_EINVAL, "arb_query_backward: NULL handle");
+    if (!f->has_norm) return fail(ARB_EINVAL, "backward needs a scalar interpolant (scalar, norm or both)");
+    if (n < 0 || (n > 0 && (!q_dev || !gv_dev || !gg_dev || ldq < f->g.ndim)))
+        return fail(ARB_EINVAL, "arb_query_backward: bad arguments");
+    if (n > INT32_MAX - 1 || f->g.nc > INT32_MAX - 1)
+        return fail(ARB_EINVAL, "arb_query_backward: too many queries or cells for int32 indices");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = arb_build_coeffs(f, stream);
+    if (rc) return rc;
+    const int K = 1 << (2 * f->g.ndim);
+    const int64_t nc = f->g.nc;
+    if (dval_dev) {
+        if (n > f->bw_n) {
+            if (f->bw_key) (void)hipFree(f->bw_key);
+            if (f->bw_perm) (void)hipFree(f->bw_perm);
+            f->bw_key = f->bw_perm = nullptr;
+            f->bw_n = 0;
+            HIPCHK(hipMalloc(&f->bw_key, sizeof(int32_t) * (n ? n : 1)));
+            HIPCHK(hipMalloc(&f->bw_perm, sizeof(int32_t) * (n ? n : 1)));
+            f->bw_n = n;
+        }
+        if (!f->bw_bbar) {
+            HIPCHK(hipMalloc(&f->bw_count, sizeof(int32_t) * (nc + 1)));
+            HIPCHK(hipMalloc(&f->bw_offs, sizeof(int32_t) * (nc + 1)));
+            HIPCHK(hipMalloc(&f->bw_cursor, sizeof(int32_t) * (nc + 1)));
+            HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * nc * K));
+        }
+        HIPCHK(arb_launch_fill_i32(f->bw_count, nc + 1, 0, s));
+        HIPCHK(arb_launch_fill_i32(f->bw_cursor, nc + 1, 0, s));
+    }
+    if (n > 0)
+        HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
+                                     q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
+                                     dval_dev ? f->bw_key : nullptr, f->bw_count, s));
+    if (dval_dev) {
+        HIPCHK(arb_launch_scan(f->bw_count, f->bw_offs, nc, s));
+        HIPCHK(arb_launch_seg_scatter(f->bw_key, f->bw_offs, f->bw_cursor, f->bw_perm, n, s));
+        HIPCHK(arb_launch_seg_sort(f->bw_offs, f->bw_perm, nc, s));
+        HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_offs,
+                                    f->bw_perm, f->bw_bbar, s));
+        HIPCHK(arb_launch_bwd_nodes(f->g, f->bw_bbar, dval_dev, s));
+    }
+    return ARB_OK;
+}
+
+int arb_push_rk4(arb_field* f, double* x_dev, double* v_dev, int64_t n, double dt,
+                 int64_t nsteps, int32_t steps_per_launch, int32_t use_graph, void* stream) {
+    if (!f) return fail(ARB_EINVAL, "arb_push_rk4: NULL handle");
+    if (f->g.ndim != 3 || !f->has_norm)
+        return fail(ARB_EINVAL, "the particle push needs a 3D field with a scalar interpolant");
+    if (n < 0 || nsteps < 0 || (n > 0 && (!x_dev || !v_dev)) || steps_per_launch < 1)
+        return fail(ARB_EINVAL, "arb_push_rk4: bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = arb_build_coeffs(f, stream);
+    if (rc) return rc;
+    if (n == 0 || nsteps == 0) return ARB_OK;
+    const int64_t nlaunch = (nsteps + steps_per_launch - 1) / steps_per_launch;
+    auto issue = [&](hipStream_t st) -> hipError_t {
+        int64_t left = nsteps;
+        for (int64_t k = 0; k < nlaunch; ++k) {
+            const int32_t m = (int32_t)(left < steps_per_launch ? left : steps_per_launch);
+            hipError_t e = arb_launch_rk4(f->g, f->precision, f->table, f->ninterp,
+                                          f->ninterp - 1, x_dev, v_dev, n, dt, m, st);
+            if (e != hipSuccess) return e;
+            left -= m;
+        }
+        return hipSuccess;
+    };
+    if (!use_graph) {
+        HIPCHK(issue(s));
+        return ARB_OK;
+    }
+    const int64_t key[5] = {(int64_t)(uintptr_t)x_dev, (int64_t)(uintptr_t)v_dev, n, nsteps,
+                            steps_per_launch};
+    bool same = f->push_exec && f->push_key_d == dt;
+    for (int k = 0; k < 5 && same; ++k) same = f->push_key[k] == key[k];
+    if (!same) {
+        if (f->push_exec) {
+            HIPCHK(hipGraphExecDestroy(f->push_exec));
+            f->push_exec = nullptr;
+        }
+        if (!f->cap_stream) HIPCHK(hipStreamCreateWithFlags(&f->cap_stream, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(f->cap_stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = issue(f->cap_stream);
+        hipError_t e2 = hipStreamEndCapture(f->cap_stream, &graph);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(ARB_EHIP, "graph capture failed: %s",
+                        hipGetErrorString(e != hipSuccess ? e : e2));
+        }
+        e = hipGraphInstantiate(&f->push_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            f->push_exec = nullptr;
+            return fail(ARB_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+        }
+        for (int k = 0; k < 5; ++k) f->push_key[k] = key[k];
+        f->push_key_d = dt;
+    }
+    HIPCHK(hipGraphLaunch(f->push_exec, s));
+    return ARB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches of fields
+// ---------------------------------------------------------------------------
+struct arb_batch {
+    std::vector<arb_field*> fields;
+    int32_t ndim = 3, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
+    int32_t chunk = 256, threads = 256, blocks_per_cu = 4, ncu = 256;
+    std::vector<int64_t> last_counts;
+    std::vector<unsigned char> desc_host;
+    void* desc_dev = nullptr;
+    int64_t nitems = 0;
+    unsigned int* counter = nullptr;
+    int32_t* flag = nullptr;
+};
+
+int arb_batch_create(arb_field* const* fields, int32_t nfields, void* stream, arb_batch** out) {
+    if (!out || !fields || nfields < 1) return fail(ARB_EINVAL, "arb_batch_create: bad arguments");
+    *out = nullptr;
+    arb_batch* b = new (std::nothrow) arb_batch();
+    if (!b) return fail(ARB_ENOMEM, "host allocation failed");
+    struct Guard { arb_batch* b; ~Guard() { if (b) arb_batch_destroy(b); } } guard{b};
+    for (int k = 0; k < nfields; ++k) {
+        arb_field* f = fields[k];
+        if (!f) return fail(ARB_EINVAL, "field %d is NULL", k);
+        if (k == 0) {
+            b->ndim = f->g.ndim;
+            b->precision = f->precision;
+            b->ninterp = f->ninterp;
+            b->has_comps = f->has_comps;
+            b->has_norm = f->has_norm;
+        } else if (f->g.ndim != b->ndim || f->precision != b->precision ||
+                   f->ninterp != b->ninterp || f->has_comps != b->has_comps ||
+                   f->has_norm != b->has_norm) {
+            return fail(ARB_EINVAL, "field %d differs in ndim/mode/precision from field 0", k);
+        }
+        int rc = arb_build_coeffs(f, stream);
+        if (rc) return rc;
+        b->fields.push_back(f);
+    }
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t dsz = arb_batch_field_desc_size();
+    b->desc_host.assign(dsz * nfields, 0);
+    HIPCHK(hipMalloc(&b->desc_dev, dsz * nfields));
+    HIPCHK(hipMalloc(&b->counter, sizeof(unsigned int)));
+    HIPCHK(hipMalloc(&b->flag, sizeof(int32_t)));
+    HIPCHK(hipMemsetAsync(b->flag, 0, sizeof(int32_t), (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    guard.b = nullptr;
+    *out = b;
+    return ARB_OK;
+}
+
+int arb_batch_set_tuning(arb_batch* b, int32_t chunk, int32_t threads, int32_t blocks_per_cu) {
+    if (!b) return fail(ARB_EINVAL, "arb_batch_set_tuning: NULL handle");
+    if (chunk && (chunk < 64 || chunk % 64 || chunk > 65536))
+        return fail(ARB_EINVAL, "chunk must be a multiple of 64 in [64, 65536]");
+    if (threads && (threads < 64 || threads > 1024 || threads % 64))
+        return fail(ARB_EINVAL, "threads must be a multiple of 64 in [64, 1024]");
+    if (blocks_per_cu && (blocks_per_cu < 1 || blocks_per_cu > 64))
+        return fail(ARB_EINVAL, "blocks_per_cu must be 1..64");
+    if (chunk) b->chunk = chunk;
+    if (threads) b->threads = threads;
+    if (blocks_per_cu) b->blocks_per_cu = blocks_per_cu;
+    b->last_counts.clear();          // item layout depends on chunk
+    return ARB_OK;
+}
+
+int arb_batch_query(arb_batch* b, double* q_dev, const int64_t* counts, int64_t ldq,
+                    void* comps_dev, void* norms_dev, void* grads_dev, int32_t flags,
+                    void* stream) {
+    if (!b || !counts) return fail(ARB_EINVAL, "arb_batch_query: NULL argument");
+    const int nf = (int)b->fields.size();
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<int64_t> cnt(counts, counts + nf);
+    int64_t total = 0;
+    for (int64_t c : cnt) {
+        if (c < 0) return fail(ARB_EINVAL, "negative query count");
+        total += c;
+    }
+    if (total == 0) return ARB_OK;
+    if (!q_dev || ldq < b->ndim) return fail(ARB_EINVAL, "arb_batch_query: bad positions");
+    if ((b->has_comps && !comps_dev) || (b->has_norm && (!norms_dev || !grads_dev)))
+        return fail(ARB_EINVAL, "arb_batch_query: missing output buffer");
+    if (cnt != b->last_counts) {
+        const size_t dsz = arb_batch_field_desc_size();
+        int64_t q0 = 0, item = 0;
+        for (int k = 0; k < nf; ++k) {
+            arb_batch_fill_desc(b->desc_host.data() + dsz * k, b->fields[k]->g, b->fields[k]->table,
+                                q0, cnt[k], item);
+            q0 += cnt[k];
+            item += (cnt[k] + b->chunk - 1) / b->chunk;
+        }
+        if (item >= (int64_t)UINT32_MAX - (int64_t)b->ncu * 64)
+            return fail(ARB_EINVAL, "too many work items");
+        b->nitems = item;
+        HIPCHK(hipMemcpyAsync(b->desc_dev, b->desc_host.data(), dsz * nf, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        b->last_counts = cnt;
+    }
+    if (flags & ARB_Q_CHECK) HIPCHK(hipMemsetAsync(b->flag, 0, sizeof(int32_t), s));
+    int64_t nblocks = (int64_t)b->ncu * b->blocks_per_cu;
+    if (nblocks > b->nitems) nblocks = b->nitems;
+    HIPCHK(arb_launch_batch_query(b->ndim, b->precision, (flags & ARB_Q_OUT_F64) ? 1 : 0,
+                                  b->desc_dev, nf, b->nitems, b->chunk, b->threads, (int32_t)nblocks,
+                                  b->ninterp, b->has_comps, b->has_norm, q_dev, ldq, comps_dev,
+                                  norms_dev, grads_dev, b->flag, b->counter, s));
+    if (flags & ARB_Q_CHECK) {
+        int32_t h = 0;
+        HIPCHK(hipMemcpyAsync(&h, b->flag, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (h) return fail(ARB_EINDEX, "a query reduced to a cell index past the sentinel");
+    }
+    return ARB_OK;
+}
+
+void arb_batch_destroy(arb_batch* b) {
+    if (!b) return;
+    if (b->desc_dev) (void)hipFree(b->desc_dev);
+    if (b->counter) (void)hipFree(b->counter);
+    if (b->flag) (void)hipFree(b->flag);
+    delete b;
+}
+
 int arb_last_timings(const arb_field* f, float* ms4) {
     if (!f || !ms4) return fail(ARB_EINVAL, "arb_last_timings: NULL argument");
     for (int k = 0; k < 4; ++k) ms4[k] = f->last_ms[k];
@@ -554,8 +832,8 @@ int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads) {
     if (block_cells) {
         int64_t cells = 1;
         for (int a = 0; a < d; ++a) {
-            if (block_cells[a] < 1 || block_cells[a] > 64)
-                return fail(ARB_EINVAL, "block edge must be 1..64 cells");
+            if (block_cells[a] < 1 || block_cells[a] > 64 || (block_cells[a] & (block_cells[a] - 1)))
+                return fail(ARB_EINVAL, "block edge must be a power of two in 1..64 cells");
             cells *= block_cells[a];
         }
         const int K = 1 << (2 * d);
@@ -595,10 +873,19 @@ void arb_field_destroy(arb_field* f) {
     if (f->ws_bkt) (void)hipFree(f->ws_bkt);
     if (f->ws_rank) (void)hipFree(f->ws_rank);
     if (f->ws_perm) (void)hipFree(f->ws_perm);
+    if (f->ws_qs) (void)hipFree(f->ws_qs);
     if (f->ws_count) (void)hipFree(f->ws_count);
     if (f->ws_offs) (void)hipFree(f->ws_offs);
+    if (f->bw_key) (void)hipFree(f->bw_key);
+    if (f->bw_perm) (void)hipFree(f->bw_perm);
+    if (f->bw_count) (void)hipFree(f->bw_count);
+    if (f->bw_offs) (void)hipFree(f->bw_offs);
+    if (f->bw_cursor) (void)hipFree(f->bw_cursor);
+    if (f->bw_bbar) (void)hipFree(f->bw_bbar);
     if (f->ev_ready)
         for (auto& e : f->ev) (void)hipEventDestroy(e);
+    if (f->push_exec) (void)hipGraphExecDestroy(f->push_exec);
+    if (f->cap_stream) (void)hipStreamDestroy(f->cap_stream);
     delete f;
 }
 

@@ -36,10 +36,14 @@ struct ArbOut {
 // Cell blocks used to bucket queries: a bucket is a block of b[0] x b[1] x
 // b[2] [x b[3]] cells; its coefficients are staged in LDS together.
 struct ArbBuckets {
-    int32_t b[ARB_MAXD];     // block edge (cells) per axis
+    int32_t b[ARB_MAXD];     // block edge (cells) per axis, a power of two
+    int32_t bsh[ARB_MAXD];   // log2 of b
+    int32_t lsh[ARB_MAXD];   // shift of axis a inside the local cell index
     int64_t nb[ARB_MAXD];    // blocks per axis
+    int64_t nbs[ARB_MAXD];   // bucket-index stride of axis a
     int64_t nbt;             // number of buckets
     int32_t cellvec;         // 16-byte vectors per cell of coefficients
+    int32_t cvsh;            // log2(cellvec) when cellvec is a power of two, else -1
     int32_t cellstride;      // LDS stride per cell in 16-byte vectors
     int32_t ncell;           // cells per block
 };
@@ -74,6 +78,35 @@ __device__ __forceinline__ int64_t arb_reduce(const double* __restrict__ q, cons
         bad |= isnan(iu);
         int64_t ia = bad ? 0 : (int64_t)fl;
         flat += ia * mul;
+        mul *= g.npos[a];
+    }
+    return bad ? g.nc : flat;
+}
+
+// Same reduction, also returning the per-axis cell coordinates floor(iu)
+// (valid only when the returned flat index is below nc).
+template <int D>
+__device__ __forceinline__ int64_t arb_reduce_axes(const double* __restrict__ q, const ArbGeom& g,
+                                                   double* __restrict__ loc, int64_t* ia,
+                                                   bool* oob) {
+    bool out = false;
+    double x[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        x[a] = q[a];
+        out |= (x[a] < g.lo[a]) | (x[a] > g.hi[a]);
+    }
+    *oob = out;
+    int64_t flat = 0, mul = 1;
+    bool bad = out;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        double iu = out ? NAN : (x[a] - g.lo[a]) / g.h[a];
+        double fl = floor(iu);
+        loc[a] = iu - fl;
+        bad |= isnan(iu);
+        ia[a] = bad ? 0 : (int64_t)fl;
+        flat += ia[a] * mul;
         mul *= g.npos[a];
     }
     return bad ? g.nc : flat;
@@ -192,6 +225,8 @@ hipError_t arb_launch_coef_mixed(const ArbGeom& g, const double* vals, int32_t n
                                  const double* A, float* table, hipStream_t s);
 hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
                                   const unsigned short* Ab, float* table, hipStream_t s);
+hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                   const double* A, double* table, hipStream_t s);
 hipError_t arb_launch_sentinel(void* table, int32_t f32, int64_t nc, int32_t ninterp, int32_t K,
                                int32_t nan, hipStream_t s);
 
@@ -203,17 +238,41 @@ hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec
                              hipStream_t s);
 hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
-                              int32_t* perm, int64_t n, hipStream_t s);
+                              int32_t* perm, const double* q, int64_t ldq, int32_t ndim,
+                              double* qs, int64_t n, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
-                                  int32_t has_comps, int32_t has_norm, const double* q,
-                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
+                                  int32_t has_comps, int32_t has_norm, const double* qs,
+                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
+                                  int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                 int32_t out_f64, const void* table, int32_t ninterp,
                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
                                 const int32_t* perm, hipStream_t s);
+hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
+                                 int32_t ninterp, int32_t kn, const double* q, int64_t n,
+                                 int64_t ldq, const double* gv, const double* gg, double* dq,
+                                 int32_t* key, int32_t* count, hipStream_t s);
+hipError_t arb_launch_seg_scatter(const int32_t* key, const int32_t* offs, int32_t* cursor,
+                                  int32_t* perm, int64_t n, hipStream_t s);
+hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, hipStream_t s);
+hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
+                                const double* gv, const double* gg, const int32_t* offs,
+                                const int32_t* perm, double* bbar, hipStream_t s);
+hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s);
+size_t arb_batch_field_desc_size();
+void arb_batch_fill_desc(void* dst, const ArbGeom& g, const void* table, int64_t q0, int64_t nq,
+                         int64_t item0);
+hipError_t arb_launch_batch_query(int32_t ndim, int32_t prec, int32_t out_f64, const void* fields,
+                                  int32_t nfields, int64_t nitems, int32_t chunk, int32_t threads,
+                                  int32_t nblocks, int32_t ninterp, int32_t has_comps,
+                                  int32_t has_norm, const double* q, int64_t ldq, void* comps,
+                                  void* norms, void* grads, int32_t* flag, unsigned int* counter,
+                                  hipStream_t s);
+hipError_t arb_launch_rk4(const ArbGeom& g, int32_t prec, const void* table, int32_t ninterp,
+                          int32_t kn, double* X, double* V, int64_t n, double dt, int32_t nsteps,
+                          hipStream_t s);
 hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f64,
                                    const void* table, int32_t ninterp, int32_t has_comps,
                                    int32_t has_norm, const double* q, int64_t n, int64_t ldq,

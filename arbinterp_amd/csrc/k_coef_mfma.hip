@@ -6,14 +6,20 @@
 // FP64 from an LDS tile of node values, then alpha = inv(B) . b is one dense
 // contraction per tile of 16 cells:
 //     (K x K) x (K x 16),  K = 64 (3D) or 256 (4D)
-// on v_mfma_f32_16x16x32_bf16.  inv(B) is integer (|.| <= 81) and exact in
-// BF16; b is split as b = hi + mid + lo (three BF16 parts, 24 significant
-// bits), so alpha = inv(B).lo + inv(B).mid + inv(B).hi with FP32 accumulation,
-// smallest part first.  Before the split the cell's base value f(corner 0) is
-// subtracted from the 8 (16) corner values: inv(B) maps the constant data
-// (1 at every corner, 0 derivatives) to e_0, so alpha_0 gets it back exactly
-// and the GEMM only carries differences, which keeps the FP32 accumulation
-// error proportional to the field's variation across the stencil.
+// on v_mfma_f32_16x16x32_bf16 with FP32 accumulation, made exact:
+//   * the cell's base value f(corner 0) is subtracted from its 8 (16) corner
+//     values (inv(B) maps constant data to e_0, so alpha_0 takes it back);
+//   * the residual b' is scaled by a power of two 2^-E so |b'| < 1 and split
+//     on a common fixed-point grid into four 8-bit integer parts,
+//     b' 2^-E = p0/2^8 + p1/2^16 + p2/2^24 + p3/2^32 (+ < 2^-33),
+//     |p_k| <= 256, exact in BF16;
+//   * inv(B) is integer (|.| <= 81), exact in BF16.
+// Every product is then an integer below 2^15 and every partial sum an
+// integer below 2^23 (K * 81 * 256), so each part's MFMA chain is exact in
+// FP32 whatever its internal order.  The four accumulators are combined in
+// FP64 (exact), scaled back by 2^E and rounded once to FP32: the stored
+// alpha is the FP64 alpha of the quantised b' (quantisation < 2^-33 of the
+// cell's largest residual) correctly rounded to FP32.
 //
 // Work split: one workgroup = 4 waves = a 16(x) x 4(y) tile of cells at one
 // z (and t); wave w owns cell row y0+w, lane l the cell x0 + (l & 15).  The
@@ -133,7 +139,8 @@ k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
         const int lxc = valid ? lx : 0;
         const int lyc = w;
         const double f0 = fd_entry<D>(tile, lxc, lyc, 0);
-        us8 bp[3][KS];
+        double bv[KS][8];
+        double mx = 0.0;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
 #pragma unroll
@@ -141,33 +148,124 @@ k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
                 const int e = ks * 32 + 8 * grp + j;
                 double b = fd_entry<D>(tile, lxc, lyc, e);
                 if (e < nco) b -= f0;                   // base value carried by alpha_0
-                const unsigned short hi = bf16_rne((float)b);
-                const double r1 = b - bf16_to_double(hi);
-                const unsigned short mid = bf16_rne((float)r1);
-                const double r2 = r1 - bf16_to_double(mid);
-                const unsigned short lo = bf16_rne((float)r2);
-                bp[0][ks][j] = lo;
-                bp[1][ks][j] = mid;
-                bp[2][ks][j] = hi;
+                bv[ks][j] = b;
+                mx = fmax(mx, fabs(b));
+                if (b != b) mx = b;                      // NaN data: NaN coefficients
+            }
+        }
+        // the cell's entries live in lanes lx, lx+16, lx+32, lx+48
+        double m1 = __shfl_xor(mx, 16);
+        mx = (m1 != m1) ? m1 : fmax(mx, m1);
+        m1 = __shfl_xor(mx, 32);
+        mx = (m1 != m1) ? m1 : fmax(mx, m1);
+        int E = 0;
+        if (mx > 0.0 && mx == mx && mx < INFINITY) (void)frexp(mx, &E);   // mx < 2^E
+        us8 bp[4][KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                double x = ldexp(bv[ks][j], 8 - E);      // |x| < 2^8
+#pragma unroll
+                for (int pp = 0; pp < 4; ++pp) {
+                    const double pk = rint(x);           // integer, |pk| <= 256
+                    x = (x - pk) * 256.0;                // exact
+                    bp[pp][ks][j] = (unsigned short)(__float_as_uint((float)pk) >> 16);
+                }
             }
         }
         float* out = table + (cell * ninterp + k) * K;
 #pragma unroll 1
         for (int mt = 0; mt < MT; ++mt) {
-            f4 acc = {0.f, 0.f, 0.f, 0.f};
+            f4 acc[4];
 #pragma unroll
-            for (int p = 0; p < 3; ++p) {
+            for (int pp = 0; pp < 4; ++pp) {
+                acc[pp] = (f4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int ks = 0; ks < KS; ++ks) {
                     const us8 a = *reinterpret_cast<const us8*>(Ab + (((mt * KS + ks) * 64 + lane) * 8));
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
-                                                                  __builtin_bit_cast(bf8, bp[p][ks]),
-                                                                  acc, 0, 0, 0);
+                    acc[pp] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, a),
+                                                                      __builtin_bit_cast(bf8, bp[pp][ks]),
+                                                                      acc[pp], 0, 0, 0);
                 }
             }
             // lane holds alpha[mt*16 + 4*grp + r] of cell lx
-            if (mt == 0 && grp == 0) acc[0] = (float)((double)acc[0] + f0);
-            if (valid) *reinterpret_cast<f4*>(out + mt * 16 + 4 * grp) = acc;
+            f4 res;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double v = (double)acc[0][r] * 0x1p-8 + (double)acc[1][r] * 0x1p-16 +
+                           (double)acc[2][r] * 0x1p-24 + (double)acc[3][r] * 0x1p-32;
+                v = ldexp(v, E);
+                if (mt == 0 && grp == 0 && r == 0) v += f0;
+                res[r] = (float)v;
+            }
+            if (valid) *reinterpret_cast<f4*>(out + mt * 16 + 4 * grp) = res;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plain FP64 (ARB_PREC_FP64), 3D: alpha = A . F with A = inv(B).D (exact
+// dyadic, the reference's A of ARBInterp.py:175) and F the 64 neighbourhood
+// values of each cell, on v_mfma_f64_16x16x4f64.  Same tiling as above:
+// 4 waves x 16 cells; per wave 4 row tiles x 16 k-steps; each lane keeps its
+// 64 A operands in registers; F operands come from the LDS node tile.
+// Operands (lane l): A[row l%16][k l/16], B[k l/16][col l%16]; accumulator
+// register r of lane l holds D[row l/16 + 4r][col l%16].
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+k_coef_f64mfma3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
+                const double* __restrict__ A, double* __restrict__ table) {
+    __shared__ double tile[TileGeom<3>::NODES];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lx = lane & 15, kq = lane >> 4;
+    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
+    const int64_t cy0 = (int64_t)blockIdx.y * TYC;
+    const int64_t cz = blockIdx.z;
+    const int64_t cy = cy0 + w;
+    double aop[4][16];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) aop[mt][ks] = A[(mt * 16 + lx) * 64 + ks * 4 + kq];
+    const bool colvalid = (cx0 + lx) < g.npos[0];
+    const int lxc = colvalid ? lx : 0;
+    const bool rowvalid = cy < g.npos[1];
+    const int64_t cell = (cx0 + lx) + g.npos[0] * (cy + g.npos[1] * cz);
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();
+        for (int i = threadIdx.x; i < TileGeom<3>::NODES; i += 256) {
+            const int x = i % TX;
+            const int y = (i / TX) % TY;
+            const int z = i / (TX * TY);
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
+            tile[i] = v;
+        }
+        __syncthreads();
+        double bop[16];
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks) {
+            const int j = ks * 4 + kq;
+            const int ox = j & 3, oy = (j >> 2) & 3, oz = j >> 4;
+            bop[ks] = tile[(oz * TY + (w + oy)) * TX + (lxc + ox)];
+        }
+        double* out = table + (cell * ninterp + k) * 64;
+#pragma unroll 1
+        for (int mt = 0; mt < 4; ++mt) {
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < 16; ++ks)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[mt][ks], bop[ks], acc, 0, 0, 0);
+            if (rowvalid && colvalid) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) out[mt * 16 + kq + 4 * r] = acc[r];
+            }
         }
     }
 }
@@ -175,6 +273,15 @@ k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
+
+hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t ninterp,
+                                   const double* A, double* table, hipStream_t s) {
+    if (g.nc == 0) return hipSuccess;
+    if (g.ndim != 3) return hipErrorInvalidValue;
+    dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]), block(256);
+    hipLaunchKernelGGL(k_coef_f64mfma3, grid, block, 0, s, g, vals, ninterp, A, table);
+    return hipGetLastError();
+}
 
 // Ab: inv(B) as BF16 in operand order [mt][ks][lane][8] (host-prepared).
 hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,

@@ -88,22 +88,35 @@ k_query_direct(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t
 // the atomic ranks leave the outputs bitwise deterministic.
 // ---------------------------------------------------------------------------
 
-// Bucket of an in-range flat cell index and the cell's index inside its block.
+// Bucket of a cell and the cell's index inside its block.  The per-axis cell
+// coordinates come from the reduction; a query at IntMax can alias into the
+// next row/slab (floor(iu) == nPos on an axis while the flat index stays
+// below nc, as in the reference), and only then is the flat index
+// decomposed.  Block edges are powers of two: shifts and masks only.
 template <int D>
-__device__ __forceinline__ int64_t bucket_of(int64_t flat, const ArbGeom& g, const ArbBuckets& B,
-                                             int* lc) {
-    int64_t rem = flat, bidx = 0, bmul = 1;
-    int l = 0, lmul = 1;
+__device__ __forceinline__ int64_t bucket_of(int64_t flat, const int64_t* ia, const ArbGeom& g,
+                                             const ArbBuckets& B, int* lc) {
+    int64_t c[D];
+    bool alias = false;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        const int64_t c = rem % g.npos[a];
-        rem /= g.npos[a];
-        const int64_t bi = c / B.b[a];
-        const int li = (int)(c - bi * B.b[a]);
-        bidx += bi * bmul;
-        bmul *= B.nb[a];
-        l += li * lmul;
-        lmul *= B.b[a];
+        c[a] = ia[a];
+        alias |= ia[a] >= g.npos[a];
+    }
+    if (alias) {
+        int64_t rem = flat;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            c[a] = rem % g.npos[a];
+            rem /= g.npos[a];
+        }
+    }
+    int64_t bidx = 0;
+    int l = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        bidx += (c[a] >> B.bsh[a]) * B.nbs[a];
+        l |= (int)(c[a] & (B.b[a] - 1)) << B.lsh[a];
     }
     *lc = l;
     return bidx;
@@ -125,59 +138,86 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double loc[D];
-    const int64_t flat = reduce_row<D>(q, ldq, g, o, i, loc);
+    int64_t ia[D];
+    bool oob;
+    int64_t flat = arb_reduce_axes<D>(q + i * ldq, g, loc, ia, &oob);
+    if (o.cells) o.cells[i] = flat;
+    if (oob && o.qmut) {
+        for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
+    }
     if (flat >= g.nc) {
+        if (flat > g.nc) o.flag[0] = 1;
         eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
                                  o, i);
         bkt[i] = -1;
         return;
     }
     int lc;
-    const int64_t b = bucket_of<D>(flat, g, B, &lc);
+    const int64_t b = bucket_of<D>(flat, ia, g, B, &lc);
     bkt[i] = (int32_t)b;
     rank[i] = atomicAdd(&count[b], 1);
 }
 
-// Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024.
+// Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024
+// threads: each thread keeps 64 consecutive counts in registers per pass, so
+// a pass costs one round of independent loads and one of stores.
 __global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count,
                                                int32_t* __restrict__ offs, int64_t nb) {
-    __shared__ int64_t part[1024];
+    __shared__ int32_t part[1024];
     const int t = threadIdx.x;
-    const int64_t chunk = (nb + 1023) / 1024;
-    const int64_t s0 = t * chunk;
-    const int64_t s1 = (s0 + chunk < nb) ? s0 + chunk : nb;
-    int64_t sum = 0;
-    for (int64_t j = s0; j < s1; ++j) sum += count[j];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int64_t v = (t >= off) ? part[t - off] : 0;
+    int32_t carry = 0;
+    for (int64_t base = 0; base < nb; base += 1024 * 64) {
+        const int64_t i0 = base + (int64_t)t * 64;
+        int32_t v[64];
+        int32_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            v[k] = (i0 + k < nb) ? count[i0 + k] : 0;
+            sum += v[k];
+        }
+        part[t] = sum;
         __syncthreads();
-        part[t] += v;
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int32_t x = (t >= off) ? part[t - off] : 0;
+            __syncthreads();
+            part[t] += x;
+            __syncthreads();
+        }
+        int32_t run = carry + part[t] - sum;
+#pragma unroll
+        for (int k = 0; k < 64; ++k) {
+            if (i0 + k < nb) offs[i0 + k] = run;
+            run += v[k];
+        }
+        carry += part[1023];
         __syncthreads();
     }
-    int64_t run = part[t] - sum;        // exclusive prefix of this chunk
-    for (int64_t j = s0; j < s1; ++j) {
-        offs[j] = (int32_t)run;
-        run += count[j];
-    }
-    if (t == 1023) offs[nb] = (int32_t)part[1023];
+    if (t == 0) offs[nb] = carry;
 }
 
+// Scatter: query index and a copy of its position into bucket order, so the
+// evaluation kernel reads both contiguously.
+template <int D>
 __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
-                          const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
+                          const int32_t* __restrict__ offs, int32_t* __restrict__ perm,
+                          const double* __restrict__ q, int64_t ldq, double* __restrict__ qs,
+                          int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t b = bkt[i];
-    if (b >= 0) perm[offs[b] + rank[i]] = (int32_t)i;
+    if (b < 0) return;
+    const int64_t pos = offs[b] + rank[i];
+    perm[pos] = (int32_t)i;
+#pragma unroll
+    for (int a = 0; a < D; ++a) qs[pos * D + a] = q[i * ldq + a];
 }
 
 // One workgroup per bucket: stage the block's coefficients in LDS, then
-// evaluate the bucket's queries from LDS.
+// evaluate the bucket's queries (contiguous in qs/perm) from LDS.
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
 k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has_comps,
-              int32_t has_norm, const double* __restrict__ q, int64_t ldq, ArbOut o,
+              int32_t has_norm, const double* __restrict__ qs, ArbOut o,
               const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
     extern __shared__ uint4 smem[];
     const int64_t b = blockIdx.x;
@@ -190,7 +230,7 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
         for (int a = 0; a < D; ++a) {
             const int64_t bi = rem % B.nb[a];
             rem /= B.nb[a];
-            c0[a] = bi * B.b[a];
+            c0[a] = bi << B.bsh[a];
             const int64_t left = g.npos[a] - c0[a];
             ext[a] = left < B.b[a] ? left : B.b[a];
             cstr[a] = st;
@@ -200,15 +240,13 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     const uint4* __restrict__ tv = reinterpret_cast<const uint4*>(table);
     const int total = B.ncell * B.cellvec;
     for (int v = threadIdx.x; v < total; v += blockDim.x) {
-        const int lc = v / B.cellvec;
+        const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
         const int j = v - lc * B.cellvec;
-        int rem = lc;
         bool inside = true;
         int64_t gcell = 0;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            const int la = rem % B.b[a];
-            rem /= B.b[a];
+            const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
             inside &= la < ext[a];
             gcell += (c0[a] + la) * cstr[a];
         }
@@ -218,10 +256,11 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     for (int32_t s = start + threadIdx.x; s < end; s += blockDim.x) {
         const int64_t i = perm[s];
         double loc[D];
+        int64_t ia[D];
         bool oob;
-        const int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+        const int64_t flat = arb_reduce_axes<D>(qs + (int64_t)s * D, g, loc, ia, &oob);
         int lc;
-        const int64_t bb = bucket_of<D>(flat, g, B, &lc);
+        const int64_t bb = bucket_of<D>(flat, ia, g, B, &lc);
         if (bb != b || flat >= g.nc) continue;      // cannot happen: same reduction as k_bucket
         const TC* a = reinterpret_cast<const TC*>(smem + lc * B.cellstride);
         eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, i);
@@ -315,23 +354,29 @@ hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipS
 }
 
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
-                              int32_t* perm, int64_t n, hipStream_t s) {
+                              int32_t* perm, const double* q, int64_t ldq, int32_t ndim,
+                              double* qs, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_scatter, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs, perm, n);
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_scatter<3>, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs,
+                           perm, q, ldq, qs, n);
+    else
+        hipLaunchKernelGGL(k_scatter<4>, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs,
+                           perm, q, ldq, qs, n);
     return hipGetLastError();
 }
 
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
-                                  int32_t has_comps, int32_t has_norm, const double* q,
-                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
+                                  int32_t has_comps, int32_t has_norm, const double* qs,
+                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
+                                  int32_t block_threads, hipStream_t s) {
     (void)ninterp;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16;
     dim3 grid((unsigned)B.nbt), block(block_threads);
 #define L_EVALB(DD, T, TC, TO)                                                              \
     hipLaunchKernelGGL((k_eval_bucket<DD, T, TC, TO>), grid, block, lds, s, g, B,             \
-                       (const TC*)table, has_comps, has_norm, q, ldq, o, offs, perm)
+                       (const TC*)table, has_comps, has_norm, qs, o, offs, perm)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
 #undef L_EVALB

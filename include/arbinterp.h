@@ -125,6 +125,67 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
               void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
               int32_t flags, void* stream);
 
+/* Node values of the interpolated columns as stored, [ninterp][nnodes] FP64
+ * in canonical node order: scalar f; vector Bx,By,Bz; norm |B|; both
+ * Bx,By,Bz,|B| (the reference's Bn/Bx/By/Bz views of its sorted field,
+ * ARBInterp.py:32,46-48,58,71-74). */
+int arb_field_values(const arb_field* f, double* dst_dev, void* stream);
+
+/* Replace the stored node values (same layout as arb_field_values) and mark
+ * the coefficient table stale; the next query or arb_build_coeffs rebuilds
+ * it.  Used by the differentiable path, where node values are trainable.
+ * No reference counterpart (the reference field is fixed at construction). */
+int arb_field_set_values(arb_field* f, const double* src_dev, void* stream);
+
+/* Backward of the scalar interpolant's (value, gradient) outputs (modes
+ * scalar, norm, both).  No reference counterpart.  For upstream gradients
+ * gv_dev (n, FP64) and gg_dev (n x d, FP64), writes
+ *   dq_dev   (n x d FP64, nullable): gv*grad U + Hess U . gg  (NaN rows where
+ *            the position is outside the interpolation box);
+ *   dval_dev (nnodes FP64, nullable): the gradient with respect to the stored
+ *            scalar node values (last column of arb_field_values), computed
+ *            as A^T applied to per-cell sums taken in query-index order and
+ *            gathered per node in a fixed order (bitwise deterministic). */
+int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq,
+                       const double* gv_dev, const double* gg_dev, double* dq_dev,
+                       double* dval_dev, void* stream);
+
+/* ---- many fields, one launch ----------------------------------------------
+ * No reference counterpart (the reference queries one field per call).  All
+ * fields of a batch share ndim, mode and precision; their tables are built at
+ * creation.  The batch keeps pointers to the fields, which must outlive it. */
+typedef struct arb_batch arb_batch;
+
+int arb_batch_create(arb_field* const* fields, int32_t nfields, void* stream, arb_batch** out);
+
+/* Query every field of the batch in one persistent launch.  q_dev holds the
+ * positions of field 0, then field 1, ... (counts[k] rows each, row stride
+ * ldq); outputs use the same rows (comps n x 3, norms n, grads n x d, as in
+ * arb_query).  Returns ARB_EINDEX (after synchronising) with ARB_Q_CHECK if a
+ * row reduced past its field's sentinel. */
+int arb_batch_query(arb_batch* b, double* q_dev, const int64_t* counts, int64_t ldq,
+                    void* comps_dev, void* norms_dev, void* grads_dev, int32_t flags,
+                    void* stream);
+
+/* Work-item shape of the batched launch (autotuner hook): queries per work
+ * item (multiple of 64), threads per workgroup (64..1024), workgroups per
+ * compute unit.  0 keeps the current value. */
+int arb_batch_set_tuning(arb_batch* b, int32_t chunk, int32_t threads, int32_t blocks_per_cu);
+
+void arb_batch_destroy(arb_batch* b);
+
+/* ---- particle push ------------------------------------------------------
+ * Advance n particles (x_dev, v_dev: n x 3 FP64, updated in place) by nsteps
+ * classical RK4 steps of dt with acceleration -grad U of the 3D field's
+ * scalar interpolant (unit mass).  A particle whose stage position leaves
+ * the interpolation box becomes NaN and stays NaN.  The steps are issued as
+ * launches of steps_per_launch steps; with use_graph = 1 that launch
+ * sequence is captured once into a hipGraph (re-captured when the arguments
+ * change) and replayed on `stream`.  No reference counterpart (the
+ * reference's trajectory code was removed in v1.8, CHANGELOG.md:8). */
+int arb_push_rk4(arb_field* f, double* x_dev, double* v_dev, int64_t n, double dt,
+                 int64_t nsteps, int32_t steps_per_launch, int32_t use_graph, void* stream);
+
 /* Stage durations (ms, HIP events on the call's stream) of the last arb_query
  * made with ARB_Q_TIMING: [0] reduction + bucketing (whole kernel on the
  * direct path), [1] bucket scan, [2] scatter, [3] evaluation.  No reference

@@ -328,6 +328,88 @@ class _Interp:
             return norms, grads
         return comps, norms, grads
 
+    # -- derivatives for the backward pass (no reference counterpart) --------
+    def _second_basis(self, local, a, b):
+        """d^2 m_i / du_a du_b for every coefficient i, tile/repeat form."""
+        N = len(local)
+        d = self.NDIM
+        K = 4 ** d
+        one, zero = np.ones(N), np.zeros(N)
+
+        def expand(v, ax):
+            return np.tile(np.repeat(v, 4 ** ax, axis=1), K // 4 ** (ax + 1))
+
+        out = None
+        for ax in range(d):
+            u = local[:, ax]
+            if ax == a and ax == b:
+                v = np.transpose(np.array([zero, zero, 2 * one, 6 * u]))
+            elif ax == a or ax == b:
+                v = np.transpose(np.array([zero, one, 2 * u, 3 * u ** 2]))
+            else:
+                v = np.transpose(np.array([one, u, u ** 2, u ** 3]))
+            e = expand(v, ax)
+            out = e if out is None else out * e
+        return out
+
+    def hessian(self, query):
+        """Hessian of the scalar interpolant (physical units), (N, d, d)."""
+        d = self.NDIM
+        inds, local = self.cell_reduce(np.array(query, dtype=np.float64, copy=True))
+        self._fill(inds[inds < self.nc])
+        t = np.transpose(self.tables["n"][:, inds])
+        H = np.zeros((len(inds), d, d))
+        for a in range(d):
+            for b in range(a, d):
+                v = (t * self._second_basis(local, a, b)).sum(axis=1) / (self.p.h[a] * self.p.h[b])
+                H[:, a, b] = v
+                H[:, b, a] = v
+        return H
+
+    def backward(self, query, gv, gg):
+        """Adjoint of (value, gradient) of the scalar interpolant.
+
+        dq[i]   = gv_i grad U(x_i) + Hess U(x_i) . gg_i
+        dval[n] = sum_i sum_j c_ij A_{j, l(n)}, c_i = gv_i m(x_i) + sum_k gg_ik dm_k(x_i)/h_k,
+        scattered onto the nodes of each cell's neighbourhood (np.add.at).
+        Rows outside the box contribute nothing.  dval is in canonical node
+        order (the sorted field's row order)."""
+        d = self.NDIM
+        q = np.array(query, dtype=np.float64, copy=True)
+        inds, local = self.cell_reduce(q.copy())
+        ok = inds < self.nc
+        N = len(inds)
+        gv = np.asarray(gv, dtype=np.float64).reshape(N)
+        gg = np.asarray(gg, dtype=np.float64).reshape(N, d)
+        self._fill(inds[ok])
+        t = np.transpose(self.tables["n"][:, inds[ok]])
+        m, dm = self._basis(local[ok])
+        prod = m[0]
+        for a in range(1, d):
+            prod = prod * m[a]
+        grads = []
+        dbas = []
+        for a in range(d):
+            b = dm[a] if a == 0 else m[0]
+            for c in range(1, d):
+                b = b * (dm[c] if c == a else m[c])
+            dbas.append(b / self.p.h[a])
+            grads.append((t * dbas[-1]).sum(axis=1))
+        G = np.stack(grads, axis=1)
+        H = self.hessian(q[ok])
+        dq = np.full((N, d), np.nan)
+        dq[ok] = gv[ok, None] * G + np.einsum("nab,nb->na", H, gg[ok])
+        coef = gv[ok, None] * prod
+        for a in range(d):
+            coef = coef + gg[ok, a:a + 1] * dbas[a]
+        A = _A(d)
+        bbar = coef @ A                                    # (n_ok, K): sum_j c_j A_jl
+        off = neighbour_offsets(self.p.n, d)
+        nodes = self.p.base[inds[ok]][:, None] + off[None, :]
+        dval = np.zeros(len(self.p.field))
+        np.add.at(dval, nodes.reshape(-1), bbar.reshape(-1))
+        return dq, dval
+
     def allCoeffs(self):
         """ARBInterp.py:523-525 / :1260-1262."""
         step = 1 << 15

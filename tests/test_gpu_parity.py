@@ -214,7 +214,7 @@ def test_query_paths_bitwise_identical(dev, case):
     prec = "mixed" if "mixed" in case else "fp64"
     t = cls(rows, "quiet", precision=prec, **kw)
     if case == "3d_vector_tuned":
-        t.set_tuning((3, 2, 5), 128)
+        t.set_tuning((2, 2, 8), 128)
     d = t.NDIM
     lo = np.array([getattr(t, "xyzt"[a] + "IntMin") for a in range(d)])
     hi = np.array([getattr(t, "xyzt"[a] + "IntMax") for a in range(d)])
