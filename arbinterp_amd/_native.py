@@ -35,6 +35,7 @@ ARB_Q_DIRECT = 8
 ARB_Q_SORTED = 16
 ARB_Q_SORTED_GLOBAL = 32
 ARB_Q_TIMING = 64
+ARB_Q_GROUP_ATOMIC = 128
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64

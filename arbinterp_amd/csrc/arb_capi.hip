@@ -182,11 +182,18 @@ struct arb_field {
     int32_t* ws_bkt = nullptr;  // per query: bucket (-1 = handled in place)
     int32_t* ws_rank = nullptr; // per query: rank inside its bucket
     int32_t* ws_perm = nullptr; // bucket-sorted query indices
-    double* ws_qs = nullptr;    // positions in bucket order (n x ndim)
     int64_t ws_n = 0;
     int32_t* ws_count = nullptr;
     int32_t* ws_offs = nullptr;
     int64_t ws_nb = 0;
+    // radix-sort workspace: keys/values x2, digit histograms, scan temporaries
+    uint32_t* rs_k[2] = {nullptr, nullptr};
+    int32_t* rs_v[2] = {nullptr, nullptr};
+    int32_t* rs_hist = nullptr;
+    int32_t* rs_hscan = nullptr;
+    int32_t* rs_tmp = nullptr;
+    int32_t* rs_tmps = nullptr;
+    int64_t rs_n = 0;
     // backward workspace
     int32_t* bw_key = nullptr;     // per query: cell (-1 outside)
     int32_t* bw_perm = nullptr;    // queries grouped per cell, index order
@@ -261,13 +268,43 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         HIPCHK(hipMalloc(&f->ws_bkt, sizeof(int32_t) * n));
         HIPCHK(hipMalloc(&f->ws_rank, sizeof(int32_t) * n));
         HIPCHK(hipMalloc(&f->ws_perm, sizeof(int32_t) * n));
-        if (f->ws_qs) (void)hipFree(f->ws_qs);
-        f->ws_qs = nullptr;
-        HIPCHK(hipMalloc(&f->ws_qs, sizeof(double) * f->g.ndim * n));
         f->ws_n = n;
     }
+    if (n > f->rs_n) {
+        for (int k = 0; k < 2; ++k) {
+            if (f->rs_k[k]) (void)hipFree(f->rs_k[k]);
+            if (f->rs_v[k]) (void)hipFree(f->rs_v[k]);
+            f->rs_k[k] = nullptr;
+            f->rs_v[k] = nullptr;
+        }
+        if (f->rs_hist) (void)hipFree(f->rs_hist);
+        if (f->rs_hscan) (void)hipFree(f->rs_hscan);
+        if (f->rs_tmp) (void)hipFree(f->rs_tmp);
+        if (f->rs_tmps) (void)hipFree(f->rs_tmps);
+        f->rs_hist = f->rs_hscan = f->rs_tmp = f->rs_tmps = nullptr;
+        f->rs_n = 0;
+        const int64_t hl = 256 * arb_rs_ntiles(n);
+        const int64_t tl = arb_scan_tmp_elems(hl) + 1;
+        for (int k = 0; k < 2; ++k) {
+            HIPCHK(hipMalloc(&f->rs_k[k], sizeof(uint32_t) * n));
+            HIPCHK(hipMalloc(&f->rs_v[k], sizeof(int32_t) * n));
+        }
+        HIPCHK(hipMalloc(&f->rs_hist, sizeof(int32_t) * hl));
+        HIPCHK(hipMalloc(&f->rs_hscan, sizeof(int32_t) * (hl + 1)));
+        HIPCHK(hipMalloc(&f->rs_tmp, sizeof(int32_t) * tl));
+        HIPCHK(hipMalloc(&f->rs_tmps, sizeof(int32_t) * tl));
+        f->rs_n = n;
+    }
     if (f->B.nbt + 1 > f->ws_nb) {
-        if (f->ws_count) (void)hipFree(f->ws_count);
+        for (int k = 0; k < 2; ++k) {
+        if (f->rs_k[k]) (void)hipFree(f->rs_k[k]);
+        if (f->rs_v[k]) (void)hipFree(f->rs_v[k]);
+    }
+    if (f->rs_hist) (void)hipFree(f->rs_hist);
+    if (f->rs_hscan) (void)hipFree(f->rs_hscan);
+    if (f->rs_tmp) (void)hipFree(f->rs_tmp);
+    if (f->rs_tmps) (void)hipFree(f->rs_tmps);
+    if (f->ws_count) (void)hipFree(f->ws_count);
         if (f->ws_offs) (void)hipFree(f->ws_offs);
         f->ws_count = f->ws_offs = nullptr;
         f->ws_nb = 0;
@@ -477,10 +514,7 @@ int arb_build_coeffs(arb_field* f, void* stream) {
     const int K = 1 << (2 * f->g.ndim);
     const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
     if (f->precision == ARB_PREC_FP64) {
-        if (f->g.ndim == 3)
-            HIPCHK(arb_launch_coef_f64mfma(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
-        else
-            HIPCHK(arb_launch_coef_fp64(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
+        HIPCHK(arb_launch_coef_f64mfma(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
     } else {
         HIPCHK(arb_launch_coef_bf16x3(f->g, f->vals, f->ninterp, f->Ab, (float*)f->table, s));
     }
@@ -536,25 +570,42 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (f->B.nbt == 0) choose_buckets(f);
         rc = ensure_workspace(f, n);
         if (rc) return rc;
+        const int32_t* perm = f->ws_perm;
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
-        HIPCHK(arb_launch_fill_i32(f->ws_count, f->B.nbt, 0, s));
-        HIPCHK(arb_launch_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
-                                 f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_bkt,
-                                 f->ws_rank, f->ws_count, s));
-        if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
-        HIPCHK(arb_launch_scan(f->ws_count, f->ws_offs, f->B.nbt, s));
-        if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
-        HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, q_dev, ldq,
-                                  f->g.ndim, f->ws_qs, n, s));
+        if (flags & ARB_Q_GROUP_ATOMIC) {
+            HIPCHK(arb_launch_fill_i32(f->ws_count, f->B.nbt, 0, s));
+            HIPCHK(arb_launch_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+                                     f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_bkt,
+                                     f->ws_rank, f->ws_count, s));
+            if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
+            HIPCHK(arb_launch_scan(f->ws_count, f->ws_offs, f->B.nbt, s));
+            if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
+            HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, n, s));
+        } else {
+            HIPCHK(arb_launch_keys(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+                                   f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
+                                   f->rs_v[0], s));
+            if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
+            int bits = 1;
+            while (((int64_t)1 << bits) <= f->B.nbt) ++bits;      // keys are 0..nbt
+            int in_first = 1;
+            HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
+                                         f->rs_hist, f->rs_hscan, f->rs_tmp, f->rs_tmps,
+                                         &in_first, s));
+            if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
+            const int w = in_first ? 0 : 1;
+            HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, s));
+            perm = f->rs_v[w];
+        }
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));
         if (flags & ARB_Q_SORTED_GLOBAL)
             HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
-                                        f->ws_perm, s));
+                                        perm, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
-                                          f->has_comps, f->has_norm, f->ws_qs, o, f->ws_offs,
-                                          f->ws_perm, f->eval_threads, s));
+                                          f->has_comps, f->has_norm, q_dev, ldq, o, f->ws_offs,
+                                          perm, f->eval_threads, s));
         if (timing) {
             HIPCHK(hipEventRecord(f->ev[4], s));
             HIPCHK(hipEventSynchronize(f->ev[4]));
@@ -873,7 +924,6 @@ void arb_field_destroy(arb_field* f) {
     if (f->ws_bkt) (void)hipFree(f->ws_bkt);
     if (f->ws_rank) (void)hipFree(f->ws_rank);
     if (f->ws_perm) (void)hipFree(f->ws_perm);
-    if (f->ws_qs) (void)hipFree(f->ws_qs);
     if (f->ws_count) (void)hipFree(f->ws_count);
     if (f->ws_offs) (void)hipFree(f->ws_offs);
     if (f->bw_key) (void)hipFree(f->bw_key);

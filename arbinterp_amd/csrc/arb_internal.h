@@ -237,14 +237,26 @@ hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec
                              const ArbOut& o, int32_t* bkt, int32_t* rank, int32_t* count,
                              hipStream_t s);
 hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipStream_t s);
+hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
+                           const void* table, int32_t ninterp, int32_t has_comps,
+                           int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                           const ArbOut& o, uint32_t* key, int32_t* val, hipStream_t s);
+int64_t arb_scan_tmp_elems(int64_t len);
+hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
+                                int32_t* tmp_scan, hipStream_t s);
+int64_t arb_rs_ntiles(int64_t n);
+hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
+                                 int bits, int32_t* hist, int32_t* hscan, int32_t* tmp,
+                                 int32_t* tmp_scan, int* in_first, hipStream_t s);
+hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
+                             hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
-                              int32_t* perm, const double* q, int64_t ldq, int32_t ndim,
-                              double* qs, int64_t n, hipStream_t s);
+                              int32_t* perm, int64_t n, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
-                                  int32_t has_comps, int32_t has_norm, const double* qs,
-                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
-                                  int32_t block_threads, hipStream_t s);
+                                  int32_t has_comps, int32_t has_norm, const double* q,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                 int32_t out_f64, const void* table, int32_t ninterp,
                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,

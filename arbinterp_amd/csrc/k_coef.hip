@@ -12,21 +12,6 @@
 
 namespace {
 
-// Node id of neighbour j (x fastest over 4^d offsets -1..2) of cell c.
-template <int D>
-__device__ __forceinline__ int64_t nbr_node(const ArbGeom& g, int64_t c, int j) {
-    int64_t node = 0;
-    int64_t rem = c;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-        int64_t ca = rem % g.npos[a];
-        rem /= g.npos[a];
-        int oa = (j >> (2 * a)) & 3;        // 0..3  <->  offsets -1..2 from the base node ca+1
-        node += (ca + oa) * g.nstride[a];
-    }
-    return node;
-}
-
 template <int D, typename TO>
 __global__ void k_coef_ref(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
                            const double* __restrict__ A, TO* __restrict__ table) {
@@ -38,10 +23,21 @@ __global__ void k_coef_ref(ArbGeom g, const double* __restrict__ vals, int32_t n
     const int64_t ck = t / K;
     const int k = (int)(ck % ninterp);
     const int64_t c = ck / ninterp;
-    const double* F = vals + (int64_t)k * g.nnodes;
+    // base of the 4^D neighbourhood: node (cx, cy, cz[, ct]) = cell coordinates
+    int64_t base = 0, rem = c;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        base += (rem % g.npos[a]) * g.nstride[a];
+        rem /= g.npos[a];
+    }
+    const double* F = vals + (int64_t)k * g.nnodes + base;
     const double* Ai = A + (int64_t)i * K;
     double acc = 0.0;
-    for (int j = 0; j < K; ++j) acc = fma(Ai[j], F[nbr_node<D>(g, c, j)], acc);
+    for (int j = 0; j < K; ++j) {
+        int64_t off = (j & 3) + ((j >> 2) & 3) * g.nstride[1] + ((j >> 4) & 3) * g.nstride[2];
+        if (D == 4) off += ((j >> 6) & 3) * g.nstride[3];
+        acc = fma(Ai[j], F[off], acc);
+    }
     table[t] = (TO)acc;
 }
 

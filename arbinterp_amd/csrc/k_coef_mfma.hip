@@ -270,6 +270,60 @@ k_coef_f64mfma3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
     }
 }
 
+// Plain FP64, 4D: alpha = A . F, K = 256.  One workgroup = 16 cells along x
+// (one row at fixed y, z, t); its 4 waves split the 16 row tiles (4 each)
+// and share the 19 x 4 x 4 x 4 node tile in LDS.  A operands stream from
+// L2 (512 KB, shared by every workgroup).
+__global__ void __launch_bounds__(256)
+k_coef_f64mfma4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
+                const double* __restrict__ A, double* __restrict__ table) {
+    constexpr int K = 256, TXN = TXC + 3;
+    constexpr int NODES = TXN * 4 * 4 * 4;
+    __shared__ double tile[NODES];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lx = lane & 15, kq = lane >> 4;
+    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
+    const int64_t cy = blockIdx.y;
+    const int64_t cz = blockIdx.z % g.npos[2], ct = blockIdx.z / g.npos[2];
+    const bool colvalid = (cx0 + lx) < g.npos[0];
+    const int lxc = colvalid ? lx : 0;
+    const int64_t cell = (cx0 + lx) + g.npos[0] * (cy + g.npos[1] * (cz + g.npos[2] * ct));
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();
+        for (int i = threadIdx.x; i < NODES; i += 256) {
+            const int x = i % TXN;
+            const int y = (i / TXN) & 3;
+            const int z = (i / (TXN * 4)) & 3;
+            const int tt = i / (TXN * 16);
+            const int64_t nx = cx0 + x;
+            double v = 0.0;
+            if (nx < g.n[0])
+                v = F[nx + g.nstride[1] * (cy + y) + g.nstride[2] * (cz + z) + g.nstride[3] * (ct + tt)];
+            tile[i] = v;
+        }
+        __syncthreads();
+        double* out = table + (cell * ninterp + k) * K;
+        for (int mq = 0; mq < 4; ++mq) {
+            const int mt = w * 4 + mq;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            const double* Arow = A + (int64_t)(mt * 16 + lx) * K + kq;
+#pragma unroll 8
+            for (int ks = 0; ks < 64; ++ks) {
+                const int j = ks * 4 + kq;
+                const int ox = j & 3, oy = (j >> 2) & 3, oz = (j >> 4) & 3, ot = j >> 6;
+                const double b = tile[((ot * 4 + oz) * 4 + oy) * TXN + (lxc + ox)];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Arow[ks * 4], b, acc, 0, 0, 0);
+            }
+            if (colvalid) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) out[mt * 16 + kq + 4 * r] = acc[r];
+            }
+        }
+    }
+}
+
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace
@@ -277,9 +331,13 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t ninterp,
                                    const double* A, double* table, hipStream_t s) {
     if (g.nc == 0) return hipSuccess;
-    if (g.ndim != 3) return hipErrorInvalidValue;
-    dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]), block(256);
-    hipLaunchKernelGGL(k_coef_f64mfma3, grid, block, 0, s, g, vals, ninterp, A, table);
+    if (g.ndim == 3) {
+        dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]), block(256);
+        hipLaunchKernelGGL(k_coef_f64mfma3, grid, block, 0, s, g, vals, ninterp, A, table);
+    } else {
+        dim3 grid(cdiv(g.npos[0], TXC), (unsigned)g.npos[1], (unsigned)(g.npos[2] * g.npos[3]));
+        hipLaunchKernelGGL(k_coef_f64mfma4, grid, dim3(256), 0, s, g, vals, ninterp, A, table);
+    }
     return hipGetLastError();
 }
 

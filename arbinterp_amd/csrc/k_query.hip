@@ -158,6 +158,36 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
     rank[i] = atomicAdd(&count[b], 1);
 }
 
+// Reduction + bucket key for the radix-sorted path (no atomics): key = bucket,
+// or nbt for rows evaluated here from the sentinel row; value = row index.
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+       int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
+       uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[D];
+    int64_t ia[D];
+    bool oob;
+    int64_t flat = arb_reduce_axes<D>(q + i * ldq, g, loc, ia, &oob);
+    if (o.cells) o.cells[i] = flat;
+    if (oob && o.qmut) {
+        for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
+    }
+    val[i] = (int32_t)i;
+    if (flat >= g.nc) {
+        if (flat > g.nc) o.flag[0] = 1;
+        eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
+                                 o, i);
+        key[i] = (uint32_t)B.nbt;
+        return;
+    }
+    int lc;
+    key[i] = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
+}
+
 // Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024
 // threads: each thread keeps 64 consecutive counts in registers per pass, so
 // a pass costs one round of independent loads and one of stores.
@@ -195,29 +225,22 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count
     if (t == 0) offs[nb] = carry;
 }
 
-// Scatter: query index and a copy of its position into bucket order, so the
-// evaluation kernel reads both contiguously.
-template <int D>
+// Scatter: query index into bucket order.
 __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
-                          const int32_t* __restrict__ offs, int32_t* __restrict__ perm,
-                          const double* __restrict__ q, int64_t ldq, double* __restrict__ qs,
-                          int64_t n) {
+                          const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t b = bkt[i];
     if (b < 0) return;
-    const int64_t pos = offs[b] + rank[i];
-    perm[pos] = (int32_t)i;
-#pragma unroll
-    for (int a = 0; a < D; ++a) qs[pos * D + a] = q[i * ldq + a];
+    perm[offs[b] + rank[i]] = (int32_t)i;
 }
 
 // One workgroup per bucket: stage the block's coefficients in LDS, then
-// evaluate the bucket's queries (contiguous in qs/perm) from LDS.
+// evaluate the bucket's queries from LDS.
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
 k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has_comps,
-              int32_t has_norm, const double* __restrict__ qs, ArbOut o,
+              int32_t has_norm, const double* __restrict__ q, int64_t ldq, ArbOut o,
               const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
     extern __shared__ uint4 smem[];
     const int64_t b = blockIdx.x;
@@ -239,31 +262,62 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     }
     const uint4* __restrict__ tv = reinterpret_cast<const uint4*>(table);
     const int total = B.ncell * B.cellvec;
-    for (int v = threadIdx.x; v < total; v += blockDim.x) {
-        const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
-        const int j = v - lc * B.cellvec;
-        bool inside = true;
-        int64_t gcell = 0;
+    // this thread's first query: issue its loads before the staging so the
+    // latencies overlap
+    int32_t s = start + threadIdx.x;
+    int64_t qi = 0;
+    double qp[D];
+    if (s < end) {
+        qi = perm[s];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
-            inside &= la < ext[a];
-            gcell += (c0[a] + la) * cstr[a];
+        for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
+    }
+    // stage the block: up to 8 independent 16-byte loads in flight per thread
+    constexpr int BATCH = 8;
+    for (int v0 = threadIdx.x; v0 < total; v0 += BATCH * blockDim.x) {
+        uint4 r[BATCH];
+        int dst[BATCH];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u) {
+            const int v = v0 + u * blockDim.x;
+            dst[u] = -1;
+            if (v < total) {
+                const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
+                const int j = v - lc * B.cellvec;
+                bool inside = true;
+                int64_t gcell = 0;
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
+                    inside &= la < ext[a];
+                    gcell += (c0[a] + la) * cstr[a];
+                }
+                if (inside) {
+                    r[u] = tv[gcell * B.cellvec + j];
+                    dst[u] = lc * B.cellstride + j;
+                }
+            }
         }
-        if (inside) smem[lc * B.cellstride + j] = tv[gcell * B.cellvec + j];
+#pragma unroll
+        for (int u = 0; u < BATCH; ++u)
+            if (dst[u] >= 0) smem[dst[u]] = r[u];
     }
     __syncthreads();
-    for (int32_t s = start + threadIdx.x; s < end; s += blockDim.x) {
-        const int64_t i = perm[s];
+    for (; s < end; s += blockDim.x) {
+        if (s != start + (int32_t)threadIdx.x) {
+            qi = perm[s];
+#pragma unroll
+            for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
+        }
         double loc[D];
         int64_t ia[D];
         bool oob;
-        const int64_t flat = arb_reduce_axes<D>(qs + (int64_t)s * D, g, loc, ia, &oob);
+        const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
         int lc;
         const int64_t bb = bucket_of<D>(flat, ia, g, B, &lc);
         if (bb != b || flat >= g.nc) continue;      // cannot happen: same reduction as k_bucket
         const TC* a = reinterpret_cast<const TC*>(smem + lc * B.cellstride);
-        eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, i);
+        eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, qi);
     }
 }
 
@@ -354,29 +408,23 @@ hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipS
 }
 
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
-                              int32_t* perm, const double* q, int64_t ldq, int32_t ndim,
-                              double* qs, int64_t n, hipStream_t s) {
+                              int32_t* perm, int64_t n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (ndim == 3)
-        hipLaunchKernelGGL(k_scatter<3>, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs,
-                           perm, q, ldq, qs, n);
-    else
-        hipLaunchKernelGGL(k_scatter<4>, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs,
-                           perm, q, ldq, qs, n);
+    hipLaunchKernelGGL(k_scatter, dim3(grid1(n, 256)), dim3(256), 0, s, bkt, rank, offs, perm, n);
     return hipGetLastError();
 }
 
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
-                                  int32_t has_comps, int32_t has_norm, const double* qs,
-                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
-                                  int32_t block_threads, hipStream_t s) {
+                                  int32_t has_comps, int32_t has_norm, const double* q,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
     (void)ninterp;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16;
     dim3 grid((unsigned)B.nbt), block(block_threads);
 #define L_EVALB(DD, T, TC, TO)                                                              \
     hipLaunchKernelGGL((k_eval_bucket<DD, T, TC, TO>), grid, block, lds, s, g, B,             \
-                       (const TC*)table, has_comps, has_norm, qs, o, offs, perm)
+                       (const TC*)table, has_comps, has_norm, q, ldq, o, offs, perm)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
 #undef L_EVALB
@@ -396,5 +444,19 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALP);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALP);
 #undef L_EVALP
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
+                           const void* table, int32_t ninterp, int32_t has_comps,
+                           int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                           const ArbOut& o, uint32_t* key, int32_t* val, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+#define L_KEYS(DD, T, TC, TO)                                                                 \
+    hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3(grid1(n, 256)), dim3(256), 0, s, g, B,      \
+                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, key, val)
+    if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
+    else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
+#undef L_KEYS
     return hipGetLastError();
 }

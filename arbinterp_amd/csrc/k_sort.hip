@@ -1,0 +1,214 @@
+// Stable LSD radix sort of (key, value) pairs with 8-bit digits, and the
+// helpers the bucketed query path needs around it.  Used to group queries by
+// bucket (block of cells) without device-scope atomics.
+//
+// One pass over digit d:
+//   upsweep   each tile of 4096 pairs counts its digits in LDS and writes
+//             hist[digit * ntiles + tile];
+//   scan      exclusive scan of hist (digit-major, so the result is the
+//             global start of every (digit, tile) run);
+//   downsweep each tile ranks its pairs stably (wave ballots match equal
+//             digits, per-wave LDS counters keep element order) and writes
+//             them to hist_scan[digit, tile] + rank.
+// Tiles are 256 threads x 16 items; items are striped inside each wave
+// (item j of lane l of wave w is element w*1024 + j*64 + l), which keeps
+// the ranking stable in element order.
+#include "arb_internal.h"
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096
+constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* __restrict__ hist,
+             int64_t ntiles) {
+    __shared__ int32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = base + j * RS_THREADS + threadIdx.x;
+        if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+               uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
+               const int32_t* __restrict__ hscan, int64_t ntiles) {
+    __shared__ int32_t wcnt[RS_WAVES][256];     // running digit counts per wave
+    __shared__ int32_t gbase[256];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = threadIdx.x; d < RS_WAVES * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
+    gbase[threadIdx.x] = hscan[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t key[RS_ITEMS];
+    int32_t val[RS_ITEMS];
+    int32_t rnk[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = base + j * 64 + lane;
+        const bool ok = e < n;
+        key[j] = ok ? kin[e] : 0xFFFFFFFFu;
+        val[j] = ok ? vin[e] : 0;
+        const uint32_t dg = (key[j] >> shift) & 255u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const uint64_t on = __ballot(ok && ((dg >> b) & 1u));
+            peers &= ((dg >> b) & 1u) ? on : ~on;
+        }
+        // peers: lanes of this item slot holding the same digit (all valid)
+        const int before = __popcll(peers & lt);
+        const int run = wcnt[w][dg];
+        rnk[j] = run + before;
+        if (ok && before == 0) wcnt[w][dg] = run + __popcll(peers);   // group leader
+    }
+    __syncthreads();
+    // exclusive prefix over the waves of this tile, per digit
+    if (threadIdx.x < 256) {
+        int32_t acc = 0;
+#pragma unroll
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const int32_t c = wcnt[ww][threadIdx.x];
+            wcnt[ww][threadIdx.x] = acc;
+            acc += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = base + j * 64 + lane;
+        if (e >= n) continue;
+        const uint32_t dg = (key[j] >> shift) & 255u;
+        const int64_t pos = (int64_t)gbase[dg] + wcnt[w][dg] + rnk[j];
+        if (pos >= n) continue;               // unreachable: ranks stay inside the tile's run
+        kout[pos] = key[j];
+        vout[pos] = val[j];
+    }
+}
+
+// Exclusive scan of len int32 values (in -> out, out[len] = total) in three
+// launches: block sums, scan of the block sums, block-local scans.
+constexpr int SC_THREADS = 256, SC_ITEMS = 16, SC_BLOCK = SC_THREADS * SC_ITEMS;
+
+__global__ void __launch_bounds__(SC_THREADS)
+k_sc_sums(const int32_t* __restrict__ in, int64_t len, int32_t* __restrict__ sums) {
+    __shared__ int32_t red[SC_THREADS];
+    const int64_t base = (int64_t)blockIdx.x * SC_BLOCK;
+    int32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; ++j) {
+        const int64_t e = base + j * SC_THREADS + threadIdx.x;
+        if (e < len) s += in[e];
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = SC_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(SC_THREADS)
+k_sc_local(const int32_t* __restrict__ in, int64_t len, const int32_t* __restrict__ bscan,
+           int32_t* __restrict__ out) {
+    __shared__ int32_t part[SC_THREADS];
+    const int64_t base = (int64_t)blockIdx.x * SC_BLOCK + (int64_t)threadIdx.x * SC_ITEMS;
+    int32_t v[SC_ITEMS];
+    int32_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; ++j) {
+        v[j] = (base + j < len) ? in[base + j] : 0;
+        s += v[j];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 1; off < SC_THREADS; off <<= 1) {
+        const int32_t x = (threadIdx.x >= off) ? part[threadIdx.x - off] : 0;
+        __syncthreads();
+        part[threadIdx.x] += x;
+        __syncthreads();
+    }
+    int32_t run = bscan[blockIdx.x] + part[threadIdx.x] - s;
+#pragma unroll
+    for (int j = 0; j < SC_ITEMS; ++j) {
+        if (base + j < len) out[base + j] = run;
+        run += v[j];
+    }
+}
+
+// offs[b] = first sorted position whose key >= b, for b in [0, nb]; keys are
+// sorted, sentinel rows carry key nb and sort last.
+__global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ offs,
+                         int64_t nb) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > n) return;
+    const int64_t kp = (s == 0) ? -1 : (int64_t)keys[s - 1];
+    int64_t k = (s == n) ? nb : (int64_t)keys[s];
+    if (k > nb) k = nb;
+    for (int64_t b = kp + 1; b <= k; ++b) offs[b] = (int32_t)s;
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+int64_t arb_scan_tmp_elems(int64_t len) { return (int64_t)cdiv(len, SC_BLOCK) + 1; }
+
+// out[0..len] exclusive scan of in[0..len); tmp holds arb_scan_tmp_elems(len)
+// ints (block sums, scanned in place).
+hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
+                                int32_t* tmp_scan, hipStream_t s) {
+    const unsigned nblk = cdiv(len, SC_BLOCK);
+    if (len <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sc_sums, dim3(nblk), dim3(SC_THREADS), 0, s, in, len, tmp);
+    hipError_t e = arb_launch_scan(tmp, tmp_scan, nblk, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sc_local, dim3(nblk), dim3(SC_THREADS), 0, s, in, len, tmp_scan, out);
+    return hipGetLastError();
+}
+
+int64_t arb_rs_ntiles(int64_t n) { return (int64_t)cdiv(n, RS_TILE); }
+
+// Sort (k0, v0) by the low `bits` bits of the keys.  Buffers: k1/v1 the same
+// size as k0/v0; hist and hscan 256 * ntiles ints; tmp/tmp_scan for the scan.
+// The sorted pairs end in (k0, v0) when the number of passes is even, else in
+// (k1, v1); *in_first reports which.
+hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
+                                 int bits, int32_t* hist, int32_t* hscan, int32_t* tmp,
+                                 int32_t* tmp_scan, int* in_first, hipStream_t s) {
+    const int64_t ntiles = arb_rs_ntiles(n);
+    uint32_t* ka = k0;
+    int32_t* va = v0;
+    uint32_t* kb = k1;
+    int32_t* vb = v1;
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8, ++passes) {
+        hipLaunchKernelGGL(k_rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n,
+                           shift, hist, ntiles);
+        hipError_t e = arb_launch_scan_long(hist, hscan, 256 * ntiles, tmp, tmp_scan, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va,
+                           kb, vb, n, shift, hscan, ntiles);
+        uint32_t* tk = ka; ka = kb; kb = tk;
+        int32_t* tv = va; va = vb; vb = tv;
+    }
+    *in_first = (passes % 2 == 0);
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, s, keys, n, offs, nb);
+    return hipGetLastError();
+}

@@ -42,8 +42,17 @@ def parse():
     ap.add_argument("--err-sample", type=int, default=200_000)
     ap.add_argument("--flags", type=int, default=0, help="extra ARB_Q_* flags")
     ap.add_argument("--variants", default="", help="comma list of query paths to time "
-                    "(auto,direct,sorted_lds,sorted_global); default all")
+                    "(auto,direct,sorted_lds,sorted_atomic,sorted_global); default all")
     ap.add_argument("--timing-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run), then exit")
+    ap.add_argument("--workload", default="dipole3d",
+                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield"],
+                    help="dipole3d: the headline JSON line; the others print their own line")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only the mixed-precision default path: build, warm up, --steps queries "
+                         "(for counter runs; no oracle, no CPU baseline, no JSON line)")
     return ap.parse_args()
 
 
@@ -75,6 +84,31 @@ def main():
     from arbinterp_amd import workloads as W
     from arbinterp_amd.ARBInterp import tricubic
 
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes 5 times" % nbytes)
+        return
+    if args.profile_only:
+        n = (args.grid,) * 3
+        rows, q = W.dipole3d(seed=2, n=n, nq=args.nq)
+        qd = torch.from_numpy(q).cuda()
+        t = tricubic(rows, "quiet", mode="both", precision="mixed")
+        t.allCoeffs()
+        out = t.query_device(qd)
+        for _ in range(args.steps):
+            t.query_device(qd, out=out, flags=args.flags)
+        torch.cuda.synchronize()
+        log("profile-only: %d steps done" % args.steps)
+        return
+
+    if args.workload != "dipole3d":
+        return OTHER[args.workload](args, torch)
+
     n = (args.grid,) * 3
     t0 = time.time()
     rows, q = W.dipole3d(seed=2, n=n, nq=args.nq)
@@ -83,6 +117,7 @@ def main():
 
     from arbinterp_amd import _native as NN
     variants = {"auto": 0, "direct": NN.ARB_Q_DIRECT, "sorted_lds": NN.ARB_Q_SORTED,
+                "sorted_atomic": NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC,
                 "sorted_global": NN.ARB_Q_SORTED_GLOBAL}
     if args.variants:
         variants = {k: v for k, v in variants.items() if k in args.variants.split(",")}
@@ -198,6 +233,142 @@ def main():
         "algorithmic_bytes_per_step": alg_bytes,
     }
     print(json.dumps(line), flush=True)
+
+
+def _events_ms(torch, fn, steps, warmup):
+    return time_steps(torch, fn, steps, warmup)
+
+
+def bench_scalar3d(args, torch):
+    """24x20x16 scalar field, 1e4 queries over the full box: latency of one
+    batch query (device-resident and host numpy path) against the oracle."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import tricubic
+    from oracle import arbinterp_oracle as O
+    rows, q = W.scalar3d_gaussians(seed=1)
+    t = tricubic(rows, "quiet")
+    t.allCoeffs()
+    qd = torch.from_numpy(q).cuda()
+    out = t.query_device(qd)
+    dev_ms = _events_ms(torch, lambda: t.query_device(qd, out=out), args.steps, args.warmup)
+    t0 = time.time()
+    for _ in range(args.steps):
+        t.Query(q.copy())
+    host_ms = (time.time() - t0) * 1e3 / args.steps
+    o = O.tricubic(rows, "quiet")
+    o.allCoeffs()
+    t0 = time.time()
+    for _ in range(3):
+        o.Query(q.copy())
+    cpu_ms = (time.time() - t0) * 1e3 / 3
+    print(json.dumps({"workload": "scalar3d 24x20x16, 1e4 queries (full box)",
+                      "device_ms_per_query_call": dev_ms, "host_numpy_ms_per_call": host_ms,
+                      "cpu_oracle_ms_per_call": cpu_ms, "queries": len(q)}))
+
+
+def bench_moving4d(args, torch):
+    """48^3 x 16 scalar quadcubic, 2e6 interior queries, mixed and FP64."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import quadcubic
+    from oracle import arbinterp_oracle as O
+    rows, q = W.moving4d(seed=3)
+    qd = torch.from_numpy(q).cuda()
+    res = {}
+    for prec in ("mixed", "fp64"):
+        t = quadcubic(rows, "quiet", precision=prec)
+        torch.cuda.synchronize()
+        t0 = time.time()
+        t.allCoeffs()
+        build = (time.time() - t0) * 1e3
+        out = t.query_device(qd)
+        ms = _events_ms(torch, lambda: t.query_device(qd, out=out), args.steps, args.warmup)
+        res[prec] = {"ms_per_step": ms, "queries_per_s": len(q) / ms * 1e3, "table_build_ms": build,
+                     "out": out}
+    idx = np.random.default_rng(8).choice(len(q), 20000, replace=False)
+    o = O.quadcubic(rows, "quiet")
+    ref = o.Query(q[idx].copy())
+    sc = o.condition_scale(q[idx])
+    errs = {}
+    for prec in ("mixed", "fp64"):
+        _, nv, g = (x.double().cpu().numpy() if x is not None else None for x in res[prec].pop("out"))
+        errs[prec] = {"value": O.rel_err_normwise(nv[idx], ref[0][:, 0], sc[0][:, 0]),
+                      "grad": O.rel_err_normwise(g[idx], ref[1], sc[1])}
+    print(json.dumps({"workload": "quadcubic moving Gaussians 48^3x16, 2e6 queries",
+                      "results": res, "rel_err_normwise_vs_oracle": errs}))
+
+
+def bench_smooth3d(args, torch):
+    """96^3 smooth scalar field, 2^20 FP32 positions: forward + backward of
+    L = sum w U + sum |grad U|^2 through the autograd function."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import tricubic
+    from arbinterp_amd import autograd as AG
+    rows, q, w = W.smooth3d(seed=4)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        t = tricubic(rows, "quiet", precision=prec)
+        fld = AG.DifferentiableField(t, dtype=torch.float32)
+        x = torch.from_numpy(q).cuda().requires_grad_(True)
+        wt = torch.from_numpy(w).cuda()
+
+        def step():
+            x.grad = None
+            fld.values.grad = None
+            U, G = fld(x)
+            (wt * U + (G * G).sum(dim=1)).sum().backward()
+        ms = _events_ms(torch, step, args.steps, args.warmup)
+        res[prec] = {"ms_per_forward_backward": ms, "queries": len(q)}
+    print(json.dumps({"workload": "autograd 96^3 scalar, 2^20 FP32 positions", "results": res}))
+
+
+def bench_multifield(args, torch):
+    """64 uneven scalar fields in one persistent launch, then the RK4 push of
+    1e6 particles for 1000 steps (dt = 1e-3) in field 0."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import tricubic
+    from arbinterp_amd import batch as B
+    fields = W.multi_field(seed=5)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        interps = [tricubic(r, "quiet", precision=prec) for r, _ in fields]
+        counts = [len(q) for _, q in fields]
+        qd = torch.from_numpy(np.concatenate([q for _, q in fields])).cuda()
+        bt = B.FieldBatch(interps)
+        out = bt.query(qd, counts)
+        ms = _events_ms(torch, lambda: bt.query(qd, counts, out=out), args.steps, args.warmup)
+        # the push uses field 0 with its sign flipped: the Gaussian bumps become
+        # wells, so that under a = -grad U the particles stay bound
+        well = fields[0][0].copy()
+        well[:, 3] *= -1.0
+        t0 = tricubic(well, "quiet", precision=prec)
+        lo = np.array([t0.xIntMin, t0.yIntMin, t0.zIntMin])
+        hi = np.array([t0.xIntMax, t0.yIntMax, t0.zIntMax])
+        rng = np.random.default_rng(11)
+        x0 = lo + (hi - lo) * (0.25 + 0.5 * rng.uniform(size=(1_000_000, 3)))
+        x = torch.from_numpy(x0).cuda()
+        v = torch.zeros_like(x)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        B.push_rk4(t0, x, v, dt=1e-3, nsteps=1000, steps_per_launch=50, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        push_ms = e0.elapsed_time(e1)
+        res[prec] = {"batch_ms": ms, "batch_queries": int(sum(counts)),
+                     "batch_queries_per_s": sum(counts) / ms * 1e3, "push_ms_1000_steps": push_ms,
+                     "push_steps_per_s": 1000 / push_ms * 1e3,
+                     "push_evals_per_s": 4 * 1000 * 1e6 / push_ms * 1e3,
+                     "lost_particles": int(torch.isnan(x[:, 0]).sum())}
+        res[prec]["final_x"] = x.cpu().numpy()
+    d = res["mixed"].pop("final_x") - res["fp64"].pop("final_x")
+    m = ~np.isnan(d).any(axis=1)
+    print(json.dumps({"workload": "64 uneven fields, batched; RK4 push 1e6 particles x 1000 steps",
+                      "results": res, "push_mixed_vs_fp64_max_abs_dx": float(np.abs(d[m]).max())}))
+
+
+OTHER = {"scalar3d": bench_scalar3d, "moving4d": bench_moving4d, "smooth3d": bench_smooth3d,
+         "multifield": bench_multifield}
 
 
 if __name__ == "__main__":

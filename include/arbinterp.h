@@ -74,6 +74,8 @@ extern "C" {
 #define ARB_Q_SORTED_GLOBAL 32 /* bucketed order, coefficients read from HBM/L2  */
 #define ARB_Q_TIMING   64   /* record per-stage HIP events; the call then waits
                                for its own work (see arb_last_timings)           */
+#define ARB_Q_GROUP_ATOMIC 128 /* bucketed path: group queries with per-bucket
+                               atomic ranks instead of the radix sort            */
 
 typedef struct arb_field arb_field;
 

@@ -228,10 +228,11 @@ def test_query_paths_bitwise_identical(dev, case):
     qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
     outs = {}
     for name, fl in (("direct", NN.ARB_Q_DIRECT), ("lds", NN.ARB_Q_SORTED),
+                     ("atomic", NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC),
                      ("global", NN.ARB_Q_SORTED_GLOBAL)):
         outs[name] = t.query_device(qd.clone(), cells=True, flags=fl)
     torch.cuda.synchronize()
-    for name in ("lds", "global"):
+    for name in ("lds", "atomic", "global"):
         for a, b in zip(outs["direct"], outs[name]):
             if a is None:
                 assert b is None
