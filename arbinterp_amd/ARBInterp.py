@@ -194,6 +194,39 @@ class _Interp:
         self._info.table_built = 1
         return out.cpu().numpy()
 
+    def _table_meta(self):
+        d = self.NDIM
+        info = self._info
+        return np.array([d, info.mode, info.precision, info.ninterp]
+                        + [info.n[a] for a in range(d)], dtype=np.int64), \
+            np.array([info.h[a] for a in range(d)] + [info.int_min[a] for a in range(d)]
+                     + [info.int_max[a] for a in range(d)], dtype=np.float64)
+
+    def save_coefficients(self, path):
+        """Write the whole coefficient table (built if needed) with the grid
+        constants it belongs to, as an .npz file."""
+        imeta, fmeta = self._table_meta()
+        np.savez(path, table=self.coefficient_table(), imeta=imeta, fmeta=fmeta,
+                 version=np.array(__version__))
+
+    def load_coefficients(self, path):
+        """Load a table written by save_coefficients for the same grid, mode and
+        precision (checked exactly); the field's own table is not rebuilt."""
+        torch = _torch()
+        z = np.load(path, allow_pickle=False)
+        imeta, fmeta = self._table_meta()
+        if not (np.array_equal(z["imeta"], imeta) and np.array_equal(z["fmeta"], fmeta)):
+            raise ValueError("coefficient file belongs to a different grid, mode or precision")
+        tab = z["table"]
+        dt = torch.float64 if self.precision == "fp64" else torch.float32
+        src = torch.from_numpy(np.ascontiguousarray(tab)).to(device="cuda", dtype=dt)
+        if src.shape[0] != self.nc + 1:
+            raise ValueError("coefficient file has %d cells, field has %d" % (src.shape[0] - 1, self.nc))
+        N.check(N.lib().arb_table_load(self._h, 0, self.nc + 1, ctypes.c_void_p(src.data_ptr()),
+                                       _stream_handle(torch)), "load")
+        torch.cuda.current_stream().synchronize()
+        self._info.table_built = 1
+
     # -- queries ---------------------------------------------------------------
     def _alloc_out(self, torch, n, dtype):
         d = self.NDIM

@@ -296,15 +296,7 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         f->rs_n = n;
     }
     if (f->B.nbt + 1 > f->ws_nb) {
-        for (int k = 0; k < 2; ++k) {
-        if (f->rs_k[k]) (void)hipFree(f->rs_k[k]);
-        if (f->rs_v[k]) (void)hipFree(f->rs_v[k]);
-    }
-    if (f->rs_hist) (void)hipFree(f->rs_hist);
-    if (f->rs_hscan) (void)hipFree(f->rs_hscan);
-    if (f->rs_tmp) (void)hipFree(f->rs_tmp);
-    if (f->rs_tmps) (void)hipFree(f->rs_tmps);
-    if (f->ws_count) (void)hipFree(f->ws_count);
+        if (f->ws_count) (void)hipFree(f->ws_count);
         if (f->ws_offs) (void)hipFree(f->ws_offs);
         f->ws_count = f->ws_offs = nullptr;
         f->ws_nb = 0;
@@ -604,8 +596,8 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                         perm, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
-                                          f->has_comps, f->has_norm, q_dev, ldq, o, f->ws_offs,
-                                          perm, f->eval_threads, s));
+                                          f->has_comps, f->has_norm, q_dev, n, ldq, o,
+                                          f->ws_offs, perm, f->eval_threads, s));
         if (timing) {
             HIPCHK(hipEventRecord(f->ev[4], s));
             HIPCHK(hipEventSynchronize(f->ev[4]));
@@ -914,6 +906,19 @@ int arb_table_copy(arb_field* f, int64_t cell0, int64_t ncells, void* dst_dev, v
     return ARB_OK;
 }
 
+int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_dev,
+                   void* stream) {
+    if (!f || !src_dev || cell0 < 0 || ncells < 0 || cell0 + ncells > f->g.nc + 1)
+        return fail(ARB_EINVAL, "arb_table_load: bad range");
+    const int K = 1 << (2 * f->g.ndim);
+    const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const size_t row = esz * (size_t)f->ninterp * K;
+    HIPCHK(hipMemcpyAsync((char*)f->table + row * cell0, src_dev, row * ncells,
+                          hipMemcpyDeviceToDevice, (hipStream_t)stream));
+    if (cell0 == 0 && ncells == f->g.nc + 1) f->table_built = 1;
+    return ARB_OK;
+}
+
 void arb_field_destroy(arb_field* f) {
     if (!f) return;
     if (f->vals) (void)hipFree(f->vals);
@@ -926,6 +931,14 @@ void arb_field_destroy(arb_field* f) {
     if (f->ws_perm) (void)hipFree(f->ws_perm);
     if (f->ws_count) (void)hipFree(f->ws_count);
     if (f->ws_offs) (void)hipFree(f->ws_offs);
+    for (int k = 0; k < 2; ++k) {
+        if (f->rs_k[k]) (void)hipFree(f->rs_k[k]);
+        if (f->rs_v[k]) (void)hipFree(f->rs_v[k]);
+    }
+    if (f->rs_hist) (void)hipFree(f->rs_hist);
+    if (f->rs_hscan) (void)hipFree(f->rs_hscan);
+    if (f->rs_tmp) (void)hipFree(f->rs_tmp);
+    if (f->rs_tmps) (void)hipFree(f->rs_tmps);
     if (f->bw_key) (void)hipFree(f->bw_key);
     if (f->bw_perm) (void)hipFree(f->bw_perm);
     if (f->bw_count) (void)hipFree(f->bw_count);

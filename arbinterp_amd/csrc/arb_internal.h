@@ -255,7 +255,7 @@ hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q,
-                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  int64_t n, int64_t ldq, const ArbOut& o, const int32_t* offs,
                                   const int32_t* perm, int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                 int32_t out_f64, const void* table, int32_t ninterp,

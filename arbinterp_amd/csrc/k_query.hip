@@ -240,7 +240,7 @@ __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __rest
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
 k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has_comps,
-              int32_t has_norm, const double* __restrict__ q, int64_t ldq, ArbOut o,
+              int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
               const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
     extern __shared__ uint4 smem[];
     const int64_t b = blockIdx.x;
@@ -269,6 +269,7 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     double qp[D];
     if (s < end) {
         qi = perm[s];
+        if (qi < 0 || qi >= n) qi = 0;              // unreachable: perm holds row indices
 #pragma unroll
         for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
     }
@@ -306,8 +307,11 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     for (; s < end; s += blockDim.x) {
         if (s != start + (int32_t)threadIdx.x) {
             qi = perm[s];
+            if (qi < 0 || qi >= n) continue;        // unreachable: perm holds row indices
 #pragma unroll
             for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
+        } else if (perm[s] < 0 || perm[s] >= n) {
+            continue;
         }
         double loc[D];
         int64_t ia[D];
@@ -331,6 +335,7 @@ k_eval_perm(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t ha
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n || s >= total[0]) return;
     const int64_t i = perm[s];
+    if (i < 0 || i >= n) return;                    // unreachable: perm holds row indices
     double loc[D];
     bool oob;
     const int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
@@ -417,14 +422,14 @@ hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q,
-                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  int64_t n, int64_t ldq, const ArbOut& o, const int32_t* offs,
                                   const int32_t* perm, int32_t block_threads, hipStream_t s) {
     (void)ninterp;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16;
     dim3 grid((unsigned)B.nbt), block(block_threads);
 #define L_EVALB(DD, T, TC, TO)                                                              \
     hipLaunchKernelGGL((k_eval_bucket<DD, T, TC, TO>), grid, block, lds, s, g, B,             \
-                       (const TC*)table, has_comps, has_norm, q, ldq, o, offs, perm)
+                       (const TC*)table, has_comps, has_norm, q, n, ldq, o, offs, perm)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
 #undef L_EVALB

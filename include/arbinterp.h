@@ -206,6 +206,12 @@ int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads);
  * Builds the table first if needed. */
 int arb_table_copy(arb_field* f, int64_t cell0, int64_t ncells, void* dst_dev, void* stream);
 
+/* Upload cells [cell0, cell0+ncells) of the coefficient table from src_dev
+ * (same layout and type as arb_table_copy) and mark the table built when it
+ * covers every cell.  With arb_table_copy this saves and restores tables
+ * (CHANGELOG.md:9 of the reference plans saving coefficients to a file). */
+int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_dev, void* stream);
+
 /* Host-only: write the K x K operator matrix of makeAMatrix
  * (ARBInterp.py:107-175 / :726-878) into out (row-major, K = 4^ndim).
  * which: 0 = B (monomials and derivatives at the corners, = the reference's

@@ -238,3 +238,20 @@ def test_query_paths_bitwise_identical(dev, case):
                 assert b is None
                 continue
             assert _nan_equal(a, b), (case, name)
+
+
+def test_coefficient_save_load_roundtrip(dev, tmp_path):
+    rows, q = W.dipole3d(seed=2, n=(20, 18, 16), nq=5000)
+    for prec in ("fp64", "mixed"):
+        t = dev.tricubic(rows.copy(), "quiet", mode="both", precision=prec)
+        a = t.Query(q.copy())
+        path = str(tmp_path / ("tab_%s.npz" % prec))
+        t.save_coefficients(path)
+        t2 = dev.tricubic(rows.copy(), "quiet", mode="both", precision=prec)
+        t2.load_coefficients(path)
+        b = t2.Query(q.copy())
+        for x, y in zip(a, b):
+            assert np.array_equal(np.nan_to_num(x, nan=7.0), np.nan_to_num(y, nan=7.0))
+        other = dev.tricubic(rows.copy(), "quiet", mode="norm", precision=prec)
+        with pytest.raises(ValueError):
+            other.load_coefficients(path)
