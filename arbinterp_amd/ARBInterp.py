@@ -273,6 +273,35 @@ class _Interp:
         N.check(rc, "query")
         return (comps, norms, grads, cel) if cells else (comps, norms, grads)
 
+    def query_packed(self, q, out=None, flags=0, out_f64=False):
+        """Like query_device, but each row's outputs are written as one aligned
+        record of `packed_width` elements, [comps(3)] [value, grad(d)], zero
+        padded.  Returns the (n, packed_width) tensor; `unpack` gives views."""
+        torch = _torch()
+        if q.dtype != torch.float64 or not q.is_cuda or q.stride(1) != 1:
+            raise ValueError("query_packed needs a CUDA float64 tensor with contiguous rows")
+        n = q.shape[0]
+        W = int(self._info.packed_width)
+        dt = torch.float64 if (self.precision == "fp64" or out_f64) else torch.float32
+        rec = out if out is not None else torch.empty((n, W), dtype=dt, device="cuda")
+        if not flags & (N.ARB_Q_DIRECT | N.ARB_Q_SORTED | N.ARB_Q_SORTED_GLOBAL):
+            flags |= self.default_flags
+        fl = flags | N.ARB_Q_PACKED | (N.ARB_Q_OUT_F64 if out_f64 else 0)
+        N.check(self._run(torch, q, n, q.stride(0), rec, None, None, None, fl), "query")
+        return rec
+
+    def unpack(self, rec):
+        """(comps, values, grads) views of a packed record tensor (None where
+        the mode has no such output)."""
+        d = self.NDIM
+        comps = rec[:, 0:3] if self._info.has_comps else None
+        vals = grads = None
+        if self._info.has_norm:
+            b = 3 if self._info.has_comps else 0
+            vals = rec[:, b]
+            grads = rec[:, b + 1:b + 1 + d]
+        return comps, vals, grads
+
     def rQuery(self, query):
         """Batch query (rQuery1/2/3, ARBInterp.py:344-521 / :1064-1258).
 

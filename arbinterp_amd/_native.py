@@ -36,6 +36,8 @@ ARB_Q_SORTED = 16
 ARB_Q_SORTED_GLOBAL = 32
 ARB_Q_TIMING = 64
 ARB_Q_GROUP_ATOMIC = 128
+ARB_Q_PACKED = 256
+ARB_Q_SORT_CELL = 512
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
@@ -50,6 +52,7 @@ class FieldInfo(ctypes.Structure):
         ("n", c_i64 * 4), ("npos", c_i64 * 4), ("h", ctypes.c_double * 4),
         ("int_min", ctypes.c_double * 4), ("int_max", ctypes.c_double * 4),
         ("nc", c_i64), ("nnodes", c_i64), ("table_bytes", c_i64),
+        ("packed_width", c_i32), ("reserved", c_i32),
     ]
 
 

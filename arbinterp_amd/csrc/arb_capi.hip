@@ -496,6 +496,7 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     out->nc = f->g.nc;
     out->nnodes = f->g.nnodes;
     out->table_bytes = f->table_bytes;
+    out->packed_width = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
     return ARB_OK;
 }
 
@@ -523,8 +524,10 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     if (n < 0 || (n > 0 && (!q_dev || ldq < f->g.ndim)))
         return fail(ARB_EINVAL, "arb_query: bad positions (n=%lld ldq=%lld)", (long long)n,
                     (long long)ldq);
-    if (n > 0 && f->has_comps && !comps_dev) return fail(ARB_EINVAL, "arb_query: comps is NULL");
-    if (n > 0 && f->has_norm && (!norms_dev || !grads_dev))
+    const bool packed = (flags & ARB_Q_PACKED) != 0;
+    if (n > 0 && (f->has_comps || packed) && !comps_dev)
+        return fail(ARB_EINVAL, "arb_query: comps is NULL");
+    if (n > 0 && !packed && f->has_norm && (!norms_dev || !grads_dev))
         return fail(ARB_EINVAL, "arb_query: norms/grads is NULL");
     hipStream_t s = (hipStream_t)stream;
     int rc = arb_build_coeffs(f, stream);
@@ -539,6 +542,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     o.qmut = (flags & ARB_Q_NAN_ROWS) ? q_dev : nullptr;
     o.ldq = ldq;
     o.flag = f->flag + 1;
+    o.packed = packed ? arb_packed_width(f->has_comps, f->has_norm, f->g.ndim) : 0;
     const int out64 = (flags & ARB_Q_OUT_F64) ? 1 : 0;
     const bool timing = (flags & ARB_Q_TIMING) != 0;
     if (timing && !f->ev_ready) {
@@ -574,19 +578,28 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, n, s));
         } else {
+            // key = bucket (then the cell inside it with ARB_Q_SORT_CELL)
+            int cellbits = 0;
+            if (flags & ARB_Q_SORT_CELL)
+                while ((1 << cellbits) < f->B.ncell) ++cellbits;
+            int bits = 1;
+            while (((int64_t)1 << bits) <= f->B.nbt) ++bits;      // buckets are 0..nbt
+            bits += cellbits;
+            if (bits > 32) {          // no room for the cell: order by bucket only
+                bits -= cellbits;
+                cellbits = 0;
+            }
             HIPCHK(arb_launch_keys(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                    f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
-                                   f->rs_v[0], s));
+                                   f->rs_v[0], cellbits, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
-            int bits = 1;
-            while (((int64_t)1 << bits) <= f->B.nbt) ++bits;      // keys are 0..nbt
             int in_first = 1;
             HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
                                          f->rs_hist, f->rs_hscan, f->rs_tmp, f->rs_tmps,
                                          &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             const int w = in_first ? 0 : 1;
-            HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, s));
+            HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, cellbits, s));
             perm = f->rs_v[w];
         }
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));

@@ -31,7 +31,15 @@ struct ArbOut {
     double* qmut;     // positions to NaN-fill for out-of-range rows (nullable)
     int64_t ldq;      // row stride of the positions (also the fill width)
     int32_t* flag;    // set to 1 when a row reduced past the sentinel
+    int32_t packed;   // outputs are records of `packed` elements in `comps`
 };
+
+// Width (elements) of a packed output record: [comps(3)] [value, grad(d)],
+// padded to 4 or 8 so every record is one aligned 16/32/64-byte block.
+__host__ __device__ inline int arb_packed_width(int has_comps, int has_norm, int d) {
+    const int w = (has_comps ? 3 : 0) + (has_norm ? 1 + d : 0);
+    return w <= 4 ? 4 : 8;
+}
 
 // Cell blocks used to bucket queries: a bucket is a block of b[0] x b[1] x
 // b[2] [x b[3]] cells; its coefficients are staged in LDS together.
@@ -240,7 +248,8 @@ hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipS
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* table, int32_t ninterp, int32_t has_comps,
                            int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t* val, hipStream_t s);
+                           const ArbOut& o, uint32_t* key, int32_t* val, int32_t cellbits,
+                           hipStream_t s);
 int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
@@ -249,7 +258,7 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
                                  int bits, int32_t* hist, int32_t* hscan, int32_t* tmp,
                                  int32_t* tmp_scan, int* in_first, hipStream_t s);
 hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
-                             hipStream_t s);
+                             int32_t shift, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
                               int32_t* perm, int64_t n, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,

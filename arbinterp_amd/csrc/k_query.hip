@@ -23,25 +23,65 @@ __device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32
     T x[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) x[d] = (T)loc[d];
-    int k = 0;
+    TO rec[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) rec[r] = (TO)0;
+    int k = 0, w = 0;
     if (has_comps) {
 #pragma unroll 1
         for (int c = 0; c < 3; ++c) {
             T v;
             if constexpr (D == 3) v = arb_val3<T, TC>(a + c * K, x[0], x[1], x[2]);
             else v = arb_val4<T, TC>(a + c * K, x[0], x[1], x[2], x[3]);
-            put<TO>(o.comps, i * 3 + c, v);
+            rec[c] = (TO)v;
         }
         k = 3;
+        w = 3;
     }
     if (has_norm) {
         T v, gr[D];
         if constexpr (D == 3) arb_valgrad3<T, TC>(a + k * K, x[0], x[1], x[2], v, gr);
         else arb_valgrad4<T, TC>(a + k * K, x[0], x[1], x[2], x[3], v, gr);
-        put<TO>(o.norms, i, v);
+        if (has_comps) {
+            rec[3] = (TO)v;
 #pragma unroll
-        for (int d = 0; d < D; ++d) put<TO>(o.grads, i * D + d, gr[d] / (T)g.h[d]);
+            for (int d = 0; d < D; ++d) rec[4 + d] = (TO)(gr[d] / (T)g.h[d]);
+        } else {
+            rec[0] = (TO)v;
+#pragma unroll
+            for (int d = 0; d < D; ++d) rec[1 + d] = (TO)(gr[d] / (T)g.h[d]);
+        }
+        w += 1 + D;
     }
+    if (o.packed) {
+        // one aligned record per row, written as full 16-byte vectors
+        TO* dst = reinterpret_cast<TO*>(o.comps) + i * o.packed;
+        if constexpr (sizeof(TO) == 4) {
+            float4* d4 = reinterpret_cast<float4*>(dst);
+            d4[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+            if (o.packed == 8) d4[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+        } else {
+            double2* d2 = reinterpret_cast<double2*>(dst);
+            d2[0] = make_double2(rec[0], rec[1]);
+            d2[1] = make_double2(rec[2], rec[3]);
+            if (o.packed == 8) {
+                d2[2] = make_double2(rec[4], rec[5]);
+                d2[3] = make_double2(rec[6], rec[7]);
+            }
+        }
+        return;
+    }
+    if (has_comps) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) put<TO>(o.comps, i * 3 + c, rec[c]);
+    }
+    if (has_norm) {
+        const int b = has_comps ? 3 : 0;
+        put<TO>(o.norms, i, rec[b]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) put<TO>(o.grads, i * D + d, rec[b + 1 + d]);
+    }
+    (void)w;
 }
 
 // Cell reduction of row i with the side outputs every path writes: queryInds,
@@ -164,7 +204,7 @@ template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
 k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-       uint32_t* __restrict__ key, int32_t* __restrict__ val) {
+       uint32_t* __restrict__ key, int32_t* __restrict__ val, int32_t cellbits) {
     constexpr int K = (D == 3) ? 64 : 256;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -181,11 +221,12 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, i
         if (flat > g.nc) o.flag[0] = 1;
         eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
                                  o, i);
-        key[i] = (uint32_t)B.nbt;
+        key[i] = (uint32_t)B.nbt << cellbits;
         return;
     }
     int lc;
-    key[i] = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
+    const uint32_t b = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
+    key[i] = (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
 }
 
 // Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024
@@ -455,11 +496,13 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* table, int32_t ninterp, int32_t has_comps,
                            int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t* val, hipStream_t s) {
+                           const ArbOut& o, uint32_t* key, int32_t* val, int32_t cellbits,
+                           hipStream_t s) {
     if (n <= 0) return hipSuccess;
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3(grid1(n, 256)), dim3(256), 0, s, g, B,      \
-                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, key, val)
+                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, key, val,   \
+                       cellbits)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

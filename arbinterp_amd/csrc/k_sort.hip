@@ -147,14 +147,14 @@ k_sc_local(const int32_t* __restrict__ in, int64_t len, const int32_t* __restric
     }
 }
 
-// offs[b] = first sorted position whose key >= b, for b in [0, nb]; keys are
-// sorted, sentinel rows carry key nb and sort last.
+// offs[b] = first sorted position whose bucket (key >> shift) >= b, for b in
+// [0, nb]; keys are sorted, sentinel rows carry bucket nb and sort last.
 __global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ offs,
-                         int64_t nb) {
+                         int64_t nb, int32_t shift) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s > n) return;
-    const int64_t kp = (s == 0) ? -1 : (int64_t)keys[s - 1];
-    int64_t k = (s == n) ? nb : (int64_t)keys[s];
+    const int64_t kp = (s == 0) ? -1 : (int64_t)(keys[s - 1] >> shift);
+    int64_t k = (s == n) ? nb : (int64_t)(keys[s] >> shift);
     if (k > nb) k = nb;
     for (int64_t b = kp + 1; b <= k; ++b) offs[b] = (int32_t)s;
 }
@@ -208,7 +208,7 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
 }
 
 hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, s, keys, n, offs, nb);
+                             int32_t shift, hipStream_t s) {
+    hipLaunchKernelGGL(k_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, s, keys, n, offs, nb, shift);
     return hipGetLastError();
 }

@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--err-sample", type=int, default=200_000)
     ap.add_argument("--flags", type=int, default=0, help="extra ARB_Q_* flags")
     ap.add_argument("--variants", default="", help="comma list of query paths to time "
-                    "(auto,direct,sorted_lds,sorted_atomic,sorted_global); default all")
+                    "(auto,direct,sorted_lds,sorted_cell,sorted_atomic,sorted_global); default all")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
@@ -99,9 +99,9 @@ def main():
         qd = torch.from_numpy(q).cuda()
         t = tricubic(rows, "quiet", mode="both", precision="mixed")
         t.allCoeffs()
-        out = t.query_device(qd)
+        rec = t.query_packed(qd)
         for _ in range(args.steps):
-            t.query_device(qd, out=out, flags=args.flags)
+            t.query_packed(qd, out=rec, flags=args.flags)
         torch.cuda.synchronize()
         log("profile-only: %d steps done" % args.steps)
         return
@@ -117,6 +117,7 @@ def main():
 
     from arbinterp_amd import _native as NN
     variants = {"auto": 0, "direct": NN.ARB_Q_DIRECT, "sorted_lds": NN.ARB_Q_SORTED,
+                "sorted_cell": NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL,
                 "sorted_atomic": NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC,
                 "sorted_global": NN.ARB_Q_SORTED_GLOBAL}
     if args.variants:
@@ -129,20 +130,29 @@ def main():
         t.allCoeffs()
         build_ms = (time.time() - tb) * 1e3
         out = t.query_device(qd, cells=True)
+        rec = t.query_packed(qd)
         torch.cuda.synchronize()
         per = {}
         for vname, fl in variants.items():
-            fn = lambda: t.query_device(qd, out=out[:3], flags=fl | args.flags)
-            ms = time_steps(torch, fn, args.steps, args.warmup)
-            st = np.zeros(4)
-            for _ in range(args.timing_steps):
-                t.query_device(qd, out=out[:3], flags=fl | args.flags | NN.ARB_Q_TIMING)
-                st += np.array(t.timings())
-            st /= max(1, args.timing_steps)
-            per[vname] = {"ms_per_step": ms, "stage_ms": [round(x, 5) for x in st.tolist()]}
-            log("%s %-13s: %.3f ms/step  %.3e q/s  stages %s" % (prec, vname, ms,
-                                                                  args.nq / ms * 1e3, per[vname]["stage_ms"]))
-        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per)
+            for layout in ("packed", "arrays"):
+                if layout == "packed":
+                    fn = lambda fl=fl: t.query_packed(qd, out=rec, flags=fl | args.flags)
+                    fnt = lambda fl=fl: t.query_packed(qd, out=rec, flags=fl | args.flags | NN.ARB_Q_TIMING)
+                else:
+                    fn = lambda fl=fl: t.query_device(qd, out=out[:3], flags=fl | args.flags)
+                    fnt = lambda fl=fl: t.query_device(qd, out=out[:3], flags=fl | args.flags | NN.ARB_Q_TIMING)
+                ms = time_steps(torch, fn, args.steps, args.warmup)
+                st = np.zeros(4)
+                for _ in range(args.timing_steps):
+                    fnt()
+                    st += np.array(t.timings())
+                st /= max(1, args.timing_steps)
+                key = vname if layout == "packed" else vname + "_arrays"
+                per[key] = {"ms_per_step": ms, "stage_ms": [round(x, 5) for x in st.tolist()]}
+                log("%s %-20s: %.3f ms/step  %.3e q/s  stages %s" % (prec, key, ms,
+                                                                     args.nq / ms * 1e3, per[key]["stage_ms"]))
+        t.unpack(rec)
+        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=rec)
     nc = res["mixed"]["obj"].nc
     cells = res["mixed"]["out"][3]
     valid = cells[cells < nc]
@@ -157,7 +167,7 @@ def main():
     scales = o.condition_scale(q[idx])
     err = {}
     for prec in ("mixed", "fp64"):
-        c, nv, g = (x.double().cpu().numpy() for x in res[prec]["out"][:3])
+        c, nv, g = (x.double().cpu().numpy() for x in res[prec]["obj"].unpack(res[prec]["rec"]))
         got = (c[idx], nv[idx, None], g[idx])
         e = {}
         for name, a, b, s in zip(("comps", "norm", "grad"), got, ref, scales):
@@ -175,6 +185,8 @@ def main():
     head = "auto" if "auto" in res["mixed"]["per"] else next(iter(res["mixed"]["per"]))
     ms = res["mixed"]["per"][head]["ms_per_step"]
     ms64 = res["fp64"]["per"][head]["ms_per_step"]
+    # algorithmic output bytes are the 7 values per row (the record's pad word
+    # is overhead of the packed layout, not counted)
     alg_bytes = N * (24 + 7 * 4) + U * 4 * 64 * 4
     alg64 = N * (24 + 7 * 8) + U * 4 * 64 * 8
     ev_ms = res["mixed"]["per"][head]["stage_ms"][3] or ms
@@ -217,6 +229,8 @@ def main():
             "workload": "tricubic dipole field %dx%dx%d, h=(1.0,0.8,1.25) mm, mode='both', "
                         "%d interior FP64 positions resident in HBM" % (n + (N,)),
             "precision": "mixed (FP64 cell reduction, FP32 table and evaluation)",
+            "outputs": "comps, |B|, grad|B| per row as one 32-byte FP32 record (query_packed); "
+                       "the separate-array layout is timed under paths.*_arrays",
             "queries": N, "cells": nc, "distinct_cells_touched": U,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -76,6 +76,12 @@ extern "C" {
                                for its own work (see arb_last_timings)           */
 #define ARB_Q_GROUP_ATOMIC 128 /* bucketed path: group queries with per-bucket
                                atomic ranks instead of the radix sort            */
+#define ARB_Q_PACKED  256   /* write each row's outputs as one aligned record of
+                               arb_field_info.packed_width elements into
+                               comps_dev: [comps(3)] [value, grad(d)], zero
+                               padded; norms_dev/grads_dev are not used        */
+#define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
+                               cell inside the bucket (default: bucket only)    */
 
 typedef struct arb_field arb_field;
 
@@ -96,6 +102,8 @@ typedef struct arb_field_info {
     int64_t nc;            /* number of cells (ARBInterp.py:568)                */
     int64_t nnodes;
     int64_t table_bytes;   /* bytes of the coefficient table                    */
+    int32_t packed_width;  /* elements per row with ARB_Q_PACKED (4 or 8)       */
+    int32_t reserved;
 } arb_field_info;
 
 /* Analyse a field and upload it (tricubic/quadcubic constructor minus the

@@ -228,11 +228,12 @@ def test_query_paths_bitwise_identical(dev, case):
     qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
     outs = {}
     for name, fl in (("direct", NN.ARB_Q_DIRECT), ("lds", NN.ARB_Q_SORTED),
+                     ("cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL),
                      ("atomic", NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC),
                      ("global", NN.ARB_Q_SORTED_GLOBAL)):
         outs[name] = t.query_device(qd.clone(), cells=True, flags=fl)
     torch.cuda.synchronize()
-    for name in ("lds", "atomic", "global"):
+    for name in ("lds", "cell", "atomic", "global"):
         for a, b in zip(outs["direct"], outs[name]):
             if a is None:
                 assert b is None
@@ -255,3 +256,32 @@ def test_coefficient_save_load_roundtrip(dev, tmp_path):
         other = dev.tricubic(rows.copy(), "quiet", mode="norm", precision=prec)
         with pytest.raises(ValueError):
             other.load_coefficients(path)
+
+
+@pytest.mark.parametrize("case", ["3d_both_mixed", "3d_scalar_fp64", "3d_vector", "4d_both_fp64",
+                                  "4d_scalar_mixed"])
+def test_packed_records_match_arrays(dev, case):
+    import torch
+    from arbinterp_amd import _native as NN
+    if case.startswith("3d"):
+        rows, q = W.dipole3d(seed=2, n=(21, 19, 17), nq=70_000)
+        if "scalar" in case:
+            rows = rows[:, :4].copy()
+        cls = dev.tricubic
+    else:
+        rows, q = W.moving4d(seed=3, n=(10, 9, 8, 7), nq=70_000)
+        if "both" in case:
+            extra = np.stack([np.sin(rows[:, 0]), np.cos(rows[:, 1]), rows[:, 4]], axis=1)
+            rows = np.concatenate([rows[:, :4], extra], axis=1)
+        cls = dev.quadcubic
+    kw = {"mode": "both"} if "both" in case else ({"mode": "vector"} if "vector" in case else {})
+    t = cls(rows, "quiet", precision="mixed" if "mixed" in case else "fp64", **kw)
+    qd = torch.from_numpy(np.concatenate([q, q[:100] * 1.3])).cuda()
+    for fl in (NN.ARB_Q_DIRECT, NN.ARB_Q_SORTED):
+        ref = t.query_device(qd, flags=fl)
+        rec = t.query_packed(qd, flags=fl)
+        for a, b in zip(ref, t.unpack(rec)):
+            if a is None:
+                assert b is None
+                continue
+            assert _nan_equal(a, b.contiguous()), (case, fl)

@@ -55,3 +55,16 @@ def test_argument_errors_without_gpu():
         N.check(N.ARB_ESHAPE)
     with pytest.raises(IndexError):
         N.check(N.ARB_EINDEX)
+
+
+def test_packed_record_layout_documented():
+    """Record widths: [comps(3)] [value, grad(d)] padded to 4 or 8 elements."""
+    txt = open(HEADER).read()
+    assert "ARB_Q_PACKED" in txt and "packed_width" in txt
+    import ctypes as C
+    assert any(f[0] == "packed_width" for f in N.FieldInfo._fields_)
+    # 3D both: 3 + 1 + 3 = 7 -> 8; 3D scalar: 4 -> 4; 4D both: 8; 4D scalar: 5 -> 8; vector: 4
+    width = lambda comps, norm, d: 4 if (3 * comps + (1 + d) * norm) <= 4 else 8
+    assert (width(1, 1, 3), width(0, 1, 3), width(1, 1, 4), width(0, 1, 4), width(1, 0, 3)) == \
+        (8, 4, 8, 8, 4)
+    assert C.sizeof(N.FieldInfo) % 8 == 0
