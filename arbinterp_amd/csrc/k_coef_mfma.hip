@@ -256,8 +256,8 @@ k_coef_f64mfma3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
             bop[ks] = tile[(oz * TY + (w + oy)) * TX + (lxc + ox)];
         }
         double* out = table + (cell * ninterp + k) * 64;
-#pragma unroll 1
-        for (int mt = 0; mt < 4; ++mt) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {          // unrolled: aop stays in registers
             d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int ks = 0; ks < 16; ++ks)

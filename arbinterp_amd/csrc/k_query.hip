@@ -314,7 +314,11 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
 #pragma unroll
         for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
     }
-    // stage the block: up to 8 independent 16-byte loads in flight per thread
+    // stage the block: 8 independent 16-byte loads in flight per thread.  Loads
+    // and LDS stores are unconditional so that they issue back to back: a
+    // slot of a cell outside the grid reads the table's first vector into its
+    // own LDS slot (no query reads it), and a slot past the block writes the
+    // padding vector of cell 0.
     constexpr int BATCH = 8;
     for (int v0 = threadIdx.x; v0 < total; v0 += BATCH * blockDim.x) {
         uint4 r[BATCH];
@@ -322,27 +326,22 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) {
             const int v = v0 + u * blockDim.x;
-            dst[u] = -1;
-            if (v < total) {
-                const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
-                const int j = v - lc * B.cellvec;
-                bool inside = true;
-                int64_t gcell = 0;
+            const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
+            const int j = v - lc * B.cellvec;
+            const bool live = v < total;
+            bool inside = live;
+            int64_t gcell = 0;
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
-                    inside &= la < ext[a];
-                    gcell += (c0[a] + la) * cstr[a];
-                }
-                if (inside) {
-                    r[u] = tv[gcell * B.cellvec + j];
-                    dst[u] = lc * B.cellstride + j;
-                }
+            for (int a = 0; a < D; ++a) {
+                const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
+                inside &= la < ext[a];
+                gcell += (c0[a] + la) * cstr[a];
             }
+            r[u] = tv[inside ? gcell * B.cellvec + j : 0];
+            dst[u] = live ? lc * B.cellstride + j : B.cellvec;
         }
 #pragma unroll
-        for (int u = 0; u < BATCH; ++u)
-            if (dst[u] >= 0) smem[dst[u]] = r[u];
+        for (int u = 0; u < BATCH; ++u) smem[dst[u]] = r[u];
     }
     __syncthreads();
     for (; s < end; s += blockDim.x) {

@@ -43,7 +43,10 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
                uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
                const int32_t* __restrict__ hscan, int64_t ntiles) {
     __shared__ int32_t wcnt[RS_WAVES][256];     // running digit counts per wave
-    __shared__ int32_t gbase[256];
+    __shared__ int32_t gbase[256];              // global start of each digit's run
+    __shared__ int32_t toff[256];               // tile-local start of each digit's run
+    __shared__ uint32_t skey[RS_TILE];          // the tile's pairs in digit order
+    __shared__ int32_t sval[RS_TILE];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int d = threadIdx.x; d < RS_WAVES * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
     gbase[threadIdx.x] = hscan[(int64_t)threadIdx.x * ntiles + blockIdx.x];
@@ -73,26 +76,46 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         if (ok && before == 0) wcnt[w][dg] = run + __popcll(peers);   // group leader
     }
     __syncthreads();
-    // exclusive prefix over the waves of this tile, per digit
-    if (threadIdx.x < 256) {
-        int32_t acc = 0;
+    // exclusive prefix over the waves of this tile, per digit (thread = digit)
+    int32_t acc = 0;
 #pragma unroll
-        for (int ww = 0; ww < RS_WAVES; ++ww) {
-            const int32_t c = wcnt[ww][threadIdx.x];
-            wcnt[ww][threadIdx.x] = acc;
-            acc += c;
-        }
+    for (int ww = 0; ww < RS_WAVES; ++ww) {
+        const int32_t c = wcnt[ww][threadIdx.x];
+        wcnt[ww][threadIdx.x] = acc;
+        acc += c;
     }
+    // exclusive scan of the tile's digit totals over the 256 digits
+    toff[threadIdx.x] = acc;
     __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int32_t x = threadIdx.x >= off ? toff[threadIdx.x - off] : 0;
+        __syncthreads();
+        toff[threadIdx.x] += x;
+        __syncthreads();
+    }
+    toff[threadIdx.x] -= acc;
+    __syncthreads();
+    // local reorder: digit-major, element order inside each digit
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int64_t e = base + j * 64 + lane;
         if (e >= n) continue;
         const uint32_t dg = (key[j] >> shift) & 255u;
-        const int64_t pos = (int64_t)gbase[dg] + wcnt[w][dg] + rnk[j];
-        if (pos >= n) continue;               // unreachable: ranks stay inside the tile's run
-        kout[pos] = key[j];
-        vout[pos] = val[j];
+        const int lp = toff[dg] + wcnt[w][dg] + rnk[j];
+        skey[lp] = key[j];
+        sval[lp] = val[j];
+    }
+    __syncthreads();
+    // write out: consecutive slots of one digit go to consecutive addresses
+    const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+    const int cnt = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
+    for (int i = threadIdx.x; i < cnt; i += RS_THREADS) {
+        const uint32_t k = skey[i];
+        const uint32_t dg = (k >> shift) & 255u;
+        const int64_t pos = (int64_t)gbase[dg] + (i - toff[dg]);
+        if (pos < 0 || pos >= n) continue;    // unreachable: a digit's run stays inside [0, n)
+        kout[pos] = k;
+        vout[pos] = sval[i];
     }
 }
 

@@ -231,7 +231,14 @@ def test_query_paths_bitwise_identical(dev, case):
                      ("cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL),
                      ("atomic", NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC),
                      ("global", NN.ARB_Q_SORTED_GLOBAL)):
-        outs[name] = t.query_device(qd.clone(), cells=True, flags=fl)
+        # fresh buffers holding a marker, so a row that a path skips (or a
+        # sort that loses a row) cannot pass on stale memory
+        buf = t.query_device(qd.clone(), cells=True, flags=NN.ARB_Q_DIRECT)
+        torch.cuda.synchronize()
+        for x in buf:
+            if x is not None:
+                x.fill_(-7 if x.dtype == torch.int64 else 3.0e30)
+        outs[name] = t.query_device(qd.clone(), cells=True, flags=fl, out=buf)
     torch.cuda.synchronize()
     for name in ("lds", "cell", "atomic", "global"):
         for a, b in zip(outs["direct"], outs[name]):
