@@ -420,6 +420,9 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
         g.hi[a] = axes[a][g.n[a] - 2];
     }
     g.nnodes = prod;
+    // cell strides are 32-bit in the staging arithmetic of the query kernels
+    if (g.nc >= ((int64_t)1 << 32))
+        return fail(ARB_EGRID, "grid too large: %lld cells (limit 2^32)", (long long)g.nc);
 
     // 3. rank rows, scatter values into canonical order, check completeness
     DevBuf axes_dev[ARB_MAXD], seen;

@@ -19,6 +19,12 @@ struct Basis1 {
     double b0[D][4], b1[D][4], b2[D][4];
 };
 
+// t[e] for a runtime e in 0..3 by selects (a dynamic index into a register
+// array would place the array in scratch memory)
+__device__ __forceinline__ double sel4(const double* t, int e) {
+    return e == 0 ? t[0] : e == 1 ? t[1] : e == 2 ? t[2] : t[3];
+}
+
 template <int D>
 __device__ __forceinline__ void make_basis(const double* loc, Basis1<D>& B) {
 #pragma unroll
@@ -47,23 +53,23 @@ __device__ void val_grad_hess(const TC* __restrict__ a, const Basis1<D>& B, doub
         for (int x = 0; x < D; ++x) e[x] = (i >> (2 * x)) & 3;
         double p0 = c;
 #pragma unroll
-        for (int x = 0; x < D; ++x) p0 *= B.b0[x][e[x]];
+        for (int x = 0; x < D; ++x) p0 *= sel4(B.b0[x], e[x]);
         val += p0;
 #pragma unroll
         for (int x = 0; x < D; ++x) {
-            double gx = c * B.b1[x][e[x]];
-            double hxx = c * B.b2[x][e[x]];
+            double gx = c * sel4(B.b1[x], e[x]);
+            double hxx = c * sel4(B.b2[x], e[x]);
 #pragma unroll
             for (int y = 0; y < D; ++y)
-                if (y != x) { gx *= B.b0[y][e[y]]; hxx *= B.b0[y][e[y]]; }
+                if (y != x) { gx *= sel4(B.b0[y], e[y]); hxx *= sel4(B.b0[y], e[y]); }
             g[x] += gx;
             H[x][x] += hxx;
 #pragma unroll
             for (int y = x + 1; y < D; ++y) {
-                double hxy = c * B.b1[x][e[x]] * B.b1[y][e[y]];
+                double hxy = c * sel4(B.b1[x], e[x]) * sel4(B.b1[y], e[y]);
 #pragma unroll
                 for (int z = 0; z < D; ++z)
-                    if (z != x && z != y) hxy *= B.b0[z][e[z]];
+                    if (z != x && z != y) hxy *= sel4(B.b0[z], e[z]);
                 H[x][y] += hxy;
             }
         }
@@ -174,14 +180,14 @@ k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ 
             for (int x = 0; x < D; ++x) e[x] = (j >> (2 * x)) & 3;
             double m = 1.0;
 #pragma unroll
-            for (int x = 0; x < D; ++x) m *= B.b0[x][e[x]];
+            for (int x = 0; x < D; ++x) m *= sel4(B.b0[x], e[x]);
             double t = v * m;
 #pragma unroll
             for (int x = 0; x < D; ++x) {
-                double dm = B.b1[x][e[x]];
+                double dm = sel4(B.b1[x], e[x]);
 #pragma unroll
                 for (int y = 0; y < D; ++y)
-                    if (y != x) dm *= B.b0[y][e[y]];
+                    if (y != x) dm *= sel4(B.b0[y], e[y]);
                 t += gk[x] * dm;
             }
             acc[r] += t;

@@ -119,13 +119,14 @@ k_query_direct(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t
 
 // ---------------------------------------------------------------------------
 // Cell-bucketed path.  Queries are grouped by the block of cells they fall in
-// (counting sort: atomic rank per bucket, one scan, one scatter of the query
-// index); one workgroup then stages its block's coefficients in LDS with
+// (k_keys + the radix sort of k_sort.hip by default; with ARB_Q_GROUP_ATOMIC
+// a counting sort: atomic rank per bucket, one scan, one scatter of the query
+// index).  A workgroup then stages a block's coefficients in LDS with
 // contiguous 16-byte loads -- each cell's coefficients leave HBM once per
 // call -- and evaluates the bucket's queries from LDS.  Output rows are
 // written at the caller's row index, so no unsort pass is needed.  Results do
 // not depend on the order inside a bucket (each row is computed alone), so
-// the atomic ranks leave the outputs bitwise deterministic.
+// every grouping gives bitwise identical outputs.
 // ---------------------------------------------------------------------------
 
 // Bucket of a cell and the cell's index inside its block.  The per-axis cell
@@ -276,6 +277,92 @@ __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __rest
     perm[offs[b] + rank[i]] = (int32_t)i;
 }
 
+// ---------------------------------------------------------------------------
+// Staging map.  Slot v (0 <= v < ncell * cellvec) of a bucket holds 16-byte
+// vector j of local cell lc (v = lc * cellvec + j); in LDS it sits at
+// lc * cellstride + j.  Its table source is the bucket's first cell plus
+// sum_a la_a * cs_a cells, la_a the local coordinates of lc; cs_a, the cell
+// stride of axis a, fits 32 bits (fewer than 2^32 cells, checked when the
+// field is created), so the per-slot arithmetic is 32-bit multiply-adds.
+// ---------------------------------------------------------------------------
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+template <int D>
+struct StageBase {
+    uint64_t vbase;          // first cell of the bucket, in 16-byte vectors
+    int32_t ext[D];          // cells of the bucket inside the grid, per axis
+    bool edge;               // some axis has ext < block edge
+};
+
+template <int D>
+__device__ __forceinline__ void stage_base(int64_t b, const ArbGeom& g, const ArbBuckets& B,
+                                           StageBase<D>& sb) {
+    int64_t rem = b;
+    uint64_t cell = 0, st = 1;
+    bool edge = false;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const int64_t bi = rem % B.nb[a];
+        rem /= B.nb[a];
+        const int64_t c0 = bi << B.bsh[a];
+        const int64_t left = g.npos[a] - c0;
+        sb.ext[a] = (int32_t)(left < B.b[a] ? left : B.b[a]);
+        edge |= sb.ext[a] < B.b[a];
+        cell += (uint64_t)c0 * st;
+        st *= (uint64_t)g.npos[a];
+    }
+    sb.edge = edge;
+    sb.vbase = cell * (uint64_t)B.cellvec;
+}
+
+template <int D>
+__device__ __forceinline__ void cell_strides32(const ArbGeom& g, uint32_t* cs) {
+    uint32_t st = 1;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        cs[a] = st;
+        st *= (uint32_t)g.npos[a];
+    }
+}
+
+// Table source of slot v (vector index; 0, a valid dummy, for a slot past the
+// block or of a cell outside the grid) and its LDS slot (the padding vector
+// of cell 0 for a slot past the block).
+template <int D>
+__device__ __forceinline__ uint64_t stage_slot(int v, int total, const ArbBuckets& B,
+                                               const uint32_t* cs, const StageBase<D>& sb,
+                                               int* dst) {
+    const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
+    const int j = v - lc * B.cellvec;
+    bool inside = v < total;
+    uint64_t off = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        const uint32_t la = (uint32_t)(lc >> B.lsh[a]) & (uint32_t)(B.b[a] - 1);
+        if (sb.edge) inside &= (int32_t)la < sb.ext[a];
+        off += (uint64_t)la * cs[a];
+    }
+    *dst = v < total ? lc * B.cellstride + j : B.cellvec;
+    return inside ? sb.vbase + off * (uint32_t)B.cellvec + (uint32_t)j : 0;
+}
+
+// Evaluate row qi (position qp) of bucket b from the block staged in LDS.
+template <int D, typename T, typename TC, typename TO>
+__device__ __forceinline__ void eval_row_lds(const ArbGeom& g, const ArbBuckets& B,
+                                             const uint4* smem, int32_t has_comps,
+                                             int32_t has_norm, const ArbOut& o, int64_t b,
+                                             int64_t qi, const double* qp) {
+    double loc[D];
+    int64_t ia[D];
+    bool oob;
+    const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
+    int lc;
+    const int64_t bb = bucket_of<D>(flat, ia, g, B, &lc);
+    if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
+    const TC* a = reinterpret_cast<const TC*>(smem + lc * B.cellstride);
+    eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, qi);
+}
+
 // One workgroup per bucket: stage the block's coefficients in LDS, then
 // evaluate the bucket's queries from LDS.
 template <int D, typename T, typename TC, typename TO>
@@ -287,81 +374,45 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     const int64_t b = blockIdx.x;
     const int32_t start = offs[b], end = offs[b + 1];
     if (start == end) return;                       // uniform over the workgroup
-    int64_t c0[D], ext[D], cstr[D];
-    {
-        int64_t rem = b, st = 1;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const int64_t bi = rem % B.nb[a];
-            rem /= B.nb[a];
-            c0[a] = bi << B.bsh[a];
-            const int64_t left = g.npos[a] - c0[a];
-            ext[a] = left < B.b[a] ? left : B.b[a];
-            cstr[a] = st;
-            st *= g.npos[a];
-        }
-    }
-    const uint4* __restrict__ tv = reinterpret_cast<const uint4*>(table);
+    StageBase<D> sb;
+    stage_base<D>(b, g, B, sb);
+    uint32_t cs[D];
+    cell_strides32<D>(g, cs);
+    const v4u* __restrict__ tv = reinterpret_cast<const v4u*>(table);
+    v4u* lds = reinterpret_cast<v4u*>(smem);
     const int total = B.ncell * B.cellvec;
     // this thread's first query: issue its loads before the staging so the
     // latencies overlap
     int32_t s = start + threadIdx.x;
-    int64_t qi = 0;
+    int64_t qi = -1;
     double qp[D];
     if (s < end) {
-        qi = perm[s];
-        if (qi < 0 || qi >= n) qi = 0;              // unreachable: perm holds row indices
+        const int64_t pi = perm[s];
+        qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
+        const int64_t qq = qi >= 0 ? qi : 0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
+        for (int a = 0; a < D; ++a) qp[a] = q[qq * ldq + a];
     }
     // stage the block: 8 independent 16-byte loads in flight per thread.  Loads
-    // and LDS stores are unconditional so that they issue back to back: a
-    // slot of a cell outside the grid reads the table's first vector into its
-    // own LDS slot (no query reads it), and a slot past the block writes the
-    // padding vector of cell 0.
+    // and LDS stores are unconditional so that they issue back to back.
     constexpr int BATCH = 8;
     for (int v0 = threadIdx.x; v0 < total; v0 += BATCH * blockDim.x) {
-        uint4 r[BATCH];
+        v4u r[BATCH];
         int dst[BATCH];
 #pragma unroll
-        for (int u = 0; u < BATCH; ++u) {
-            const int v = v0 + u * blockDim.x;
-            const int lc = B.cvsh >= 0 ? (v >> B.cvsh) : v / B.cellvec;
-            const int j = v - lc * B.cellvec;
-            const bool live = v < total;
-            bool inside = live;
-            int64_t gcell = 0;
+        for (int u = 0; u < BATCH; ++u)
+            r[u] = tv[stage_slot<D>(v0 + u * blockDim.x, total, B, cs, sb, &dst[u])];
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const int la = (lc >> B.lsh[a]) & (B.b[a] - 1);
-                inside &= la < ext[a];
-                gcell += (c0[a] + la) * cstr[a];
-            }
-            r[u] = tv[inside ? gcell * B.cellvec + j : 0];
-            dst[u] = live ? lc * B.cellstride + j : B.cellvec;
-        }
-#pragma unroll
-        for (int u = 0; u < BATCH; ++u) smem[dst[u]] = r[u];
+        for (int u = 0; u < BATCH; ++u) lds[dst[u]] = r[u];
     }
     __syncthreads();
-    for (; s < end; s += blockDim.x) {
-        if (s != start + (int32_t)threadIdx.x) {
-            qi = perm[s];
-            if (qi < 0 || qi >= n) continue;        // unreachable: perm holds row indices
+    if (qi >= 0) eval_row_lds<D, T, TC, TO>(g, B, smem, has_comps, has_norm, o, b, qi, qp);
+    for (s += blockDim.x; s < end; s += blockDim.x) {
+        const int64_t pi = perm[s];
+        if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
 #pragma unroll
-            for (int a = 0; a < D; ++a) qp[a] = q[qi * ldq + a];
-        } else if (perm[s] < 0 || perm[s] >= n) {
-            continue;
-        }
-        double loc[D];
-        int64_t ia[D];
-        bool oob;
-        const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
-        int lc;
-        const int64_t bb = bucket_of<D>(flat, ia, g, B, &lc);
-        if (bb != b || flat >= g.nc) continue;      // cannot happen: same reduction as k_bucket
-        const TC* a = reinterpret_cast<const TC*>(smem + lc * B.cellstride);
-        eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, qi);
+        for (int a = 0; a < D; ++a) qp[a] = q[pi * ldq + a];
+        eval_row_lds<D, T, TC, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
     }
 }
 

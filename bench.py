@@ -28,6 +28,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "value+grad queries/s, mixed-prec vs FP64 plain on 1 GPU at ≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
+# evaluation kernel of the headline step (mixed, 3D, FP64 output records)
+HEAD_KERNEL = "k_eval_bucket<3, float, float, double>"
 
 
 def parse():
@@ -99,9 +101,9 @@ def main():
         qd = torch.from_numpy(q).cuda()
         t = tricubic(rows, "quiet", mode="both", precision="mixed")
         t.allCoeffs()
-        rec = t.query_packed(qd)
+        rec = t.query_packed(qd, out_f64=True)
         for _ in range(args.steps):
-            t.query_packed(qd, out=rec, flags=args.flags)
+            t.query_packed(qd, out=rec, flags=args.flags, out_f64=True)
         torch.cuda.synchronize()
         log("profile-only: %d steps done" % args.steps)
         return
@@ -130,29 +132,34 @@ def main():
         t.allCoeffs()
         build_ms = (time.time() - tb) * 1e3
         out = t.query_device(qd, cells=True)
-        rec = t.query_packed(qd)
+        # output layouts: "p64" one 64-byte FP64 record per row (the headline;
+        # full 64-byte writes), "p32" one 32-byte FP32 record (mixed only),
+        # "arrays" separate comps / norm / grad arrays
+        recs = {"p64": t.query_packed(qd, out_f64=True)}
+        if prec == "mixed":
+            recs["p32"] = t.query_packed(qd)
         torch.cuda.synchronize()
         per = {}
         for vname, fl in variants.items():
-            for layout in ("packed", "arrays"):
-                if layout == "packed":
-                    fn = lambda fl=fl: t.query_packed(qd, out=rec, flags=fl | args.flags)
-                    fnt = lambda fl=fl: t.query_packed(qd, out=rec, flags=fl | args.flags | NN.ARB_Q_TIMING)
+            for layout in list(recs) + ["arrays"]:
+                f = fl | args.flags
+                if layout == "arrays":
+                    fn = lambda f=f: t.query_device(qd, out=out[:3], flags=f)
                 else:
-                    fn = lambda fl=fl: t.query_device(qd, out=out[:3], flags=fl | args.flags)
-                    fnt = lambda fl=fl: t.query_device(qd, out=out[:3], flags=fl | args.flags | NN.ARB_Q_TIMING)
+                    r64 = layout == "p64"
+                    fn = lambda f=f, r=recs[layout], r64=r64: t.query_packed(qd, out=r, flags=f,
+                                                                            out_f64=r64)
                 ms = time_steps(torch, fn, args.steps, args.warmup)
                 st = np.zeros(4)
                 for _ in range(args.timing_steps):
-                    fnt()
+                    fn(f=f | NN.ARB_Q_TIMING)
                     st += np.array(t.timings())
                 st /= max(1, args.timing_steps)
-                key = vname if layout == "packed" else vname + "_arrays"
+                key = vname if layout == "p64" else vname + "_" + layout
                 per[key] = {"ms_per_step": ms, "stage_ms": [round(x, 5) for x in st.tolist()]}
                 log("%s %-20s: %.3f ms/step  %.3e q/s  stages %s" % (prec, key, ms,
                                                                      args.nq / ms * 1e3, per[key]["stage_ms"]))
-        t.unpack(rec)
-        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=rec)
+        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=recs["p64"])
     nc = res["mixed"]["obj"].nc
     cells = res["mixed"]["out"][3]
     valid = cells[cells < nc]
@@ -185,9 +192,9 @@ def main():
     head = "auto" if "auto" in res["mixed"]["per"] else next(iter(res["mixed"]["per"]))
     ms = res["mixed"]["per"][head]["ms_per_step"]
     ms64 = res["fp64"]["per"][head]["ms_per_step"]
-    # algorithmic output bytes are the 7 values per row (the record's pad word
-    # is overhead of the packed layout, not counted)
-    alg_bytes = N * (24 + 7 * 4) + U * 4 * 64 * 4
+    # algorithmic output bytes are the 7 values per row, 8 bytes each in the
+    # FP64 record (the record's pad word is overhead of the layout, not counted)
+    alg_bytes = N * (24 + 7 * 8) + U * 4 * 64 * 4
     alg64 = N * (24 + 7 * 8) + U * 4 * 64 * 8
     ev_ms = res["mixed"]["per"][head]["stage_ms"][3] or ms
     ev64 = res["fp64"]["per"][head]["stage_ms"][3] or ms64
@@ -229,13 +236,17 @@ def main():
             "workload": "tricubic dipole field %dx%dx%d, h=(1.0,0.8,1.25) mm, mode='both', "
                         "%d interior FP64 positions resident in HBM" % (n + (N,)),
             "precision": "mixed (FP64 cell reduction, FP32 table and evaluation)",
-            "outputs": "comps, |B|, grad|B| per row as one 32-byte FP32 record (query_packed); "
-                       "the separate-array layout is timed under paths.*_arrays",
+            "outputs": "comps, |B|, grad|B| per row as one 64-byte record of FP64 values "
+                       "(query_packed(out_f64=True); FP32 arithmetic, exact widening); the "
+                       "32-byte FP32 record and the separate-array layouts are timed under "
+                       "paths.*_p32 and paths.*_arrays",
             "queries": N, "cells": nc, "distinct_cells_touched": U,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "evaluation kernel of the default path", "kernel_ms": ev_ms},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic(HEAD_KERNEL),
+                     "kernel": HEAD_KERNEL, "kernel_ms": ev_ms,
+                     "algorithmic_bytes": alg_bytes},
         "cpu_baseline": cpu,
         "fp64_plain": {"value": N / (ms64 * 1e-3), "ms_per_step": ms64,
                        "eval_kernel_ms": ev64, "roofline_achieved_GBs": ach64,
@@ -247,6 +258,24 @@ def main():
         "algorithmic_bytes_per_step": alg_bytes,
     }
     print(json.dumps(line), flush=True)
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the rocprofv3 counter summaries
+    under profiles/ (line "traffic_bytes_per_dispatch <kernel> = <bytes>":
+    FETCH_SIZE + WRITE_SIZE of the headline workload, each scaled by the
+    factor a plain copy of known size reads for that counter), or None."""
+    import glob
+    import re
+    pat = re.compile(r"^traffic_bytes_per_dispatch (.+?) = (\d+)\s*$")
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path in sorted(glob.glob(os.path.join(here, "profiles", "*.txt"))):
+        with open(path) as fh:
+            for line in fh:
+                m = pat.match(line)
+                if m and m.group(1) == kernel:
+                    return int(m.group(2))
+    return None
 
 
 def _events_ms(torch, fn, steps, warmup):
