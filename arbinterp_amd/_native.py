@@ -38,6 +38,7 @@ ARB_Q_TIMING = 64
 ARB_Q_GROUP_ATOMIC = 128
 ARB_Q_PACKED = 256
 ARB_Q_SORT_CELL = 512
+ARB_Q_FUSED = 1024
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64

@@ -168,12 +168,14 @@ struct arb_field {
     ArbGeom g{};
     int32_t mode = 0, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
     int32_t table_built = 0;
+    int32_t sentinel_set = 0;   // table row nc written
+    int32_t table_loaded = 0;   // rows came from arb_table_load
     int device = 0;
     double* vals = nullptr;     // [ninterp][nnodes] FP64, canonical node order
     void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
     int64_t table_bytes = 0;
     double* A = nullptr;        // K x K FP64
-    unsigned short* Ab = nullptr;  // inv(B) as BF16 in MFMA operand order (mixed)
+    unsigned short* Ab = nullptr;  // BF16 MFMA operand table (mixed): 16 A (3D), inv(B) (4D)
     int32_t* flag = nullptr;    // device status word
     // bucketed query path
     ArbBuckets B{};
@@ -454,14 +456,21 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     std::vector<unsigned short> Ab;
     if (precision == ARB_PREC_MIXED) {
         // operand order [mt][ks][lane][j]: row mt*16 + (lane & 15), column
-        // ks*32 + 8*(lane >> 4) + j.  Entries are small integers: exact in BF16.
+        // ks*32 + 8*(lane >> 4) + j.  3D: 16 A (alpha = A . F from node
+        // values); 4D: inv(B) (alpha = inv(B) . b from finite differences).
+        // Entries are integers of magnitude <= 250: exact in BF16.
         const int KS = K / 32, MT = K / 16;
+        const double* src = ndim == 3 ? A.data() : Bi.data();
+        const double scale = ndim == 3 ? 16.0 : 1.0;
         Ab.resize((size_t)K * K);
         for (int mt = 0; mt < MT; ++mt)
             for (int ks = 0; ks < KS; ++ks)
                 for (int l = 0; l < 64; ++l)
                     for (int j = 0; j < 8; ++j) {
-                        float v = (float)Bi[(size_t)(mt * 16 + (l & 15)) * K + ks * 32 + 8 * (l >> 4) + j];
+                        const double e = scale * src[(size_t)(mt * 16 + (l & 15)) * K + ks * 32 + 8 * (l >> 4) + j];
+                        if (e != std::rint(e) || std::fabs(e) > 256.0)
+                            return fail(ARB_EINVAL, "BF16 operand table entry %g is not a small integer", e);
+                        float v = (float)e;
                         uint32_t u;
                         std::memcpy(&u, &v, 4);
                         Ab[(((size_t)mt * KS + ks) * 64 + l) * 8 + j] = (unsigned short)(u >> 16);
@@ -503,19 +512,29 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     return ARB_OK;
 }
 
+// The sentinel row nc of the table (zeros, or NaN, per mode: F5), which rows
+// outside the box are evaluated from.
+static int ensure_sentinel(arb_field* f, hipStream_t s) {
+    if (f->sentinel_set) return ARB_OK;
+    const int K = 1 << (2 * f->g.ndim);
+    const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
+    HIPCHK(arb_launch_sentinel(f->table, f->precision != ARB_PREC_FP64, f->g.nc, f->ninterp, K,
+                               sentinel_nan, s));
+    f->sentinel_set = 1;
+    return ARB_OK;
+}
+
 int arb_build_coeffs(arb_field* f, void* stream) {
     if (!f) return fail(ARB_EINVAL, "arb_build_coeffs: NULL handle");
     if (f->table_built) return ARB_OK;
     hipStream_t s = (hipStream_t)stream;
-    const int K = 1 << (2 * f->g.ndim);
-    const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
     if (f->precision == ARB_PREC_FP64) {
         HIPCHK(arb_launch_coef_f64mfma(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
     } else {
         HIPCHK(arb_launch_coef_bf16x3(f->g, f->vals, f->ninterp, f->Ab, (float*)f->table, s));
     }
-    HIPCHK(arb_launch_sentinel(f->table, f->precision != ARB_PREC_FP64, f->g.nc, f->ninterp, K,
-                               sentinel_nan, s));
+    int rc = ensure_sentinel(f, s);
+    if (rc) return rc;
     f->table_built = 1;
     return ARB_OK;
 }
@@ -533,7 +552,17 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     if (n > 0 && !packed && f->has_norm && (!norms_dev || !grads_dev))
         return fail(ARB_EINVAL, "arb_query: norms/grads is NULL");
     hipStream_t s = (hipStream_t)stream;
-    int rc = arb_build_coeffs(f, stream);
+    bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
+    if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
+    if (n > INT32_MAX - 1) sorted = false;
+    if (sorted && f->B.nbt == 0) choose_buckets(f);
+    // mixed 3D, bucketed, on request: coefficients formed in the query kernel
+    // from node values, no table (ignored once a table has been loaded, or
+    // when the block does not fit the kernel's LDS)
+    const bool fused = (flags & ARB_Q_FUSED) && sorted && f->precision == ARB_PREC_MIXED &&
+                       f->g.ndim == 3 && !f->table_loaded && !(flags & ARB_Q_SORTED_GLOBAL) &&
+                       arb_fused_lds(f->g, f->B, f->ninterp) > 0;
+    int rc = fused ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (n == 0) return ARB_OK;
     if (flags & ARB_Q_CHECK) HIPCHK(hipMemsetAsync(f->flag + 1, 0, sizeof(int32_t), s));
@@ -552,9 +581,6 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         for (auto& e : f->ev) HIPCHK(hipEventCreate(&e));
         f->ev_ready = 1;
     }
-    bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
-    if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
-    if (n > INT32_MAX - 1) sorted = false;
     for (float& m : f->last_ms) m = 0.f;
     if (!sorted) {
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
@@ -566,7 +592,6 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(hipEventElapsedTime(&f->last_ms[3], f->ev[0], f->ev[4]));
         }
     } else {
-        if (f->B.nbt == 0) choose_buckets(f);
         rc = ensure_workspace(f, n);
         if (rc) return rc;
         const int32_t* perm = f->ws_perm;
@@ -610,6 +635,10 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                         perm, s));
+        else if (fused)
+            HIPCHK(arb_launch_eval_fused(f->g, f->B, out64, f->vals, f->ninterp, f->Ab,
+                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
+                                         perm, f->eval_threads, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
@@ -929,9 +958,16 @@ int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_
     const int K = 1 << (2 * f->g.ndim);
     const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
     const size_t row = esz * (size_t)f->ninterp * K;
+    // rows not loaded keep the built coefficients; a partial load overlays them
+    if (!(cell0 == 0 && ncells == f->g.nc + 1)) {
+        int rc = arb_build_coeffs(f, stream);
+        if (rc) return rc;
+    }
     HIPCHK(hipMemcpyAsync((char*)f->table + row * cell0, src_dev, row * ncells,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
-    if (cell0 == 0 && ncells == f->g.nc + 1) f->table_built = 1;
+    f->table_built = 1;
+    f->sentinel_set = 1;
+    f->table_loaded = 1;      // the table is authoritative: queries read it
     return ARB_OK;
 }
 

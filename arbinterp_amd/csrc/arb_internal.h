@@ -227,10 +227,6 @@ hipError_t arb_launch_rank_scatter(const double* rows, int64_t nrows, int64_t ld
                                    int32_t* seen, hipStream_t s);
 hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* flag, hipStream_t s);
 
-hipError_t arb_launch_coef_fp64(const ArbGeom& g, const double* vals, int32_t ninterp,
-                                const double* A, double* table, hipStream_t s);
-hipError_t arb_launch_coef_mixed(const ArbGeom& g, const double* vals, int32_t ninterp,
-                                 const double* A, float* table, hipStream_t s);
 hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
                                   const unsigned short* Ab, float* table, hipStream_t s);
 hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t ninterp,
@@ -261,6 +257,12 @@ hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int
                              int32_t shift, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
                               int32_t* perm, int64_t n, hipStream_t s);
+size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp);
+hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t out_f64,
+                                 const double* vals, int32_t ninterp, const unsigned short* Af,
+                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                 const int32_t* perm, int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q,

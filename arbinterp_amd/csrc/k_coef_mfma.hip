@@ -26,6 +26,7 @@
 // node tile (19 x 7 x 4 [x 4]) is loaded once per interpolant into LDS with
 // row-contiguous reads.
 #include "arb_internal.h"
+#include "arb_coef3.h"
 
 namespace {
 
@@ -40,16 +41,6 @@ constexpr int TX = TXC + 3, TY = TYC + 3;   // nodes per tile
 // reference's block order (ARBInterp.py:118-125, :738-757).
 __constant__ unsigned char kBlockMask3[8] = {0, 1, 2, 4, 3, 5, 6, 7};
 __constant__ unsigned char kBlockMask4[16] = {0, 1, 2, 4, 8, 3, 5, 9, 6, 10, 12, 7, 11, 13, 14, 15};
-
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-    unsigned int u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;     // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ double bf16_to_double(unsigned short b) {
-    return (double)__uint_as_float(((unsigned int)b) << 16);
-}
 
 template <int D>
 struct TileGeom {
@@ -204,6 +195,47 @@ k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
     }
 }
 
+// Mixed precision, 3D: alpha = A . F directly from the node tile (no
+// finite-difference stage), by arbc3::coef3_mixed; the tiling is the one
+// above.  Af is 16 A as BF16 in operand order.
+__global__ void __launch_bounds__(256)
+k_coef_af3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
+           const unsigned short* __restrict__ Af, float* __restrict__ table) {
+    __shared__ double tile[TileGeom<3>::NODES];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lx = lane & 15;
+    const int grp = lane >> 4;
+    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
+    const int64_t cy0 = (int64_t)blockIdx.y * TYC;
+    const int64_t cz = blockIdx.z;
+    const int64_t cx = cx0 + lx, cy = cy0 + w;
+    const bool valid = (cx < g.npos[0]) && (cy < g.npos[1]);
+    const int lxc = valid ? lx : 0;
+    const int64_t cell = cx + g.npos[0] * (cy + g.npos[1] * cz);
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();
+        for (int i = threadIdx.x; i < TileGeom<3>::NODES; i += 256) {
+            const int x = i % TX;
+            const int y = (i / TX) % TY;
+            const int z = i / (TX * TY);
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
+            tile[i] = v;
+        }
+        __syncthreads();
+        auto nb = [&](int kk) {
+            return tile[((kk >> 4) * TY + (w + ((kk >> 2) & 3))) * TX + (lxc + (kk & 3))];
+        };
+        float* out = table + (cell * ninterp + k) * 64 + 4 * grp;
+        arbc3::coef3_mixed(nb, Af, [&](int mt, const arbc3::f4& v) {
+            if (valid) *reinterpret_cast<arbc3::f4*>(out + mt * 16) = v;
+        });
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Plain FP64 (ARB_PREC_FP64), 3D: alpha = A . F with A = inv(B).D (exact
 // dyadic, the reference's A of ARBInterp.py:175) and F the 64 neighbourhood
@@ -341,14 +373,15 @@ hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t
     return hipGetLastError();
 }
 
-// Ab: inv(B) as BF16 in operand order [mt][ks][lane][8] (host-prepared).
+// Ab: BF16 operand table in order [mt][ks][lane][8] (host-prepared): 16 A for
+// 3D (k_coef_af3), inv(B) for 4D (k_coef_bf16x3<4>).
 hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
                                   const unsigned short* Ab, float* table, hipStream_t s) {
     if (g.nc == 0) return hipSuccess;
     dim3 block(256);
     if (g.ndim == 3) {
         dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]);
-        hipLaunchKernelGGL(k_coef_bf16x3<3>, grid, block, 0, s, g, vals, ninterp, Ab, table);
+        hipLaunchKernelGGL(k_coef_af3, grid, block, 0, s, g, vals, ninterp, Ab, table);
     } else {
         dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)(g.npos[2] * g.npos[3]));
         hipLaunchKernelGGL(k_coef_bf16x3<4>, grid, block, 0, s, g, vals, ninterp, Ab, table);

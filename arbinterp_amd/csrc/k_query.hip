@@ -5,6 +5,7 @@
 // gradients.  The reference builds ~40 N x 64 temporaries per call; here each
 // query is one lane.
 #include "arb_internal.h"
+#include "arb_coef3.h"
 
 namespace {
 
@@ -416,6 +417,97 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     }
 }
 
+// Mixed precision, 3D, without the coefficient table: one workgroup per
+// bucket loads the bucket's node values (its cells + 3 per axis, every
+// interpolant, FP64) into LDS, forms the coefficients of its cells on the
+// matrix cores (arbc3::coef3_mixed, the function the table build uses, so
+// the results are bitwise those of the table), writes them into the same LDS
+// block layout the staged kernel uses, and evaluates the bucket's queries.
+// Work items (interpolant, column tile of 16 cells) are spread over the
+// waves.  Dynamic LDS: ncell * cellstride * 16 bytes of coefficients, then
+// ninterp * (b0+3)(b1+3)(b2+3) doubles of node values.
+template <typename TO>
+__global__ void __launch_bounds__(256)
+k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
+              const unsigned short* __restrict__ Af, int32_t has_comps, int32_t has_norm,
+              const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
+              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
+    extern __shared__ uint4 smem[];
+    const int64_t b = blockIdx.x;
+    const int32_t start = offs[b], end = offs[b + 1];
+    if (start == end) return;                       // uniform over the workgroup
+    int64_t c0[3];
+    {
+        int64_t rem = b;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            c0[a] = (rem % B.nb[a]) << B.bsh[a];
+            rem /= B.nb[a];
+        }
+    }
+    const int NX = B.b[0] + 3, NY = B.b[1] + 3, NZ = B.b[2] + 3;
+    const int tile_nodes = NX * NY * NZ;
+    double* nodes = reinterpret_cast<double*>(smem + B.ncell * B.cellstride);
+    // this thread's first query, loaded before the node tile
+    int32_t s = start + threadIdx.x;
+    int64_t qi = -1;
+    double qp[3];
+    if (s < end) {
+        const int64_t pi = perm[s];
+        qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
+        const int64_t qq = qi >= 0 ? qi : 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
+    }
+    // node tile, x fastest; nodes past the grid edge read as 0 (only cells
+    // outside the grid use them)
+    for (int i = threadIdx.x; i < ninterp * tile_nodes; i += blockDim.x) {
+        const int k = i / tile_nodes;
+        const int r = i - k * tile_nodes;
+        const int x = r % NX;
+        const int y = (r / NX) % NY;
+        const int z = r / (NX * NY);
+        const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
+        double v = 0.0;
+        if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
+            v = vals[(int64_t)k * g.nnodes + nx + g.nstride[1] * ny + g.nstride[2] * nz];
+        nodes[i] = v;
+    }
+    __syncthreads();
+    // coefficients of every (interpolant, 16-cell column tile), wave-uniform
+    {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+        const int ntile = (B.ncell + 15) >> 4;
+        float* cf = reinterpret_cast<float*>(smem);
+        for (int it = w; it < ninterp * ntile; it += nw) {
+            const int k = it / ntile;
+            const int lc = (it - k * ntile) * 16 + (lane & 15);
+            const bool live = lc < B.ncell;
+            const int lcc = live ? lc : 0;
+            const int lx = (lcc >> B.lsh[0]) & (B.b[0] - 1);
+            const int ly = (lcc >> B.lsh[1]) & (B.b[1] - 1);
+            const int lz = (lcc >> B.lsh[2]) & (B.b[2] - 1);
+            const double* nk = nodes + k * tile_nodes;
+            auto nb = [&](int kk) {
+                return nk[((lz + (kk >> 4)) * NY + (ly + ((kk >> 2) & 3))) * NX + (lx + (kk & 3))];
+            };
+            float* dst = cf + (size_t)lcc * B.cellstride * 4 + k * 64 + 4 * (lane >> 4);
+            arbc3::coef3_mixed(nb, Af, [&](int mt, const arbc3::f4& v) {
+                if (live) *reinterpret_cast<arbc3::f4*>(dst + mt * 16) = v;
+            });
+        }
+    }
+    __syncthreads();
+    if (qi >= 0) eval_row_lds<3, float, float, TO>(g, B, smem, has_comps, has_norm, o, b, qi, qp);
+    for (s += blockDim.x; s < end; s += blockDim.x) {
+        const int64_t pi = perm[s];
+        if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
+#pragma unroll
+        for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
+        eval_row_lds<3, float, float, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
+    }
+}
+
 // Sorted order, coefficients read from the table in global memory.
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
@@ -524,6 +616,32 @@ hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
 #undef L_EVALB
+    return hipGetLastError();
+}
+
+// LDS bytes of k_eval_fused3 for the bucket geometry B (0 when the path does
+// not apply: 3D mixed only, block within 64 KiB).
+size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp) {
+    if (g.ndim != 3) return 0;
+    const size_t lds = (size_t)B.ncell * B.cellstride * 16 +
+                       (size_t)ninterp * (B.b[0] + 3) * (B.b[1] + 3) * (B.b[2] + 3) * 8;
+    return lds <= 64 * 1024 ? lds : 0;
+}
+
+hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t out_f64,
+                                 const double* vals, int32_t ninterp, const unsigned short* Af,
+                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                 const int32_t* perm, int32_t block_threads, hipStream_t s) {
+    const size_t lds = arb_fused_lds(g, B, ninterp);
+    if (lds == 0) return hipErrorInvalidValue;
+    dim3 grid((unsigned)B.nbt), block(block_threads);
+    if (out_f64)
+        hipLaunchKernelGGL(k_eval_fused3<double>, grid, block, lds, s, g, B, vals, ninterp, Af,
+                           has_comps, has_norm, q, n, ldq, o, offs, perm);
+    else
+        hipLaunchKernelGGL(k_eval_fused3<float>, grid, block, lds, s, g, B, vals, ninterp, Af,
+                           has_comps, has_norm, q, n, ldq, o, offs, perm);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--err-sample", type=int, default=200_000)
     ap.add_argument("--flags", type=int, default=0, help="extra ARB_Q_* flags")
     ap.add_argument("--variants", default="", help="comma list of query paths to time "
-                    "(auto,direct,sorted_lds,sorted_cell,sorted_atomic,sorted_global); default all")
+                    "(auto,direct,sorted_lds,sorted_fused,sorted_cell,sorted_atomic,sorted_global); default all")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
@@ -119,6 +119,7 @@ def main():
 
     from arbinterp_amd import _native as NN
     variants = {"auto": 0, "direct": NN.ARB_Q_DIRECT, "sorted_lds": NN.ARB_Q_SORTED,
+                "sorted_fused": NN.ARB_Q_SORTED | NN.ARB_Q_FUSED,
                 "sorted_cell": NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL,
                 "sorted_atomic": NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC,
                 "sorted_global": NN.ARB_Q_SORTED_GLOBAL}

@@ -80,6 +80,11 @@ extern "C" {
                                arb_field_info.packed_width elements into
                                comps_dev: [comps(3)] [value, grad(d)], zero
                                padded; norms_dev/grads_dev are not used        */
+#define ARB_Q_FUSED  1024   /* mixed-precision 3D bucketed path: form each
+                               block's coefficients in the query kernel from
+                               node values instead of staging them from the
+                               table (default).  No table is built or read;
+                               the results are bitwise the same              */
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
 
@@ -215,8 +220,10 @@ int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads);
 int arb_table_copy(arb_field* f, int64_t cell0, int64_t ncells, void* dst_dev, void* stream);
 
 /* Upload cells [cell0, cell0+ncells) of the coefficient table from src_dev
- * (same layout and type as arb_table_copy) and mark the table built when it
- * covers every cell.  With arb_table_copy this saves and restores tables
+ * (same layout and type as arb_table_copy).  A partial upload overlays the
+ * built table (built first if needed).  From then on every query of the
+ * field reads the table (the mixed 3D path no longer forms coefficients from
+ * node values).  With arb_table_copy this saves and restores tables
  * (CHANGELOG.md:9 of the reference plans saving coefficients to a file). */
 int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_dev, void* stream);
 

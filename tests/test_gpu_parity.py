@@ -90,6 +90,28 @@ def test_coefficient_tables_golden(dev, golden):
     assert (np.abs(got[:, :-1] - ref[:, :-1]) <= 1e-15 * scale).all()
 
 
+def test_mixed_coefficient_table_bound(dev):
+    """Mixed-precision 3D table against the oracle's FP64 alpha: the table holds
+    alpha of the node values quantised to 2^(E-32) (E: exponent of the cell's
+    largest |F - f0|) rounded once to FP32, so
+        |got - ref| <= 2^-24 |ref| + 2^(E-32) sum_k |A_ik|."""
+    rows, _ = W.dipole3d(seed=2, n=(21, 18, 15), nq=10)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    tab = t.coefficient_table().astype(np.float64)     # (nc+1, 4, 64)
+    o = O.tricubic(rows.copy(), "quiet", mode="both")
+    A = O.make_A(3)
+    off = O.neighbour_offsets(o.p.n, 3)
+    rowsum = np.abs(A).sum(axis=1)
+    for k, name in enumerate("xyzn"):
+        F = o.interps[name][o.p.base[None, :] + off[:, None]]     # (64, nc)
+        ref = A @ F
+        r = np.abs(F - F[21][None, :]).max(axis=0)
+        E = np.frexp(r)[1]
+        bound = 2.0 ** -24 * np.abs(ref) + np.ldexp(rowsum[:, None], E[None, :] - 32)
+        got = tab[:-1, k, :].T
+        assert (np.abs(got - ref) <= bound).all(), name
+
+
 def test_edge_battery_golden(dev, golden):
     g = golden("edge_battery")
     rows, _ = W.scalar3d_gaussians(seed=1)
@@ -188,16 +210,19 @@ def _nan_equal(a, b):
 
 
 @pytest.mark.parametrize("case", ["3d_both_mixed", "3d_both_fp64", "3d_scalar", "4d_scalar",
-                                  "4d_both_mixed", "3d_vector_tuned"])
+                                  "4d_both_mixed", "3d_vector_tuned", "3d_scalar_mixed",
+                                  "3d_vector_mixed_tuned"])
 def test_query_paths_bitwise_identical(dev, case):
     """Direct, bucketed-LDS and bucketed-global paths give identical bits,
-    including rows outside the box, NaN rows, IntMax rows, partial blocks."""
+    including rows outside the box, NaN rows, IntMax rows, partial blocks.
+    For mixed 3D the fused path forms coefficients in the query kernel and
+    the others read the table: the same bits are required."""
     import torch
     from arbinterp_amd import _native as NN
     rng = np.random.default_rng(5)
     if case.startswith("3d"):
         rows, q = W.dipole3d(seed=2, n=(23, 19, 14), nq=30_000)
-        if case == "3d_scalar":
+        if case.startswith("3d_scalar"):
             rows = rows[:, :4].copy()
     else:
         rows, q = W.moving4d(seed=3, n=(11, 9, 8, 7), nq=30_000)
@@ -213,7 +238,7 @@ def test_query_paths_bitwise_identical(dev, case):
         kw["mode"] = "vector"
     prec = "mixed" if "mixed" in case else "fp64"
     t = cls(rows, "quiet", precision=prec, **kw)
-    if case == "3d_vector_tuned":
+    if case in ("3d_vector_tuned", "3d_vector_mixed_tuned"):
         t.set_tuning((2, 2, 8), 128)
     d = t.NDIM
     lo = np.array([getattr(t, "xyzt"[a] + "IntMin") for a in range(d)])
@@ -229,6 +254,7 @@ def test_query_paths_bitwise_identical(dev, case):
     outs = {}
     for name, fl in (("direct", NN.ARB_Q_DIRECT), ("lds", NN.ARB_Q_SORTED),
                      ("cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL),
+                     ("fused", NN.ARB_Q_SORTED | NN.ARB_Q_FUSED),
                      ("atomic", NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC),
                      ("global", NN.ARB_Q_SORTED_GLOBAL)):
         # fresh buffers holding a marker, so a row that a path skips (or a
@@ -240,7 +266,7 @@ def test_query_paths_bitwise_identical(dev, case):
                 x.fill_(-7 if x.dtype == torch.int64 else 3.0e30)
         outs[name] = t.query_device(qd.clone(), cells=True, flags=fl, out=buf)
     torch.cuda.synchronize()
-    for name in ("lds", "cell", "atomic", "global"):
+    for name in ("lds", "cell", "fused", "atomic", "global"):
         for a, b in zip(outs["direct"], outs[name]):
             if a is None:
                 assert b is None
