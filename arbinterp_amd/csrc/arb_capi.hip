@@ -175,7 +175,7 @@ struct arb_field {
     void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
     int64_t table_bytes = 0;
     double* A = nullptr;        // K x K FP64
-    unsigned short* Ab = nullptr;  // BF16 MFMA operand table (mixed): 16 A (3D), inv(B) (4D)
+    unsigned short* Ab = nullptr;  // inv(B) as BF16 in MFMA operand order (mixed 4D)
     int32_t* flag = nullptr;    // device status word
     // bucketed query path
     ArbBuckets B{};
@@ -203,7 +203,8 @@ struct arb_field {
     int32_t* bw_count = nullptr;   // nc + 1
     int32_t* bw_offs = nullptr;    // nc + 1
     int32_t* bw_cursor = nullptr;  // nc + 1
-    double* bw_bbar = nullptr;     // nc x K
+    int32_t* bw_tmp = nullptr;     // scan temporaries
+    double* bw_bbar = nullptr;     // K x nc (slot-major)
     // particle push graph
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t push_exec = nullptr;
@@ -454,14 +455,13 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     HIPCHK(hipMalloc(&f->A, sizeof(double) * K * K));
     HIPCHK(hipMemcpyAsync(f->A, A.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, s));
     std::vector<unsigned short> Ab;
-    if (precision == ARB_PREC_MIXED) {
-        // operand order [mt][ks][lane][j]: row mt*16 + (lane & 15), column
-        // ks*32 + 8*(lane >> 4) + j.  3D: 16 A (alpha = A . F from node
-        // values); 4D: inv(B) (alpha = inv(B) . b from finite differences).
-        // Entries are integers of magnitude <= 250: exact in BF16.
+    if (precision == ARB_PREC_MIXED && ndim == 4) {
+        // inv(B) in operand order [mt][ks][lane][j]: row mt*16 + (lane & 15),
+        // column ks*32 + 8*(lane >> 4) + j.  Entries are integers of magnitude
+        // <= 81: exact in BF16.
         const int KS = K / 32, MT = K / 16;
-        const double* src = ndim == 3 ? A.data() : Bi.data();
-        const double scale = ndim == 3 ? 16.0 : 1.0;
+        const double* src = Bi.data();
+        const double scale = 1.0;
         Ab.resize((size_t)K * K);
         for (int mt = 0; mt < MT; ++mt)
             for (int ks = 0; ks < KS; ++ks)
@@ -556,12 +556,11 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
     if (n > INT32_MAX - 1) sorted = false;
     if (sorted && f->B.nbt == 0) choose_buckets(f);
-    // mixed 3D, bucketed, on request: coefficients formed in the query kernel
-    // from node values, no table (ignored once a table has been loaded, or
-    // when the block does not fit the kernel's LDS)
-    const bool fused = (flags & ARB_Q_FUSED) && sorted && f->precision == ARB_PREC_MIXED &&
-                       f->g.ndim == 3 && !f->table_loaded && !(flags & ARB_Q_SORTED_GLOBAL) &&
-                       arb_fused_lds(f->g, f->B, f->ninterp) > 0;
+    // 3D, bucketed, on request: coefficients formed in the query kernel from
+    // node values, no table (ignored once a table has been loaded, or when
+    // the block does not fit the kernel's LDS)
+    const bool fused = (flags & ARB_Q_FUSED) && sorted && f->g.ndim == 3 && !f->table_loaded &&
+                       !(flags & ARB_Q_SORTED_GLOBAL) && arb_fused_lds(f->g, f->B) > 0;
     int rc = fused ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (n == 0) return ARB_OK;
@@ -636,7 +635,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                         perm, s));
         else if (fused)
-            HIPCHK(arb_launch_eval_fused(f->g, f->B, out64, f->vals, f->ninterp, f->Ab,
+            HIPCHK(arb_launch_eval_fused(f->g, f->B, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                          perm, f->eval_threads, s));
         else
@@ -699,9 +698,11 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             f->bw_n = n;
         }
         if (!f->bw_bbar) {
+            const int64_t tl = arb_scan_tmp_elems(nc) + 1;
             HIPCHK(hipMalloc(&f->bw_count, sizeof(int32_t) * (nc + 1)));
             HIPCHK(hipMalloc(&f->bw_offs, sizeof(int32_t) * (nc + 1)));
             HIPCHK(hipMalloc(&f->bw_cursor, sizeof(int32_t) * (nc + 1)));
+            HIPCHK(hipMalloc(&f->bw_tmp, sizeof(int32_t) * 2 * tl));
             HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * nc * K));
         }
         HIPCHK(arb_launch_fill_i32(f->bw_count, nc + 1, 0, s));
@@ -712,7 +713,8 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
                                      q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
                                      dval_dev ? f->bw_key : nullptr, f->bw_count, s));
     if (dval_dev) {
-        HIPCHK(arb_launch_scan(f->bw_count, f->bw_offs, nc, s));
+        HIPCHK(arb_launch_scan_long(f->bw_count, f->bw_offs, nc, f->bw_tmp,
+                                    f->bw_tmp + arb_scan_tmp_elems(nc) + 1, s));
         HIPCHK(arb_launch_seg_scatter(f->bw_key, f->bw_offs, f->bw_cursor, f->bw_perm, n, s));
         HIPCHK(arb_launch_seg_sort(f->bw_offs, f->bw_perm, nc, s));
         HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_offs,
@@ -996,6 +998,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->bw_count) (void)hipFree(f->bw_count);
     if (f->bw_offs) (void)hipFree(f->bw_offs);
     if (f->bw_cursor) (void)hipFree(f->bw_cursor);
+    if (f->bw_tmp) (void)hipFree(f->bw_tmp);
     if (f->bw_bbar) (void)hipFree(f->bw_bbar);
     if (f->ev_ready)
         for (auto& e : f->ev) (void)hipEventDestroy(e);

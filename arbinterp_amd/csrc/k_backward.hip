@@ -7,9 +7,11 @@
 //            c_i = gv_i m(x_i) + sum_k gg_ik dm_k(x_i)/h_k
 // The node adjoint is deterministic: queries are grouped per cell (counting
 // sort), each cell's segment is put in query-index order, the per-cell
-// 64 (256)-vector alpha_bar is summed in that order, b_bar = A^T alpha_bar,
-// and every node gathers b_bar from its <= 4^d cells in a fixed order.  No
-// floating-point atomics.
+// 64 (256)-vector alpha_bar is summed in that order, b_bar = A^T alpha_bar
+// (3D: FP64 matrix cores, 16-cell tiles), stored slot-major bbar[l][cell],
+// and every node gathers b_bar from its <= 4^d cells in a fixed slot order
+// (coalesced: neighbouring nodes read neighbouring cells).  No floating-point
+// atomics.
 #include "arb_internal.h"
 
 namespace {
@@ -36,48 +38,80 @@ __device__ __forceinline__ void make_basis(const double* loc, Basis1<D>& B) {
     }
 }
 
-// Value, gradient and Hessian (cuboid units) of sum_i a_i m_i.
-template <int D, typename TC>
-__device__ void val_grad_hess(const TC* __restrict__ a, const Basis1<D>& B, double& val,
-                              double* g, double (*H)[D]) {
-    constexpr int K = 1 << (2 * D);
-    val = 0.0;
-    for (int x = 0; x < D; ++x) {
-        g[x] = 0.0;
-        for (int y = 0; y < D; ++y) H[x][y] = 0.0;
+// p(u), p'(u) and p''(u)/2 of c0 + c1 u + c2 u^2 + c3 u^3 (synthetic division).
+__device__ __forceinline__ void h4dd(double c0, double c1, double c2, double c3, double u,
+                                     double& p, double& d, double& hh) {
+    p = c3;
+    d = 0.0;
+    hh = 0.0;
+    hh = fma(hh, u, d); d = fma(d, u, p); p = fma(p, u, c2);
+    hh = fma(hh, u, d); d = fma(d, u, p); p = fma(p, u, c1);
+    hh = fma(hh, u, d); d = fma(d, u, p); p = fma(p, u, c0);
+}
+
+// Value, gradient and Hessian (cuboid units) of sum_i a_i u^ex v^ey w^ez by
+// tensor Horner: rows along u (p, p', p''), then v, then w, keeping the
+// derivative combinations of total order <= 2.
+template <typename TC>
+__device__ __forceinline__ void hess3(const TC* __restrict__ a, double u, double v, double w,
+                                      double& val, double* g, double (*H)[3]) {
+    double Q[4], Qv[4], Qvv[4], R[4], Rv[4], S[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double P[4], Pu[4], Puu[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const TC* c = a + 16 * k + 4 * j;
+            double hh;
+            h4dd((double)c[0], (double)c[1], (double)c[2], (double)c[3], u, P[j], Pu[j], hh);
+            Puu[j] = 2.0 * hh;
+        }
+        double hq;
+        h4dd(P[0], P[1], P[2], P[3], v, Q[k], Qv[k], hq);
+        Qvv[k] = 2.0 * hq;
+        h4d<double>(Pu[0], Pu[1], Pu[2], Pu[3], v, R[k], Rv[k]);
+        S[k] = h4<double>(Puu[0], Puu[1], Puu[2], Puu[3], v);
     }
-    for (int i = 0; i < K; ++i) {
-        const double c = (double)a[i];
-        int e[D];
+    double hw;
+    h4dd(Q[0], Q[1], Q[2], Q[3], w, val, g[2], hw);
+    H[2][2] = 2.0 * hw;
+    h4d<double>(R[0], R[1], R[2], R[3], w, g[0], H[0][2]);
+    h4d<double>(Qv[0], Qv[1], Qv[2], Qv[3], w, g[1], H[1][2]);
+    H[0][0] = h4<double>(S[0], S[1], S[2], S[3], w);
+    H[1][1] = h4<double>(Qvv[0], Qvv[1], Qvv[2], Qvv[3], w);
+    H[0][1] = h4<double>(Rv[0], Rv[1], Rv[2], Rv[3], w);
+    H[1][0] = H[0][1];
+    H[2][0] = H[0][2];
+    H[2][1] = H[1][2];
+}
+
+// 4D: the 3D quantities of the four t-slices, combined along t.
+template <typename TC>
+__device__ __forceinline__ void hess4(const TC* __restrict__ a, double u, double v, double w,
+                                      double t, double& val, double* g, double (*H)[4]) {
+    double r[4], gs[3][4], hs[3][3][4];
 #pragma unroll
-        for (int x = 0; x < D; ++x) e[x] = (i >> (2 * x)) & 3;
-        double p0 = c;
+    for (int l = 0; l < 4; ++l) {
+        double gl[3], Hl[3][3];
+        hess3<TC>(a + 64 * l, u, v, w, r[l], gl, Hl);
 #pragma unroll
-        for (int x = 0; x < D; ++x) p0 *= sel4(B.b0[x], e[x]);
-        val += p0;
+        for (int x = 0; x < 3; ++x) {
+            gs[x][l] = gl[x];
 #pragma unroll
-        for (int x = 0; x < D; ++x) {
-            double gx = c * sel4(B.b1[x], e[x]);
-            double hxx = c * sel4(B.b2[x], e[x]);
-#pragma unroll
-            for (int y = 0; y < D; ++y)
-                if (y != x) { gx *= sel4(B.b0[y], e[y]); hxx *= sel4(B.b0[y], e[y]); }
-            g[x] += gx;
-            H[x][x] += hxx;
-#pragma unroll
-            for (int y = x + 1; y < D; ++y) {
-                double hxy = c * sel4(B.b1[x], e[x]) * sel4(B.b1[y], e[y]);
-#pragma unroll
-                for (int z = 0; z < D; ++z)
-                    if (z != x && z != y) hxy *= sel4(B.b0[z], e[z]);
-                H[x][y] += hxy;
-            }
+            for (int y = 0; y < 3; ++y) hs[x][y][l] = Hl[x][y];
         }
     }
+    double ht;
+    h4dd(r[0], r[1], r[2], r[3], t, val, g[3], ht);
+    H[3][3] = 2.0 * ht;
 #pragma unroll
-    for (int x = 0; x < D; ++x)
+    for (int x = 0; x < 3; ++x) {
+        h4d<double>(gs[x][0], gs[x][1], gs[x][2], gs[x][3], t, g[x], H[x][3]);
+        H[3][x] = H[x][3];
 #pragma unroll
-        for (int y = 0; y < x; ++y) H[x][y] = H[y][x];
+        for (int y = 0; y < 3; ++y)
+            H[x][y] = h4<double>(hs[x][y][0], hs[x][y][1], hs[x][y][2], hs[x][y][3], t);
+    }
 }
 
 // d/dq per query, and the cell key for the node adjoint.
@@ -104,10 +138,10 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
         for (int x = 0; x < D; ++x) dq[i * D + x] = NAN;
         return;
     }
-    Basis1<D> B;
-    make_basis<D>(loc, B);
     double val, gr[D], H[D][D];
-    val_grad_hess<D, TC>(table + (flat * ninterp + kn) * K, B, val, gr, H);
+    const TC* a = table + (flat * ninterp + kn) * K;
+    if constexpr (D == 3) hess3<TC>(a, loc[0], loc[1], loc[2], val, gr, H);
+    else hess4<TC>(a, loc[0], loc[1], loc[2], loc[3], val, gr, H);
     const double v = gv[i];
 #pragma unroll
     for (int x = 0; x < D; ++x) {
@@ -143,24 +177,16 @@ __global__ void k_seg_sort(const int32_t* __restrict__ offs, int32_t* __restrict
     }
 }
 
-// One wave per cell: alpha_bar (K) summed over the cell's queries in index
-// order, then b_bar = A^T alpha_bar, written to bbar[cell][K].
-template <int D>
-__global__ void __launch_bounds__(256)
-k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
-            const double* __restrict__ gv, const double* __restrict__ gg,
-            const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
-            double* __restrict__ bbar) {
-    constexpr int K = 1 << (2 * D);
-    constexpr int PER = K / 64;
-    __shared__ double abar[4][K];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t c = (int64_t)blockIdx.x * 4 + w;
-    const bool live = c < g.nc;
-    const int32_t s0 = live ? offs[c] : 0, s1 = live ? offs[c + 1] : 0;
-    double acc[PER];
+// alpha_bar[j] of one cell for the entries j = j0 + stride*r (r < R): the sum
+// over the cell's queries, in query-index order, of gv m_j + sum_x gk_x dm_j/dx.
+template <int D, int R>
+__device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __restrict__ q,
+                                          int64_t ldq, const double* __restrict__ gv,
+                                          const double* __restrict__ gg, const int32_t* perm,
+                                          int32_t s0, int32_t s1, int j0, int stride,
+                                          double* acc) {
 #pragma unroll
-    for (int r = 0; r < PER; ++r) acc[r] = 0.0;
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
     for (int32_t s = s0; s < s1; ++s) {
         const int64_t i = perm[s];
         double loc[D];
@@ -173,8 +199,8 @@ k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ 
 #pragma unroll
         for (int x = 0; x < D; ++x) gk[x] = gg[i * D + x] / g.h[x];
 #pragma unroll
-        for (int r = 0; r < PER; ++r) {
-            const int j = lane + 64 * r;
+        for (int r = 0; r < R; ++r) {
+            const int j = j0 + stride * r;
             int e[D];
 #pragma unroll
             for (int x = 0; x < D; ++x) e[x] = (j >> (2 * x)) & 3;
@@ -193,22 +219,101 @@ k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ 
             acc[r] += t;
         }
     }
+}
+
+// 3D: a workgroup takes 16 consecutive cells.  16 threads per cell form
+// alpha_bar into LDS: thread (ex, ey) sums, over the cell's queries in index
+// order, the four entries ex + 4 ey + 16 ez,
+//   gv u^ex v^ey w^ez + gk_u (u^ex)' v^ey w^ez + gk_v u^ex (v^ey)' w^ez
+//                     + gk_w u^ex v^ey (w^ez)'      (gk = gg / h).
+// Then wave w computes rows w*16..w*16+15 of b_bar = A^T alpha_bar for the
+// 16 cells on v_mfma_f64_16x16x4f64 and writes bbar[l][cell] (16 consecutive
+// cells per row: coalesced).
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void pow3(double x, double* p, double* dp) {
+    p[0] = 1.0; p[1] = x; p[2] = x * x; p[3] = p[2] * x;
+    dp[0] = 0.0; dp[1] = 1.0; dp[2] = 2.0 * x; dp[3] = 3.0 * p[2];
+}
+
+__global__ void __launch_bounds__(256)
+k_bwd_cells3(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
+             const double* __restrict__ gv, const double* __restrict__ gg,
+             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
+             double* __restrict__ bbar) {
+    __shared__ double ab[16][65];                  // alpha_bar[cell][k], padded
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t c0 = (int64_t)blockIdx.x * 16;
+    {
+        const int cl = threadIdx.x >> 4;
+        const int exy = threadIdx.x & 15, ex = exy & 3, ey = exy >> 2;
+        const int64_t c = c0 + cl;
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c < g.nc) {
+            const int32_t s1 = offs[c + 1];
+            for (int32_t s = offs[c]; s < s1; ++s) {
+                const int64_t i = perm[s];
+                double loc[3];
+                bool oob;
+                (void)arb_reduce<3>(q + i * ldq, g, loc, &oob);
+                double pu[4], du[4], pv[4], dv[4], pw[4], dw[4];
+                pow3(loc[0], pu, du);
+                pow3(loc[1], pv, dv);
+                pow3(loc[2], pw, dw);
+                const double gu = gg[i * 3] / g.h[0], gvv = gg[i * 3 + 1] / g.h[1],
+                             gw = gg[i * 3 + 2] / g.h[2];
+                const double a = sel4(pu, ex) * sel4(pv, ey);
+                const double au = sel4(du, ex) * sel4(pv, ey);
+                const double av = sel4(pu, ex) * sel4(dv, ey);
+                const double base = fma(gvv, av, fma(gu, au, gv[i] * a));
+#pragma unroll
+                for (int ez = 0; ez < 4; ++ez) acc[ez] += fma(gw * a, dw[ez], base * pw[ez]);
+            }
+        }
+#pragma unroll
+        for (int ez = 0; ez < 4; ++ez) ab[cl][exy + 16 * ez] = acc[ez];
+    }
+    __syncthreads();
+    d4 d = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks)             // A^T[w*16 + l%16][ks*4 + l/16]
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(ks * 4 + (lane >> 4)) * 64 + w * 16 + (lane & 15)],
+                                                 ab[lane & 15][ks * 4 + (lane >> 4)], d, 0, 0, 0);
+    const int64_t cell = c0 + (lane & 15);
+    if (cell < g.nc) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) bbar[(int64_t)(w * 16 + (lane >> 4) + 4 * r) * g.nc + cell] = d[r];
+    }
+}
+
+// 4D: one wave per cell: alpha_bar (256) summed over the cell's queries in
+// index order, then b_bar = A^T alpha_bar, written slot-major bbar[l][cell].
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
+            const double* __restrict__ gv, const double* __restrict__ gg,
+            const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
+            double* __restrict__ bbar) {
+    constexpr int K = 1 << (2 * D);
+    constexpr int PER = K / 64;
+    __shared__ double abar[4][K];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * 4 + w;
+    const bool live = c < g.nc;
+    const int32_t s0 = live ? offs[c] : 0, s1 = live ? offs[c + 1] : 0;
+    double acc[PER];
+    cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
 #pragma unroll
     for (int r = 0; r < PER; ++r) abar[w][lane + 64 * r] = acc[r];
     __syncthreads();
     if (!live) return;
-    double* out = bbar + c * K;
-    if (s0 == s1) {
-#pragma unroll
-        for (int r = 0; r < PER; ++r) out[lane + 64 * r] = 0.0;
-        return;
-    }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         const int l = lane + 64 * r;
         double b = 0.0;
-        for (int j = 0; j < K; ++j) b += A[(int64_t)j * K + l] * abar[w][j];
-        out[l] = b;
+        if (s0 != s1)
+            for (int j = 0; j < K; ++j) b += A[(int64_t)j * K + l] * abar[w][j];
+        bbar[(int64_t)l * g.nc + c] = b;
     }
 }
 
@@ -237,7 +342,7 @@ k_bwd_nodes(ArbGeom g, const double* __restrict__ bbar, double* __restrict__ dva
             cell += ca * mul;
             mul *= g.npos[a];
         }
-        if (ok) sum += bbar[cell * K + l];
+        if (ok) sum += bbar[(int64_t)l * g.nc + cell];
     }
     dval[nidx] = sum;
 }
@@ -285,11 +390,12 @@ hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, h
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const int32_t* offs,
                                 const int32_t* perm, double* bbar, hipStream_t s) {
-    dim3 grid(grid1(g.nc, 4)), block(256);
     if (g.ndim == 3)
-        hipLaunchKernelGGL(k_bwd_cells<3>, grid, block, 0, s, g, A, q, ldq, gv, gg, offs, perm, bbar);
+        hipLaunchKernelGGL(k_bwd_cells3, dim3(grid1(g.nc, 16)), dim3(256), 0, s, g, A, q, ldq, gv,
+                           gg, offs, perm, bbar);
     else
-        hipLaunchKernelGGL(k_bwd_cells<4>, grid, block, 0, s, g, A, q, ldq, gv, gg, offs, perm, bbar);
+        hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, 4)), dim3(256), 0, s, g, A, q, ldq, gv,
+                           gg, offs, perm, bbar);
     return hipGetLastError();
 }
 

@@ -1,13 +1,19 @@
-// Coefficient table build on the matrix cores (gfx950).
+// Coefficient table builds (calcCoefficients*, allCoeffs: ARBInterp.py:523-525,
+// 570-603; 4D :1260-1262, 1322-1355): alpha = A . F for every cell.
 //
-// Mixed precision (ARB_PREC_MIXED): for every cell the b-vector of
-// finite-difference data (f, fx, fy, fz, fxy, fxz, fyz, fxyz at the 8 corners;
-// 16 blocks x 16 corners in 4D, ARBInterp.py:118-125 / :738-757) is formed in
-// FP64 from an LDS tile of node values, then alpha = inv(B) . b is one dense
-// contraction per tile of 16 cells:
-//     (K x K) x (K x 16),  K = 64 (3D) or 256 (4D)
-// on v_mfma_f32_16x16x32_bf16 with FP32 accumulation, made exact:
-//   * the cell's base value f(corner 0) is subtracted from its 8 (16) corner
+// 3D, both precisions (k_coef_sep3): A = M (x) M (x) M, three separable 4-tap
+// passes in FP64 over an LDS node tile (arb_sep3.h); stored as FP64, or
+// rounded once to FP32 for mixed precision.
+//
+// 4D, FP64 (k_coef_f64mfma4): dense alpha = A . F on v_mfma_f64_16x16x4f64.
+//
+// 4D, mixed (k_coef_bf16x3<4>): the reference's row shift in the d4/dxdydzdt
+// block (ARBInterp.py:860) breaks the Kronecker form, so alpha = inv(B) . b
+// with b the FD b-vector (16 blocks x 16 corners, ARBInterp.py:738-757),
+// formed in FP64 from an LDS tile of node values, one dense contraction per
+// tile of 16 cells, (256 x 256) x (256 x 16), on v_mfma_f32_16x16x32_bf16
+// with FP32 accumulation, made exact:
+//   * the cell's base value f(corner 0) is subtracted from its 16 corner
 //     values (inv(B) maps constant data to e_0, so alpha_0 takes it back);
 //   * the residual b' is scaled by a power of two 2^-E so |b'| < 1 and split
 //     on a common fixed-point grid into four 8-bit integer parts,
@@ -15,24 +21,24 @@
 //     |p_k| <= 256, exact in BF16;
 //   * inv(B) is integer (|.| <= 81), exact in BF16.
 // Every product is then an integer below 2^15 and every partial sum an
-// integer below 2^23 (K * 81 * 256), so each part's MFMA chain is exact in
+// integer below 2^23 (256 * 81 * 256), so each part's MFMA chain is exact in
 // FP32 whatever its internal order.  The four accumulators are combined in
 // FP64 (exact), scaled back by 2^E and rounded once to FP32: the stored
 // alpha is the FP64 alpha of the quantised b' (quantisation < 2^-33 of the
 // cell's largest residual) correctly rounded to FP32.
-//
 // Work split: one workgroup = 4 waves = a 16(x) x 4(y) tile of cells at one
-// z (and t); wave w owns cell row y0+w, lane l the cell x0 + (l & 15).  The
-// node tile (19 x 7 x 4 [x 4]) is loaded once per interpolant into LDS with
+// (z, t); wave w owns cell row y0+w, lane l the cell x0 + (l & 15).  The node
+// tile (19 x 7 x 4 x 4) is loaded once per interpolant into LDS with
 // row-contiguous reads.
 #include "arb_internal.h"
-#include "arb_coef3.h"
+#include "arb_sep3.h"
 
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int TXC = 16, TYC = 4;            // cells per tile
 constexpr int TX = TXC + 3, TY = TYC + 3;   // nodes per tile
@@ -195,110 +201,38 @@ k_coef_bf16x3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
     }
 }
 
-// Mixed precision, 3D: alpha = A . F directly from the node tile (no
-// finite-difference stage), by arbc3::coef3_mixed; the tiling is the one
-// above.  Af is 16 A as BF16 in operand order.
+// 3D, both precisions: alpha = (M (x) M (x) M) . F by separable passes
+// (arbs3::sep3_passes) over an 8 x 4 x 1 tile of cells per workgroup, in
+// FP64, stored as TO (FP64 table, or FP32 rounded once for mixed).
+constexpr int SX = 8, SY = 4;
+
+template <typename TO>
 __global__ void __launch_bounds__(256)
-k_coef_af3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
-           const unsigned short* __restrict__ Af, float* __restrict__ table) {
-    __shared__ double tile[TileGeom<3>::NODES];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int lx = lane & 15;
-    const int grp = lane >> 4;
-    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
-    const int64_t cy0 = (int64_t)blockIdx.y * TYC;
+k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
+    constexpr int NX = SX + 3, NY = SY + 3, NZ = 4;
+    __shared__ double nodes[NX * NY * NZ];
+    __shared__ double t1[NY * NZ * SX * 4];
+    __shared__ double t2[NZ * SY * SX * 16];
+    const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
     const int64_t cz = blockIdx.z;
-    const int64_t cx = cx0 + lx, cy = cy0 + w;
-    const bool valid = (cx < g.npos[0]) && (cy < g.npos[1]);
-    const int lxc = valid ? lx : 0;
-    const int64_t cell = cx + g.npos[0] * (cy + g.npos[1] * cz);
     for (int k = 0; k < ninterp; ++k) {
         const double* F = vals + (int64_t)k * g.nnodes;
-        __syncthreads();
-        for (int i = threadIdx.x; i < TileGeom<3>::NODES; i += 256) {
-            const int x = i % TX;
-            const int y = (i / TX) % TY;
-            const int z = i / (TX * TY);
+        __syncthreads();                            // the previous interpolant is done
+        for (int i = threadIdx.x; i < NX * NY * NZ; i += blockDim.x) {
+            const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
             const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
             double v = 0.0;
             if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
-            tile[i] = v;
+            nodes[i] = v;
         }
         __syncthreads();
-        auto nb = [&](int kk) {
-            return tile[((kk >> 4) * TY + (w + ((kk >> 2) & 3))) * TX + (lxc + (kk & 3))];
-        };
-        float* out = table + (cell * ninterp + k) * 64 + 4 * grp;
-        arbc3::coef3_mixed(nb, Af, [&](int mt, const arbc3::f4& v) {
-            if (valid) *reinterpret_cast<arbc3::f4*>(out + mt * 16) = v;
-        });
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Plain FP64 (ARB_PREC_FP64), 3D: alpha = A . F with A = inv(B).D (exact
-// dyadic, the reference's A of ARBInterp.py:175) and F the 64 neighbourhood
-// values of each cell, on v_mfma_f64_16x16x4f64.  Same tiling as above:
-// 4 waves x 16 cells; per wave 4 row tiles x 16 k-steps; each lane keeps its
-// 64 A operands in registers; F operands come from the LDS node tile.
-// Operands (lane l): A[row l%16][k l/16], B[k l/16][col l%16]; accumulator
-// register r of lane l holds D[row l/16 + 4r][col l%16].
-// ---------------------------------------------------------------------------
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-__global__ void __launch_bounds__(256)
-k_coef_f64mfma3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp,
-                const double* __restrict__ A, double* __restrict__ table) {
-    __shared__ double tile[TileGeom<3>::NODES];
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int lx = lane & 15, kq = lane >> 4;
-    const int64_t cx0 = (int64_t)blockIdx.x * TXC;
-    const int64_t cy0 = (int64_t)blockIdx.y * TYC;
-    const int64_t cz = blockIdx.z;
-    const int64_t cy = cy0 + w;
-    double aop[4][16];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks) aop[mt][ks] = A[(mt * 16 + lx) * 64 + ks * 4 + kq];
-    const bool colvalid = (cx0 + lx) < g.npos[0];
-    const int lxc = colvalid ? lx : 0;
-    const bool rowvalid = cy < g.npos[1];
-    const int64_t cell = (cx0 + lx) + g.npos[0] * (cy + g.npos[1] * cz);
-    for (int k = 0; k < ninterp; ++k) {
-        const double* F = vals + (int64_t)k * g.nnodes;
-        __syncthreads();
-        for (int i = threadIdx.x; i < TileGeom<3>::NODES; i += 256) {
-            const int x = i % TX;
-            const int y = (i / TX) % TY;
-            const int z = i / (TX * TY);
-            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
-            double v = 0.0;
-            if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
-            tile[i] = v;
-        }
-        __syncthreads();
-        double bop[16];
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks) {
-            const int j = ks * 4 + kq;
-            const int ox = j & 3, oy = (j >> 2) & 3, oz = j >> 4;
-            bop[ks] = tile[(oz * TY + (w + oy)) * TX + (lxc + ox)];
-        }
-        double* out = table + (cell * ninterp + k) * 64;
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {          // unrolled: aop stays in registers
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < 16; ++ks)
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[mt][ks], bop[ks], acc, 0, 0, 0);
-            if (rowvalid && colvalid) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) out[mt * 16 + kq + 4 * r] = acc[r];
+        arbs3::sep3_passes(nodes, t1, t2, SX, SY, 1, [&](int cx, int cy, int, int i, double a) {
+            const int64_t x = cx0 + cx, y = cy0 + cy;
+            if (x < g.npos[0] && y < g.npos[1]) {
+                const int64_t cell = x + g.npos[0] * (y + g.npos[1] * cz);
+                table[(cell * ninterp + k) * 64 + i] = (TO)a;
             }
-        }
+        });
     }
 }
 
@@ -364,8 +298,8 @@ hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t
                                    const double* A, double* table, hipStream_t s) {
     if (g.nc == 0) return hipSuccess;
     if (g.ndim == 3) {
-        dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]), block(256);
-        hipLaunchKernelGGL(k_coef_f64mfma3, grid, block, 0, s, g, vals, ninterp, A, table);
+        dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)g.npos[2]), block(256);
+        hipLaunchKernelGGL(k_coef_sep3<double>, grid, block, 0, s, g, vals, ninterp, table);
     } else {
         dim3 grid(cdiv(g.npos[0], TXC), (unsigned)g.npos[1], (unsigned)(g.npos[2] * g.npos[3]));
         hipLaunchKernelGGL(k_coef_f64mfma4, grid, dim3(256), 0, s, g, vals, ninterp, A, table);
@@ -373,15 +307,15 @@ hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t
     return hipGetLastError();
 }
 
-// Ab: BF16 operand table in order [mt][ks][lane][8] (host-prepared): 16 A for
-// 3D (k_coef_af3), inv(B) for 4D (k_coef_bf16x3<4>).
+// 3D: separable passes in FP64, rounded to FP32.  4D: Ab = inv(B) as BF16 in
+// operand order [mt][ks][lane][8] (host-prepared), k_coef_bf16x3<4>.
 hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
                                   const unsigned short* Ab, float* table, hipStream_t s) {
     if (g.nc == 0) return hipSuccess;
     dim3 block(256);
     if (g.ndim == 3) {
-        dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)g.npos[2]);
-        hipLaunchKernelGGL(k_coef_af3, grid, block, 0, s, g, vals, ninterp, Ab, table);
+        dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)g.npos[2]);
+        hipLaunchKernelGGL(k_coef_sep3<float>, grid, block, 0, s, g, vals, ninterp, table);
     } else {
         dim3 grid(cdiv(g.npos[0], TXC), cdiv(g.npos[1], TYC), (unsigned)(g.npos[2] * g.npos[3]));
         hipLaunchKernelGGL(k_coef_bf16x3<4>, grid, block, 0, s, g, vals, ninterp, Ab, table);

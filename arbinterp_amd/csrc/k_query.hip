@@ -5,7 +5,7 @@
 // gradients.  The reference builds ~40 N x 64 temporaries per call; here each
 // query is one lane.
 #include "arb_internal.h"
-#include "arb_coef3.h"
+#include "arb_sep3.h"
 
 namespace {
 
@@ -417,21 +417,19 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
     }
 }
 
-// Mixed precision, 3D, without the coefficient table: one workgroup per
-// bucket loads the bucket's node values (its cells + 3 per axis, every
-// interpolant, FP64) into LDS, forms the coefficients of its cells on the
-// matrix cores (arbc3::coef3_mixed, the function the table build uses, so
-// the results are bitwise those of the table), writes them into the same LDS
-// block layout the staged kernel uses, and evaluates the bucket's queries.
-// Work items (interpolant, column tile of 16 cells) are spread over the
-// waves.  Dynamic LDS: ncell * cellstride * 16 bytes of coefficients, then
-// ninterp * (b0+3)(b1+3)(b2+3) doubles of node values.
-template <typename TO>
+// 3D without the coefficient table: one workgroup per bucket forms its
+// cells' coefficients from node values (per interpolant: node tile, cells + 3
+// per axis, into LDS, then the separable passes of arb_sep3.h, the routine
+// the table build uses, so the results are bitwise those of the table),
+// writes them into the same LDS block layout the staged kernel uses, and
+// evaluates the bucket's queries.  Dynamic LDS: ncell * cellstride * 16
+// bytes of coefficients, then the node tile and pass buffers (doubles).
+template <typename T, typename TO>
 __global__ void __launch_bounds__(256)
 k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
-              const unsigned short* __restrict__ Af, int32_t has_comps, int32_t has_norm,
-              const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
+              int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
+              int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
+              const int32_t* __restrict__ perm) {
     extern __shared__ uint4 smem[];
     const int64_t b = blockIdx.x;
     const int32_t start = offs[b], end = offs[b + 1];
@@ -445,10 +443,14 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
             rem /= B.nb[a];
         }
     }
-    const int NX = B.b[0] + 3, NY = B.b[1] + 3, NZ = B.b[2] + 3;
-    const int tile_nodes = NX * NY * NZ;
+    const int bx = B.b[0], by = B.b[1], bz = B.b[2];
+    const int NX = bx + 3, NY = by + 3, NZ = bz + 3;
     double* nodes = reinterpret_cast<double*>(smem + B.ncell * B.cellstride);
-    // this thread's first query, loaded before the node tile
+    double* t1 = nodes + arbs3::sep3_nodes(bx, by, bz);
+    double* t2 = t1 + arbs3::sep3_t1(bx, by, bz);
+    T* cf = reinterpret_cast<T*>(smem);
+    const int cstr = B.cellstride * (16 / (int)sizeof(T));   // elements per LDS cell
+    // this thread's first query, loaded before the coefficient work
     int32_t s = start + threadIdx.x;
     int64_t qi = -1;
     double qp[3];
@@ -459,52 +461,32 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 #pragma unroll
         for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
     }
-    // node tile, x fastest; nodes past the grid edge read as 0 (only cells
-    // outside the grid use them)
-    for (int i = threadIdx.x; i < ninterp * tile_nodes; i += blockDim.x) {
-        const int k = i / tile_nodes;
-        const int r = i - k * tile_nodes;
-        const int x = r % NX;
-        const int y = (r / NX) % NY;
-        const int z = r / (NX * NY);
-        const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
-        double v = 0.0;
-        if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
-            v = vals[(int64_t)k * g.nnodes + nx + g.nstride[1] * ny + g.nstride[2] * nz];
-        nodes[i] = v;
-    }
-    __syncthreads();
-    // coefficients of every (interpolant, 16-cell column tile), wave-uniform
-    {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-        const int ntile = (B.ncell + 15) >> 4;
-        float* cf = reinterpret_cast<float*>(smem);
-        for (int it = w; it < ninterp * ntile; it += nw) {
-            const int k = it / ntile;
-            const int lc = (it - k * ntile) * 16 + (lane & 15);
-            const bool live = lc < B.ncell;
-            const int lcc = live ? lc : 0;
-            const int lx = (lcc >> B.lsh[0]) & (B.b[0] - 1);
-            const int ly = (lcc >> B.lsh[1]) & (B.b[1] - 1);
-            const int lz = (lcc >> B.lsh[2]) & (B.b[2] - 1);
-            const double* nk = nodes + k * tile_nodes;
-            auto nb = [&](int kk) {
-                return nk[((lz + (kk >> 4)) * NY + (ly + ((kk >> 2) & 3))) * NX + (lx + (kk & 3))];
-            };
-            float* dst = cf + (size_t)lcc * B.cellstride * 4 + k * 64 + 4 * (lane >> 4);
-            arbc3::coef3_mixed(nb, Af, [&](int mt, const arbc3::f4& v) {
-                if (live) *reinterpret_cast<arbc3::f4*>(dst + mt * 16) = v;
-            });
+    for (int k = 0; k < ninterp; ++k) {
+        __syncthreads();                            // the previous passes are done
+        // node tile, x fastest; nodes past the grid edge read as 0 (only
+        // cells outside the grid use them)
+        for (int i = threadIdx.x; i < NX * NY * NZ; i += blockDim.x) {
+            const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
+            const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
+                v = vals[(int64_t)k * g.nnodes + nx + g.nstride[1] * ny + g.nstride[2] * nz];
+            nodes[i] = v;
         }
+        __syncthreads();
+        arbs3::sep3_passes(nodes, t1, t2, bx, by, bz, [&](int cx, int cy, int cz, int i, double a) {
+            const int lc = cx | (cy << B.lsh[1]) | (cz << B.lsh[2]);
+            cf[lc * cstr + k * 64 + i] = (T)a;
+        });
     }
     __syncthreads();
-    if (qi >= 0) eval_row_lds<3, float, float, TO>(g, B, smem, has_comps, has_norm, o, b, qi, qp);
+    if (qi >= 0) eval_row_lds<3, T, T, TO>(g, B, smem, has_comps, has_norm, o, b, qi, qp);
     for (s += blockDim.x; s < end; s += blockDim.x) {
         const int64_t pi = perm[s];
         if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
 #pragma unroll
         for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
-        eval_row_lds<3, float, float, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
+        eval_row_lds<3, T, T, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
     }
 }
 
@@ -620,28 +602,32 @@ hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t
 }
 
 // LDS bytes of k_eval_fused3 for the bucket geometry B (0 when the path does
-// not apply: 3D mixed only, block within 64 KiB).
-size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp) {
+// not apply: 3D only, within 64 KiB).
+size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B) {
     if (g.ndim != 3) return 0;
+    const int bx = B.b[0], by = B.b[1], bz = B.b[2];
     const size_t lds = (size_t)B.ncell * B.cellstride * 16 +
-                       (size_t)ninterp * (B.b[0] + 3) * (B.b[1] + 3) * (B.b[2] + 3) * 8;
+                       sizeof(double) * (size_t)(arbs3::sep3_nodes(bx, by, bz) +
+                                                 arbs3::sep3_t1(bx, by, bz) +
+                                                 arbs3::sep3_t2(bx, by, bz));
     return lds <= 64 * 1024 ? lds : 0;
 }
 
-hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t out_f64,
-                                 const double* vals, int32_t ninterp, const unsigned short* Af,
+hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                 int32_t out_f64, const double* vals, int32_t ninterp,
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
-    const size_t lds = arb_fused_lds(g, B, ninterp);
+    const size_t lds = arb_fused_lds(g, B);
     if (lds == 0) return hipErrorInvalidValue;
     dim3 grid((unsigned)B.nbt), block(block_threads);
-    if (out_f64)
-        hipLaunchKernelGGL(k_eval_fused3<double>, grid, block, lds, s, g, B, vals, ninterp, Af,
-                           has_comps, has_norm, q, n, ldq, o, offs, perm);
-    else
-        hipLaunchKernelGGL(k_eval_fused3<float>, grid, block, lds, s, g, B, vals, ninterp, Af,
-                           has_comps, has_norm, q, n, ldq, o, offs, perm);
+#define L_FUSED(T, TO)                                                                        \
+    hipLaunchKernelGGL((k_eval_fused3<T, TO>), grid, block, lds, s, g, B, vals, ninterp,      \
+                       has_comps, has_norm, q, n, ldq, o, offs, perm)
+    if (prec == ARB_PREC_FP64) L_FUSED(double, double);
+    else if (out_f64) L_FUSED(float, double);
+    else L_FUSED(float, float);
+#undef L_FUSED
     return hipGetLastError();
 }
 

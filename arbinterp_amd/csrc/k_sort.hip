@@ -168,6 +168,7 @@ k_sc_local(const int32_t* __restrict__ in, int64_t len, const int32_t* __restric
         if (base + j < len) out[base + j] = run;
         run += v[j];
     }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[len] = bscan[gridDim.x];   // total
 }
 
 // offs[b] = first sorted position whose bucket (key >> shift) >= b, for b in
@@ -188,8 +189,9 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 int64_t arb_scan_tmp_elems(int64_t len) { return (int64_t)cdiv(len, SC_BLOCK) + 1; }
 
-// out[0..len] exclusive scan of in[0..len); tmp holds arb_scan_tmp_elems(len)
-// ints (block sums, scanned in place).
+// out[0..len] (len + 1 entries, out[len] = the total) exclusive scan of
+// in[0..len); tmp and tmp_scan hold arb_scan_tmp_elems(len) ints each (block
+// sums and their scan).
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s) {
     const unsigned nblk = cdiv(len, SC_BLOCK);

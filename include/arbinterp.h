@@ -80,11 +80,11 @@ extern "C" {
                                arb_field_info.packed_width elements into
                                comps_dev: [comps(3)] [value, grad(d)], zero
                                padded; norms_dev/grads_dev are not used        */
-#define ARB_Q_FUSED  1024   /* mixed-precision 3D bucketed path: form each
-                               block's coefficients in the query kernel from
-                               node values instead of staging them from the
-                               table (default).  No table is built or read;
-                               the results are bitwise the same              */
+#define ARB_Q_FUSED  1024   /* 3D bucketed path: form each block's
+                               coefficients in the query kernel from node
+                               values instead of staging them from the table
+                               (default).  No table is built or read; the
+                               results are bitwise the same                  */
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
 
