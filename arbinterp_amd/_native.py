@@ -78,6 +78,8 @@ SIGNATURES = [
     ("arb_last_timings", c_i32, [c_vp, ctypes.POINTER(ctypes.c_float)]),
     ("arb_set_tuning", c_i32, [c_vp, ctypes.POINTER(c_i32), c_i32]),
     ("arb_operator_matrix", c_i32, [c_i32, c_i32, c_vp]),
+    ("arb_csv_shape", c_i32, [ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)]),
+    ("arb_csv_read", c_i32, [ctypes.c_char_p, c_vp, c_i64, c_i32]),
     ("arb_field_destroy", None, [c_vp]),
     ("arb_last_error", ctypes.c_char_p, []),
     ("arb_version", ctypes.c_char_p, []),

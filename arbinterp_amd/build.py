@@ -21,8 +21,8 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["arb_capi.hip", "k_grid.hip", "k_coef.hip", "k_coef_mfma.hip", "k_query.hip",
-           "k_backward.hip", "k_batch.hip", "k_sort.hip"]
+SOURCES = ["arb_capi.hip", "k_grid.hip", "k_coef.hip", "k_query.hip",
+           "k_backward.hip", "k_batch.hip", "k_sort.hip", "arb_io.hip"]
 
 CFLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

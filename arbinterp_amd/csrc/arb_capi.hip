@@ -175,7 +175,6 @@ struct arb_field {
     void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
     int64_t table_bytes = 0;
     double* A = nullptr;        // K x K FP64
-    unsigned short* Ab = nullptr;  // inv(B) as BF16 in MFMA operand order (mixed 4D)
     int32_t* flag = nullptr;    // device status word
     // bucketed query path
     ArbBuckets B{};
@@ -199,6 +198,7 @@ struct arb_field {
     // backward workspace
     int32_t* bw_key = nullptr;     // per query: cell (-1 outside)
     int32_t* bw_perm = nullptr;    // queries grouped per cell, index order
+    double* bw_rec = nullptr;      // 3D: per query u, v, w, gv, gg / h (8 doubles)
     int64_t bw_n = 0;
     int32_t* bw_count = nullptr;   // nc + 1
     int32_t* bw_offs = nullptr;    // nc + 1
@@ -454,31 +454,6 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     matmul(K, Bi, D, A);
     HIPCHK(hipMalloc(&f->A, sizeof(double) * K * K));
     HIPCHK(hipMemcpyAsync(f->A, A.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, s));
-    std::vector<unsigned short> Ab;
-    if (precision == ARB_PREC_MIXED && ndim == 4) {
-        // inv(B) in operand order [mt][ks][lane][j]: row mt*16 + (lane & 15),
-        // column ks*32 + 8*(lane >> 4) + j.  Entries are integers of magnitude
-        // <= 81: exact in BF16.
-        const int KS = K / 32, MT = K / 16;
-        const double* src = Bi.data();
-        const double scale = 1.0;
-        Ab.resize((size_t)K * K);
-        for (int mt = 0; mt < MT; ++mt)
-            for (int ks = 0; ks < KS; ++ks)
-                for (int l = 0; l < 64; ++l)
-                    for (int j = 0; j < 8; ++j) {
-                        const double e = scale * src[(size_t)(mt * 16 + (l & 15)) * K + ks * 32 + 8 * (l >> 4) + j];
-                        if (e != std::rint(e) || std::fabs(e) > 256.0)
-                            return fail(ARB_EINVAL, "BF16 operand table entry %g is not a small integer", e);
-                        float v = (float)e;
-                        uint32_t u;
-                        std::memcpy(&u, &v, 4);
-                        Ab[(((size_t)mt * KS + ks) * 64 + l) * 8 + j] = (unsigned short)(u >> 16);
-                    }
-        HIPCHK(hipMalloc(&f->Ab, sizeof(unsigned short) * Ab.size()));
-        HIPCHK(hipMemcpyAsync(f->Ab, Ab.data(), sizeof(unsigned short) * Ab.size(),
-                              hipMemcpyHostToDevice, s));
-    }
     const size_t esz = precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
     f->table_bytes = (int64_t)((g.nc + 1) * f->ninterp * K * esz);
     HIPCHK(hipMalloc(&f->table, (size_t)f->table_bytes));
@@ -528,11 +503,7 @@ int arb_build_coeffs(arb_field* f, void* stream) {
     if (!f) return fail(ARB_EINVAL, "arb_build_coeffs: NULL handle");
     if (f->table_built) return ARB_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (f->precision == ARB_PREC_FP64) {
-        HIPCHK(arb_launch_coef_f64mfma(f->g, f->vals, f->ninterp, f->A, (double*)f->table, s));
-    } else {
-        HIPCHK(arb_launch_coef_bf16x3(f->g, f->vals, f->ninterp, f->Ab, (float*)f->table, s));
-    }
+    HIPCHK(arb_launch_coef(f->g, f->precision, f->vals, f->ninterp, f->table, s));
     int rc = ensure_sentinel(f, s);
     if (rc) return rc;
     f->table_built = 1;
@@ -691,10 +662,13 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
         if (n > f->bw_n) {
             if (f->bw_key) (void)hipFree(f->bw_key);
             if (f->bw_perm) (void)hipFree(f->bw_perm);
+            if (f->bw_rec) (void)hipFree(f->bw_rec);
             f->bw_key = f->bw_perm = nullptr;
+            f->bw_rec = nullptr;
             f->bw_n = 0;
             HIPCHK(hipMalloc(&f->bw_key, sizeof(int32_t) * (n ? n : 1)));
             HIPCHK(hipMalloc(&f->bw_perm, sizeof(int32_t) * (n ? n : 1)));
+            if (f->g.ndim == 3) HIPCHK(hipMalloc(&f->bw_rec, sizeof(double) * 8 * (n ? n : 1)));
             f->bw_n = n;
         }
         if (!f->bw_bbar) {
@@ -711,14 +685,15 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     if (n > 0)
         HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
                                      q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
-                                     dval_dev ? f->bw_key : nullptr, f->bw_count, s));
+                                     dval_dev ? f->bw_key : nullptr, f->bw_count,
+                                     dval_dev ? f->bw_rec : nullptr, s));
     if (dval_dev) {
         HIPCHK(arb_launch_scan_long(f->bw_count, f->bw_offs, nc, f->bw_tmp,
                                     f->bw_tmp + arb_scan_tmp_elems(nc) + 1, s));
         HIPCHK(arb_launch_seg_scatter(f->bw_key, f->bw_offs, f->bw_cursor, f->bw_perm, n, s));
         HIPCHK(arb_launch_seg_sort(f->bw_offs, f->bw_perm, nc, s));
-        HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_offs,
-                                    f->bw_perm, f->bw_bbar, s));
+        HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_rec,
+                                    f->bw_offs, f->bw_perm, f->bw_bbar, s));
         HIPCHK(arb_launch_bwd_nodes(f->g, f->bw_bbar, dval_dev, s));
     }
     return ARB_OK;
@@ -978,7 +953,6 @@ void arb_field_destroy(arb_field* f) {
     if (f->vals) (void)hipFree(f->vals);
     if (f->table) (void)hipFree(f->table);
     if (f->A) (void)hipFree(f->A);
-    if (f->Ab) (void)hipFree(f->Ab);
     if (f->flag) (void)hipFree(f->flag);
     if (f->ws_bkt) (void)hipFree(f->ws_bkt);
     if (f->ws_rank) (void)hipFree(f->ws_rank);
@@ -995,6 +969,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->rs_tmps) (void)hipFree(f->rs_tmps);
     if (f->bw_key) (void)hipFree(f->bw_key);
     if (f->bw_perm) (void)hipFree(f->bw_perm);
+    if (f->bw_rec) (void)hipFree(f->bw_rec);
     if (f->bw_count) (void)hipFree(f->bw_count);
     if (f->bw_offs) (void)hipFree(f->bw_offs);
     if (f->bw_cursor) (void)hipFree(f->bw_cursor);

@@ -227,10 +227,8 @@ hipError_t arb_launch_rank_scatter(const double* rows, int64_t nrows, int64_t ld
                                    int32_t* seen, hipStream_t s);
 hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* flag, hipStream_t s);
 
-hipError_t arb_launch_coef_bf16x3(const ArbGeom& g, const double* vals, int32_t ninterp,
-                                  const unsigned short* Ab, float* table, hipStream_t s);
-hipError_t arb_launch_coef_f64mfma(const ArbGeom& g, const double* vals, int32_t ninterp,
-                                   const double* A, double* table, hipStream_t s);
+hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, int32_t ninterp,
+                           void* table, hipStream_t s);
 hipError_t arb_launch_sentinel(void* table, int32_t f32, int64_t nc, int32_t ninterp, int32_t K,
                                int32_t nan, hipStream_t s);
 
@@ -276,13 +274,14 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, int32_t* count, hipStream_t s);
+                                 int32_t* key, int32_t* count, double* rec, hipStream_t s);
 hipError_t arb_launch_seg_scatter(const int32_t* key, const int32_t* offs, int32_t* cursor,
                                   int32_t* perm, int64_t n, hipStream_t s);
 hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, hipStream_t s);
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
-                                const double* gv, const double* gg, const int32_t* offs,
-                                const int32_t* perm, double* bbar, hipStream_t s);
+                                const double* gv, const double* gg, const double* rec,
+                                const int32_t* offs, const int32_t* perm, double* bbar,
+                                hipStream_t s);
 hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s);
 size_t arb_batch_field_desc_size();
 void arb_batch_fill_desc(void* dst, const ArbGeom& g, const void* table, int64_t q0, int64_t nq,

@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256)
 k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t kn,
              const double* __restrict__ q, int64_t n, int64_t ldq, const double* __restrict__ gv,
              const double* __restrict__ gg, double* __restrict__ dq, int32_t* __restrict__ key,
-             int32_t* __restrict__ count) {
+             int32_t* __restrict__ count, double* __restrict__ rec) {
     constexpr int K = 1 << (2 * D);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -131,6 +131,12 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
     if (key) {
         key[i] = ok ? (int32_t)flat : -1;
         if (ok) atomicAdd(&count[flat], 1);
+    }
+    if (rec && ok) {
+        // the node adjoint's per-query record: local coordinates, gv, gg / h
+        double4* r4 = reinterpret_cast<double4*>(rec + i * 8);
+        r4[0] = make_double4(loc[0], loc[1], loc[2], gv[i]);
+        r4[1] = make_double4(gg[i * 3] / g.h[0], gg[i * 3 + 1] / g.h[1], gg[i * 3 + 2] / g.h[2], 0.0);
     }
     if (!dq) return;
     if (!ok) {
@@ -221,14 +227,15 @@ __device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __rest
     }
 }
 
-// 3D: a workgroup takes 16 consecutive cells.  16 threads per cell form
-// alpha_bar into LDS: thread (ex, ey) sums, over the cell's queries in index
-// order, the four entries ex + 4 ey + 16 ez,
+// 3D: a workgroup takes 64 consecutive cells.  Four threads per cell form
+// alpha_bar into LDS from the queries' records (k_bwd_points: u, v, w, gv,
+// gg / h): thread g of a cell sums, over the cell's queries in index order,
+// the entries ex + 4 ey + 16 ez with ex + 4 ey in 4g..4g+3,
 //   gv u^ex v^ey w^ez + gk_u (u^ex)' v^ey w^ez + gk_v u^ex (v^ey)' w^ez
 //                     + gk_w u^ex v^ey (w^ez)'      (gk = gg / h).
 // Then wave w computes rows w*16..w*16+15 of b_bar = A^T alpha_bar for the
-// 16 cells on v_mfma_f64_16x16x4f64 and writes bbar[l][cell] (16 consecutive
-// cells per row: coalesced).
+// four 16-cell column tiles on v_mfma_f64_16x16x4f64 (its A operands loaded
+// once) and writes bbar[l][cell] (16 consecutive cells per row: coalesced).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void pow3(double x, double* p, double* dp) {
@@ -236,53 +243,68 @@ __device__ __forceinline__ void pow3(double x, double* p, double* dp) {
     dp[0] = 0.0; dp[1] = 1.0; dp[2] = 2.0 * x; dp[3] = 3.0 * p[2];
 }
 
+constexpr int BWD_CELLS = 64;
+
 __global__ void __launch_bounds__(256)
-k_bwd_cells3(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
-             const double* __restrict__ gv, const double* __restrict__ gg,
+k_bwd_cells3(ArbGeom g, const double* __restrict__ A, const double* __restrict__ rec,
              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
              double* __restrict__ bbar) {
-    __shared__ double ab[16][65];                  // alpha_bar[cell][k], padded
+    __shared__ double ab[BWD_CELLS][65];           // alpha_bar[cell][k], padded
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * 16;
+    const int64_t c0 = (int64_t)blockIdx.x * BWD_CELLS;
+    double aop[16];                                // A^T[w*16 + l%16][ks*4 + l/16]
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) aop[ks] = A[(ks * 4 + (lane >> 4)) * 64 + w * 16 + (lane & 15)];
     {
-        const int cl = threadIdx.x >> 4;
-        const int exy = threadIdx.x & 15, ex = exy & 3, ey = exy >> 2;
+        const int cl = threadIdx.x >> 2, gq = threadIdx.x & 3;
         const int64_t c = c0 + cl;
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        double acc[4][4];                          // [pair][ez]
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
         if (c < g.nc) {
             const int32_t s1 = offs[c + 1];
             for (int32_t s = offs[c]; s < s1; ++s) {
                 const int64_t i = perm[s];
-                double loc[3];
-                bool oob;
-                (void)arb_reduce<3>(q + i * ldq, g, loc, &oob);
+                const double4* r4 = reinterpret_cast<const double4*>(rec + i * 8);
+                const double4 r0 = r4[0], r1 = r4[1];
                 double pu[4], du[4], pv[4], dv[4], pw[4], dw[4];
-                pow3(loc[0], pu, du);
-                pow3(loc[1], pv, dv);
-                pow3(loc[2], pw, dw);
-                const double gu = gg[i * 3] / g.h[0], gvv = gg[i * 3 + 1] / g.h[1],
-                             gw = gg[i * 3 + 2] / g.h[2];
-                const double a = sel4(pu, ex) * sel4(pv, ey);
-                const double au = sel4(du, ex) * sel4(pv, ey);
-                const double av = sel4(pu, ex) * sel4(dv, ey);
-                const double base = fma(gvv, av, fma(gu, au, gv[i] * a));
+                pow3(r0.x, pu, du);
+                pow3(r0.y, pv, dv);
+                pow3(r0.z, pw, dw);
 #pragma unroll
-                for (int ez = 0; ez < 4; ++ez) acc[ez] += fma(gw * a, dw[ez], base * pw[ez]);
+                for (int p = 0; p < 4; ++p) {
+                    const int exy = 4 * gq + p, ex = exy & 3, ey = exy >> 2;
+                    const double a = sel4(pu, ex) * sel4(pv, ey);
+                    const double au = sel4(du, ex) * sel4(pv, ey);
+                    const double av = sel4(pu, ex) * sel4(dv, ey);
+                    const double base = fma(r1.y, av, fma(r1.x, au, r0.w * a));
+                    const double gwa = r1.z * a;
+#pragma unroll
+                    for (int ez = 0; ez < 4; ++ez) acc[p][ez] += fma(gwa, dw[ez], base * pw[ez]);
+                }
             }
         }
 #pragma unroll
-        for (int ez = 0; ez < 4; ++ez) ab[cl][exy + 16 * ez] = acc[ez];
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+            for (int ez = 0; ez < 4; ++ez) ab[cl][4 * gq + p + 16 * ez] = acc[p][ez];
     }
     __syncthreads();
-    d4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int ks = 0; ks < 16; ++ks)             // A^T[w*16 + l%16][ks*4 + l/16]
-        d = __builtin_amdgcn_mfma_f64_16x16x4f64(A[(ks * 4 + (lane >> 4)) * 64 + w * 16 + (lane & 15)],
-                                                 ab[lane & 15][ks * 4 + (lane >> 4)], d, 0, 0, 0);
-    const int64_t cell = c0 + (lane & 15);
-    if (cell < g.nc) {
+    for (int ct = 0; ct < BWD_CELLS / 16; ++ct) {
+        d4 d = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) bbar[(int64_t)(w * 16 + (lane >> 4) + 4 * r) * g.nc + cell] = d[r];
+        for (int ks = 0; ks < 16; ++ks)
+            d = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[ks], ab[ct * 16 + (lane & 15)][ks * 4 + (lane >> 4)],
+                                                     d, 0, 0, 0);
+        const int64_t cell = c0 + ct * 16 + (lane & 15);
+        if (cell < g.nc) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                bbar[(int64_t)(w * 16 + (lane >> 4) + 4 * r) * g.nc + cell] = d[r];
+        }
     }
 }
 
@@ -354,23 +376,23 @@ inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs);
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, int32_t* count, hipStream_t s) {
+                                 int32_t* key, int32_t* count, double* rec, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 grid(grid1(n, 256)), block(256);
     if (g.ndim == 3) {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<3, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<3, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
     } else {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<4, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<4, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
     }
     return hipGetLastError();
 }
@@ -388,11 +410,12 @@ hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, h
 }
 
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
-                                const double* gv, const double* gg, const int32_t* offs,
-                                const int32_t* perm, double* bbar, hipStream_t s) {
+                                const double* gv, const double* gg, const double* rec,
+                                const int32_t* offs, const int32_t* perm, double* bbar,
+                                hipStream_t s) {
     if (g.ndim == 3)
-        hipLaunchKernelGGL(k_bwd_cells3, dim3(grid1(g.nc, 16)), dim3(256), 0, s, g, A, q, ldq, gv,
-                           gg, offs, perm, bbar);
+        hipLaunchKernelGGL(k_bwd_cells3, dim3(grid1(g.nc, BWD_CELLS)), dim3(256), 0, s, g, A, rec,
+                           offs, perm, bbar);
     else
         hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, 4)), dim3(256), 0, s, g, A, q, ldq, gv,
                            gg, offs, perm, bbar);

@@ -1,12 +1,23 @@
-// Sentinel row nc of the coefficient table, the reference's alpha*[:, nc]:
-// zero for a scalar field (ARBInterp.py:31; 4D :650) and for an explicit
-// mode='vector' (whose NaN column nc+1 no query reaches, :42-45; 4D :661-664),
-// NaN in the other modes (:57, :70, :85, :99; 4D :676, :689, :704, :718).
-// Rows that reduce to the sentinel cell are evaluated from it.  The
-// coefficient builds themselves are in k_coef_mfma.hip.
+// Coefficient table: the build (calcCoefficients*, allCoeffs:
+// ARBInterp.py:523-525, 570-603; 4D :1260-1262, 1322-1355) and the sentinel
+// row.
+//
+// alpha = A . F for every cell, in FP64, stored as FP64 or rounded once to
+// FP32 (mixed precision), by separable passes:
+//   3D: A = M (x) M (x) M (arb_sep3.h), about 625 FMAs per cell;
+//   4D: A = M (x) M (x) M (x) M + a rank-16 correction from the reference's
+//       row shift (below).
+//
+// Sentinel row nc, the reference's alpha*[:, nc]: zero for a scalar field
+// (ARBInterp.py:31; 4D :650) and for an explicit mode='vector' (whose NaN
+// column nc+1 no query reaches, :42-45; 4D :661-664), NaN in the other modes
+// (:57, :70, :85, :99; 4D :676, :689, :704, :718).  Rows that reduce to the
+// sentinel cell are evaluated from it.
 #include "arb_internal.h"
+#include "arb_sep3.h"
 
 namespace {
+
 
 template <typename TO>
 __global__ void k_sentinel(TO* table, int64_t nc, int32_t width, TO v) {
@@ -16,7 +27,165 @@ __global__ void k_sentinel(TO* table, int64_t nc, int32_t width, TO v) {
 
 inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// 3D, both precisions: alpha = (M (x) M (x) M) . F by separable passes
+// (arbs3::sep3_passes) over an 8 x 4 x 1 tile of cells per workgroup, in
+// FP64, stored as TO (FP64 table, or FP32 rounded once for mixed).
+constexpr int SX = 8, SY = 4;
+
+template <typename TO>
+__global__ void __launch_bounds__(256)
+k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
+    constexpr int NX = SX + 3, NY = SY + 3, NZ = 4;
+    __shared__ double nodes[NX * NY * NZ];
+    __shared__ double t1[NY * NZ * SX * 4];
+    __shared__ double t2[NZ * SY * SX * 16];
+    const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
+    const int64_t cz = blockIdx.z;
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();                            // the previous interpolant is done
+        for (int i = threadIdx.x; i < NX * NY * NZ; i += blockDim.x) {
+            const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
+            nodes[i] = v;
+        }
+        __syncthreads();
+        arbs3::sep3_passes(nodes, t1, t2, SX, SY, 1, [&](int cx, int cy, int, int i, double a) {
+            const int64_t x = cx0 + cx, y = cy0 + cy;
+            if (x < g.npos[0] && y < g.npos[1]) {
+                const int64_t cell = x + g.npos[0] * (y + g.npos[1] * cz);
+                table[(cell * ninterp + k) * 64 + i] = (TO)a;
+            }
+        });
+    }
+}
+
+// 4D, both precisions.  A4 = M (x) M (x) M (x) M + P . dD exactly, where
+//   P = Hd (x) Hd (x) Hd (x) Hd, Hd = the derivative columns of the 1D
+//       Hermite map, [[0,0],[1,0],[-2,-1],[1,1]] (row = power, column = end);
+//   dD row k = S(k-1) - S(k), S(k) the xyzt central-difference stencil at
+//       corner k (S(-1) = 0): the reference's row shift in the d4/dxdydzdt
+//       block (ARBInterp.py:860).
+// So alpha = the separable M passes (per power of t: the t-combined 3D node
+// tile through arbs3::sep3_passes) + P . c, c_k = G(k-1) - G(k), G the xyzt
+// central difference at the cell's corner nodes (shared by neighbouring
+// cells).  P vanishes unless every power is >= 1 (81 of 256 rows), and row
+// 85 (u v w t) of A4 is identically zero: alpha_85 is set to 0 exactly.
+// Tile: 8 x 4 cells in (x, y) at one (z, t); FP64, stored as TO.
+__device__ __forceinline__ double hd(int e, int k) {
+    // Hd[e][k]: e = power 0..3, k = end 0/1
+    return e == 0 ? 0.0 : e == 1 ? (k ? 0.0 : 1.0) : e == 2 ? (k ? -1.0 : -2.0) : 1.0;
+}
+
+template <typename TO>
+__global__ void __launch_bounds__(256)
+k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
+    constexpr int NX = SX + 3, NY = SY + 3;             // node tile 11 x 7 x 4 x 4
+    constexpr int GX = SX + 1, GY = SY + 1;             // corner nodes 9 x 5 x 2 x 2
+    __shared__ double n4[4][4][NY][NX];
+    __shared__ double n3[4 * NY * NX];
+    __shared__ double t1[NY * 4 * SX * 4];
+    __shared__ double t2[4 * SY * SX * 16];
+    __shared__ double gc[2][2][GY][GX];
+    __shared__ double cc[SX * SY][16];
+    const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
+    const int64_t cz = blockIdx.z % g.npos[2], ct = blockIdx.z / g.npos[2];
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();                            // the previous interpolant is done
+        for (int i = threadIdx.x; i < 4 * 4 * NY * NX; i += blockDim.x) {
+            const int x = i % NX, y = (i / NX) % NY, z = (i / (NX * NY)) & 3, t = i / (NX * NY * 4);
+            const int64_t nx = cx0 + x, ny = cy0 + y;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1])
+                v = F[nx + g.nstride[1] * ny + g.nstride[2] * (cz + z) + g.nstride[3] * (ct + t)];
+            (&n4[0][0][0][0])[i] = v;
+        }
+        __syncthreads();
+        // xyzt central differences at the corner nodes (tile node + 1)
+        for (int i = threadIdx.x; i < 2 * 2 * GY * GX; i += blockDim.x) {
+            const int gx = i % GX, gy = (i / GX) % GY, gz = (i / (GX * GY)) & 1, gt = i / (GX * GY * 2);
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int ox = gx + 1 + ((m & 1) ? 1 : -1), oy = gy + 1 + ((m & 2) ? 1 : -1);
+                const int oz = gz + 1 + ((m & 4) ? 1 : -1), ot = gt + 1 + ((m & 8) ? 1 : -1);
+                const double v = n4[ot][oz][oy][ox];
+                acc += (__builtin_popcount(~m & 15) & 1) ? -v : v;
+            }
+            (&gc[0][0][0][0])[i] = acc * 0.0625;
+        }
+        __syncthreads();
+        // c_k = G(corner k-1) - G(corner k) per cell
+        for (int i = threadIdx.x; i < SX * SY * 16; i += blockDim.x) {
+            const int kk = i & 15, cell = i >> 4, cx = cell % SX, cy = cell / SX;
+            auto G = [&](int c) { return gc[(c >> 3) & 1][(c >> 2) & 1][cy + ((c >> 1) & 1)][cx + (c & 1)]; };
+            cc[cell][kk] = (kk ? G(kk - 1) : 0.0) - G(kk);
+        }
+        for (int et = 0; et < 4; ++et) {
+            // t-combined 3D node tile for power et
+            for (int i = threadIdx.x; i < 4 * NY * NX; i += blockDim.x) {
+                const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
+                double c[4];
+                arbs3::cub4(n4[0][z][y][x], n4[1][z][y][x], n4[2][z][y][x], n4[3][z][y][x], c);
+                n3[i] = c[et];
+            }
+            __syncthreads();
+            arbs3::sep3_passes(n3, t1, t2, SX, SY, 1, [&](int cx, int cy, int, int i3, double a) {
+                const int ex = i3 & 3, ey = (i3 >> 2) & 3, ez = i3 >> 4;
+                const int i = i3 + 64 * et;
+                if (ex && ey && ez && et) {
+                    if (i == 85) {
+                        a = 0.0;                    // the reference's row is zero
+                    } else {
+                        const double* c = cc[cx + SX * cy];
+                        double corr = 0.0;
+#pragma unroll
+                        for (int m = 0; m < 16; ++m)
+                            corr = fma(hd(ex, m & 1) * hd(ey, (m >> 1) & 1) * hd(ez, (m >> 2) & 1) *
+                                           hd(et, (m >> 3) & 1),
+                                       c[m], corr);
+                        a += corr;
+                    }
+                }
+                const int64_t x = cx0 + cx, y = cy0 + cy;
+                if (x < g.npos[0] && y < g.npos[1]) {
+                    const int64_t cell = x + g.npos[0] * (y + g.npos[1] * (cz + g.npos[2] * ct));
+                    table[(cell * ninterp + k) * 256 + i] = (TO)a;
+                }
+            });
+        }
+    }
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
 }  // namespace
+
+// alpha for every cell into table ([nc+1][ninterp][K]; FP64 for ARB_PREC_FP64,
+// FP32 for ARB_PREC_MIXED).
+hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, int32_t ninterp,
+                           void* table, hipStream_t s) {
+    if (g.nc == 0) return hipSuccess;
+    const dim3 block(256);
+    if (g.ndim == 3) {
+        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)g.npos[2]);
+        if (prec == ARB_PREC_FP64)
+            hipLaunchKernelGGL(k_coef_sep3<double>, grid, block, 0, s, g, vals, ninterp, (double*)table);
+        else
+            hipLaunchKernelGGL(k_coef_sep3<float>, grid, block, 0, s, g, vals, ninterp, (float*)table);
+    } else {
+        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)(g.npos[2] * g.npos[3]));
+        if (prec == ARB_PREC_FP64)
+            hipLaunchKernelGGL(k_coef_sep4<double>, grid, block, 0, s, g, vals, ninterp, (double*)table);
+        else
+            hipLaunchKernelGGL(k_coef_sep4<float>, grid, block, 0, s, g, vals, ninterp, (float*)table);
+    }
+    return hipGetLastError();
+}
+
 
 hipError_t arb_launch_sentinel(void* table, int32_t f32, int64_t nc, int32_t ninterp, int32_t K,
                                int32_t nan, hipStream_t s) {

@@ -43,10 +43,18 @@ __global__ void k_hash_insert(const double* __restrict__ rows, int64_t nrows, in
     unsigned long long mask = (unsigned long long)cap - 1;
     unsigned long long slot = mix64(key) & mask;
     // The table holds at most nrows distinct keys in cap >= 2*nrows slots, so
-    // the probe always reaches an empty slot or the key itself.
+    // the probe always reaches an empty slot or the key itself.  A grid has
+    // few distinct coordinates per axis and every row repeats one of them: a
+    // plain read first finds the key already present without an atomic (a
+    // filled slot never changes; a stale empty read falls through to the CAS).
     for (int64_t probe = 0; probe < cap; ++probe) {
-        unsigned long long old = atomicCAS(&tab[slot], kEmpty, key);
-        if (old == kEmpty || old == key) return;
+        const unsigned long long cur = __hip_atomic_load(&tab[slot], __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == key) return;
+        if (cur == kEmpty) {
+            const unsigned long long old = atomicCAS(&tab[slot], kEmpty, key);
+            if (old == kEmpty || old == key) return;
+        }
         slot = (slot + 1) & mask;
     }
 }

@@ -235,6 +235,16 @@ int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_
  * 3 = A = inv(B).D.  Needs no GPU. */
 int arb_operator_matrix(int32_t ndim, int32_t which, double* out);
 
+/* Field ingestion (host only, no GPU): a numeric CSV file as
+ * np.genfromtxt(path, delimiter=',') reads it, the way the reference's
+ * examples load fields (examples/3D_ARBInterpExample.py:14).  '#' starts a
+ * comment, blank lines are skipped, empty or non-numeric fields are NaN, and
+ * every line must have the first line's column count (else ARB_ESHAPE).
+ * arb_csv_shape counts rows and columns; arb_csv_read fills out
+ * (nrows x ncols, row-major) on host threads. */
+int arb_csv_shape(const char* path, int64_t* nrows, int32_t* ncols);
+int arb_csv_read(const char* path, double* out, int64_t nrows, int32_t ncols);
+
 /* Release the handle and its device memory. */
 void arb_field_destroy(arb_field* f);
 
