@@ -250,7 +250,8 @@ class _Interp:
         """Evaluate positions held in GPU memory (torch tensor n x >= d, float64,
         row-contiguous).  Returns (comps, norms, grads[, cells]) as torch tensors
         (None where the mode has no such output), enqueued on the current
-        stream.  nan_rows=True writes NaN over out-of-range rows of `q` (F4).
+        stream.  nan_rows=True writes NaN over out-of-range rows of `q`
+        (ARBInterp.py:350-355).
         `out` = a tuple returned by an earlier call of the same size reuses
         those buffers (no allocation)."""
         torch = _torch()
@@ -311,7 +312,7 @@ class _Interp:
         cell (the reference's alphamask lookup does)."""
         d = self.NDIM
         torch = _torch()
-        # F4: whole caller rows with a coordinate strictly outside the box
+        # whole caller rows with a coordinate strictly outside the box
         oob = np.zeros(len(query), dtype=bool)
         for a in range(d):
             col = query[:, a]
@@ -350,28 +351,36 @@ class _Interp:
         for a in range(d):
             if query[a] < self._lo[a] or query[a] > self._hi[a]:
                 return np.nan
-        pt = np.array([[float(query[a]) for a in range(d)]], dtype=np.float64)
+        pt = np.array([float(query[a]) for a in range(d)], dtype=np.float64)
         if np.isnan(pt).any():
             raise IndexError("index -9223372036854775808 is out of bounds for axis 0 with size %d"
                              % (self.nc + 1))
         torch = _torch()
-        q_dev = torch.from_numpy(pt).to("cuda")
-        comps, norms, grads = self._alloc_out(torch, 1, torch.float64)
-        cells = torch.empty((1,), dtype=torch.int64, device="cuda")
-        flags = N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
-        rc = self._run(torch, q_dev, 1, d, comps, norms, grads, cells, flags)
-        self.queryInd = int(cells.cpu().numpy()[0])
+        if getattr(self, "_pt_out", None) is None:
+            # comps(3), norm, grad(4) FP64 and the cell, reused by every call
+            self._pt_out = np.zeros(8, dtype=np.float64)
+            self._pt_cell = ctypes.c_int64(0)
+            base = self._pt_out.ctypes.data
+            self._pt_ptrs = (ctypes.c_void_p(base), ctypes.c_void_p(base + 24),
+                             ctypes.c_void_p(base + 32),
+                             ctypes.c_void_p(ctypes.addressof(self._pt_cell)))
+        cp, npp, gp, cellp = self._pt_ptrs
+        rc = N.lib().arb_query_point(self._h, ctypes.c_void_p(pt.ctypes.data), cp, npp, gp,
+                                     cellp, _stream_handle(torch))
+        self._info.table_built = 1
+        self.queryInd = int(self._pt_cell.value)
         if rc == N.ARB_EINDEX:
             raise IndexError("index %d is out of bounds for axis 0 with size %d"
                              % (self.queryInd, self.nc + 1))
         N.check(rc, "query")
+        out = self._pt_out
         if self._variant == 1:
-            return comps.cpu().numpy()[0]
-        nv = np.float64(norms.cpu().numpy()[0])
-        gv = grads.cpu().numpy()[0]
+            return out[0:3].copy()
+        nv = np.float64(out[3])
+        gv = out[4:4 + d].copy()
         if self._variant == 2:
             return nv, gv
-        return comps.cpu().numpy()[0], nv, gv
+        return out[0:3].copy(), nv, gv
 
     def Query(self, query):
         """Query1/2/3 (ARBInterp.py:177-211 / :880-914): 2-D input with more

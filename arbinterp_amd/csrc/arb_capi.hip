@@ -213,6 +213,10 @@ struct arb_field {
     hipEvent_t ev[5] = {};
     int32_t ev_ready = 0;
     float last_ms[4] = {0, 0, 0, 0};
+    // single-point mailbox: pinned host record the point kernel reads and
+    // writes in place (host address, and the same memory as the GPU sees it)
+    double* pin = nullptr;
+    double* pin_dev = nullptr;
 };
 
 // Choose the cell block of one bucket: the largest of a fixed ladder whose
@@ -487,8 +491,9 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     return ARB_OK;
 }
 
-// The sentinel row nc of the table (zeros, or NaN, per mode: F5), which rows
-// outside the box are evaluated from.
+// The sentinel row nc of the table, which rows outside the box are evaluated
+// from: zeros in scalar and vector mode, NaN otherwise
+// (ARBInterp.py:31,42-45,57,70,85,99).
 static int ensure_sentinel(arb_field* f, hipStream_t s) {
     if (f->sentinel_set) return ARB_OK;
     const int K = 1 << (2 * f->g.ndim);
@@ -626,6 +631,55 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         HIPCHK(hipStreamSynchronize(s));
         if (h) return fail(ARB_EINDEX, "a query reduced to a cell index past the sentinel");
     }
+    return ARB_OK;
+}
+
+// Layout of the single-point record, in doubles: point, components, norm,
+// gradient, cell (int64) and status word (int32).
+enum { PIN_Q = 0, PIN_COMPS = 4, PIN_NORM = 7, PIN_GRAD = 8, PIN_CELL = 12, PIN_FLAG = 13,
+       PIN_DOUBLES = 16 };
+
+int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm, double* grad,
+                    int64_t* cell, void* stream) {
+    if (!f || !pt) return fail(ARB_EINVAL, "arb_query_point: NULL argument");
+    if (f->has_comps && !comps) return fail(ARB_EINVAL, "arb_query_point: comps is NULL");
+    if (f->has_norm && (!norm || !grad))
+        return fail(ARB_EINVAL, "arb_query_point: norm/grad is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    int rc = arb_build_coeffs(f, stream);
+    if (rc) return rc;
+    if (!f->pin) {
+        void* h = nullptr;
+        HIPCHK(hipHostMalloc(&h, sizeof(double) * PIN_DOUBLES, hipHostMallocDefault));
+        f->pin = static_cast<double*>(h);
+        void* dptr = nullptr;
+        HIPCHK(hipHostGetDevicePointer(&dptr, h, 0));
+        f->pin_dev = static_cast<double*>(dptr);
+    }
+    const int d = f->g.ndim;
+    volatile double* h = f->pin;
+    for (int a = 0; a < d; ++a) h[PIN_Q + a] = pt[a];
+    volatile int32_t* hflag = reinterpret_cast<volatile int32_t*>(f->pin + PIN_FLAG);
+    *hflag = 0;
+    ArbOut o{};
+    o.comps = f->has_comps ? f->pin_dev + PIN_COMPS : nullptr;
+    o.norms = f->has_norm ? f->pin_dev + PIN_NORM : nullptr;
+    o.grads = f->has_norm ? f->pin_dev + PIN_GRAD : nullptr;
+    o.cells = reinterpret_cast<int64_t*>(f->pin_dev + PIN_CELL);
+    o.ldq = d;
+    o.flag = reinterpret_cast<int32_t*>(f->pin_dev + PIN_FLAG);
+    HIPCHK(arb_launch_query_direct(f->g, f->precision, 1, f->table, f->ninterp, f->has_comps,
+                                   f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (f->has_comps)
+        for (int c = 0; c < 3; ++c) comps[c] = h[PIN_COMPS + c];
+    if (f->has_norm) {
+        *norm = h[PIN_NORM];
+        for (int a = 0; a < d; ++a) grad[a] = h[PIN_GRAD + a];
+    }
+    const int64_t flat = *reinterpret_cast<volatile int64_t*>(f->pin + PIN_CELL);
+    if (cell) *cell = flat;
+    if (*hflag) return fail(ARB_EINDEX, "the point reduced to a cell index past the sentinel");
     return ARB_OK;
 }
 
@@ -978,6 +1032,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->ev_ready)
         for (auto& e : f->ev) (void)hipEventDestroy(e);
     if (f->push_exec) (void)hipGraphExecDestroy(f->push_exec);
+    if (f->pin) (void)hipHostFree(f->pin);
     if (f->cap_stream) (void)hipStreamDestroy(f->cap_stream);
     delete f;
 }

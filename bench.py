@@ -50,7 +50,7 @@ def parse():
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run), then exit")
     ap.add_argument("--workload", default="dipole3d",
-                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield"],
+                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield", "point"],
                     help="dipole3d: the headline JSON line; the others print their own line")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the mixed-precision default path: build, warm up, --steps queries "
@@ -411,8 +411,46 @@ def bench_multifield(args, torch):
                       "results": res, "push_mixed_vs_fp64_max_abs_dx": float(np.abs(d[m]).max())}))
 
 
+def bench_point(args, torch):
+    """Single-point Query on the 128^3 dipole field, mode 'both' (the per-point
+    loop of examples/3D_ARBInterpExample.py:23): microseconds per call through
+    the drop-in, mixed and FP64, beside the oracle's sQuery on the same points
+    (which forms that cell's coefficients per call, as the reference does)."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import tricubic
+    from oracle import arbinterp_oracle as O
+    n = (args.grid,) * 3
+    rows, q = W.dipole3d(seed=2, n=n, nq=4000)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        t = tricubic(rows, "quiet", mode="both", precision=prec)
+        t.allCoeffs()
+        for i in range(200):
+            t.Query(q[i])
+        t0 = time.perf_counter()
+        for i in range(len(q)):
+            t.Query(q[i])
+        us = (time.perf_counter() - t0) * 1e6 / len(q)
+        # the same points as one-row batches (device buffers and copies per call)
+        t0 = time.perf_counter()
+        for i in range(500):
+            t.Query(q[i:i + 1].copy())
+        res[prec] = {"us_per_call": us,
+                     "one_row_batch_us_per_call": (time.perf_counter() - t0) * 1e6 / 500}
+        t.close()
+    o = O.tricubic(rows, "quiet", mode="both")
+    m = 200
+    t0 = time.perf_counter()
+    for i in range(m):
+        o.Query(q[i])
+    cpu_us = (time.perf_counter() - t0) * 1e6 / m
+    print(json.dumps({"workload": "single-point Query, 128^3 dipole field, mode both",
+                      "calls": len(q), "results": res,
+                      "cpu_oracle_us_per_call": cpu_us, "cpu_sample_calls": m}))
+
+
 OTHER = {"scalar3d": bench_scalar3d, "moving4d": bench_moving4d, "smooth3d": bench_smooth3d,
-         "multifield": bench_multifield}
+         "multifield": bench_multifield, "point": bench_point}
 
 
 if __name__ == "__main__":

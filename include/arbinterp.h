@@ -64,8 +64,8 @@ extern "C" {
 
 /* ---- query flags ------------------------------------------------------- */
 #define ARB_Q_NAN_ROWS  1   /* write NaN over every column of rows with a
-                               coordinate outside [IntMin, IntMax] (F4 of the
-                               reference: ARBInterp.py:350-355,1069-1076)      */
+                               coordinate outside [IntMin, IntMax] (as the
+                               reference does: ARBInterp.py:350-355,1069-1076)      */
 #define ARB_Q_CHECK     2   /* synchronise the stream and return ARB_EINDEX if any
                                row reduced past the sentinel                     */
 #define ARB_Q_OUT_F64   4   /* mixed precision: write FP64 outputs (default FP32) */
@@ -139,6 +139,18 @@ int arb_build_coeffs(arb_field* f, void* stream);
 int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
               void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
               int32_t flags, void* stream);
+
+/* One point, host memory in and out: pt (d FP64); comps (3), norm (1), grad
+ * (d), all FP64, for the outputs the mode has (else nullable); cell (int64,
+ * nullable) = the flat cell index.  The point and the results pass through a
+ * pinned host record owned by the field that the kernel reads and writes in
+ * place: one launch and one synchronisation of `stream` per call, no device
+ * allocation after the first call.  Returns ARB_EINDEX if the point reduces
+ * to a cell past the sentinel (cell is written first).  Calls on one field
+ * from several threads at once race on that record.  Replaces the per-point
+ * chain of the reference's sQuery1/2/3 (ARBInterp.py:213-342, :916-1062). */
+int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm, double* grad,
+                    int64_t* cell, void* stream);
 
 /* Node values of the interpolated columns as stored, [ninterp][nnodes] FP64
  * in canonical node order: scalar f; vector Bx,By,Bz; norm |B|; both

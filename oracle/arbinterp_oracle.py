@@ -253,7 +253,7 @@ class _Interp:
         self.kind, self.variant = kind, variant
         f = p.field
         self.interps = {}
-        # sentinel column nc (F5): NaN except scalar mode (zero) and explicit
+        # sentinel column nc: NaN except scalar mode (zero) and explicit
         # 'vector' mode (zero at nc, NaN at nc+1) (ARBInterp.py:31,42-45,57,70,85,99)
         ncol_tab = self.nc + 2 if kind == "vector" else self.nc + 1
         sentinel = 0.0 if kind in ("scalar", "vector") else np.nan
@@ -418,8 +418,9 @@ class _Interp:
 
     # -- cell reduction (ARBInterp.py:350-373 / :1069-1092) ------------------
     def cell_reduce(self, query):
-        """Bounds mask (written into the caller's rows, F4), cuboid coordinates,
-        float flat index with NaN -> nc, local coordinates."""
+        """Bounds mask (written into the caller's rows, ARBInterp.py:350-355),
+        cuboid coordinates, float flat index with NaN -> nc, local
+        coordinates."""
         p = self.p
         d = self.NDIM
         for a in range(d):

@@ -318,3 +318,45 @@ def test_packed_records_match_arrays(dev, case):
                 assert b is None
                 continue
             assert _nan_equal(a, b.contiguous()), (case, fl)
+
+
+def _point_cases(golden):
+    g = golden("vector4d_modes")
+    rows3, _ = W.dipole3d(seed=2, n=(20, 18, 16), nq=10)
+    srows, _ = W.scalar3d_gaussians(seed=1)
+    cases = [("tricubic", srows, None)]
+    cases += [("tricubic", rows3, m) for m in ("both", "vector", "norm")]
+    cases += [("quadcubic", g["field"], m) for m in ("both", "vector", "norm")]
+    return cases
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_single_point_matches_batch(dev, golden, precision):
+    """Query(point) runs the point mailbox entry; each call must give the batch
+    path's row bit for bit (both evaluate the same cell with FP64 outputs),
+    including points on the box faces, and the cell index."""
+    for cls, fld, mode in _point_cases(golden):
+        kw = {"precision": precision} if mode is None else {"mode": mode, "precision": precision}
+        t = getattr(dev, cls)(fld.copy(), "quiet", **kw)
+        d = t.NDIM
+        lo, hi = np.array(t._lo[:d]), np.array(t._hi[:d])
+        rng = np.random.default_rng(17)
+        pts = lo + (hi - lo) * rng.uniform(size=(64, d))
+        pts[0] = lo
+        pts[1, 0] = hi[0]
+        # the far corner reduces past the sentinel: IndexError, as the oracle
+        with pytest.raises(IndexError):
+            t.Query(hi.copy())
+        batch = t.Query(pts.copy())
+        batch = batch if isinstance(batch, tuple) else (batch,)
+        inds = t.queryInds.copy()
+        for i, p in enumerate(pts):
+            one = t.Query(p.copy())
+            one = one if isinstance(one, tuple) else (one,)
+            assert len(one) == len(batch)
+            for a, b in zip(one, batch):
+                assert np.array_equal(np.asarray(a, dtype=np.float64).ravel(),
+                                      np.asarray(b[i], dtype=np.float64).ravel()), \
+                    (cls, mode, precision, i)
+            assert t.queryInd == inds[i], (cls, mode, precision, i)
+        t.close()
