@@ -536,7 +536,8 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     // node values, no table (ignored once a table has been loaded, or when
     // the block does not fit the kernel's LDS)
     const bool fused = (flags & ARB_Q_FUSED) && sorted && f->g.ndim == 3 && !f->table_loaded &&
-                       !(flags & ARB_Q_SORTED_GLOBAL) && arb_fused_lds(f->g, f->B) > 0;
+                       !(flags & ARB_Q_SORTED_GLOBAL) &&
+                       arb_fused_lds(f->g, f->B, f->ninterp, f->eval_threads) > 0;
     int rc = fused ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (n == 0) return ARB_OK;

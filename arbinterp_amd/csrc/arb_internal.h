@@ -255,7 +255,7 @@ hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int
                              int32_t shift, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
                               int32_t* perm, int64_t n, hipStream_t s);
-size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B);
+size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads);
 hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                  int32_t out_f64, const double* vals, int32_t ninterp,
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,

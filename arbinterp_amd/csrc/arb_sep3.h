@@ -34,58 +34,91 @@ __host__ __device__ inline int sep3_t1(int bx, int by, int bz) { return (by + 3)
 __host__ __device__ inline int sep3_t2(int bx, int by, int bz) { return (bz + 3) * by * bx * 16; }
 
 // Block-wide (every thread must call it; it synchronises between passes):
-// nodes[(z * NY + y) * NX + x] (NX = bx+3, NY = by+3, NZ = bz+3) holds the
+// nodes[(z * NY + y) * NX + x] (NX = BX+3, NY = BY+3, NZ = BZ+3) holds the
 // node tile whose (0,0,0) is the neighbourhood origin of cell (0,0,0).  For
 // every cell (cx, cy, cz) of the tile and coefficient i = ex + 4 ey + 16 ez,
-// out(cx, cy, cz, i, alpha_i) is called once; sixteen consecutive threads
-// handle the sixteen (ex, ey) of one cell and ez, so a row-major store of
-// alpha is coalesced.  t1, t2: scratch of sep3_t1 / sep3_t2 doubles.
+// out(cx, cy, cz, i, alpha_i) is called once.
+//   CELL_FAST = false: sixteen consecutive threads of the z pass take the
+//     sixteen (ex, ey) of one cell, so a row-major store of alpha is
+//     coalesced; the pass buffers keep the coefficient index innermost,
+//     t1[(yz BX + cx) 4 + ex], t2[((z BY + cy) BX + cx) 16 + 4 ex + ey], and
+//     each thread writes its four results as two 16-byte stores;
+//   CELL_FAST = true: consecutive threads take consecutive cells
+//     (cx + BX (cy + BY cz)); the pass buffers keep the cell x index
+//     innermost, t1[((ex NZ + z) NY + y) BX + cx],
+//     t2[(((4 ex + ey) NZ + z) BY + cy) BX + cx], so the lanes of a wave
+//     touch consecutive doubles.
+// Either way each alpha_i comes from the same three cub4 chains on the same
+// values, so it does not depend on the tile shape, the layout or the thread
+// order.  t1, t2: scratch of sep3_t1 / sep3_t2 doubles (16-byte aligned).
 // The caller synchronises before it reuses nodes, t1 or t2.
-template <typename OUT>
-__device__ __forceinline__ void sep3_passes(const double* nodes, double* t1, double* t2, int bx,
-                                            int by, int bz, const OUT& out) {
-    const int NX = bx + 3, NY = by + 3, NZ = bz + 3;
-    // x: t1[((z NY + y) bx + cx) 4 + ex]
-    for (int t = threadIdx.x; t < bx * NY * NZ; t += blockDim.x) {
-        const int cx = t % bx, yz = t / bx;
+template <int BX, int BY, int BZ, bool CELL_FAST, typename OUT>
+__device__ __forceinline__ void sep3_passes(const double* nodes, double* t1, double* t2,
+                                            const OUT& out) {
+    constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
+    // x
+    for (int t = threadIdx.x; t < BX * NY * NZ; t += blockDim.x) {
+        const int cx = t % BX, yz = t / BX;
         const double* n = nodes + yz * NX + cx;
         double c[4];
         cub4(n[0], n[1], n[2], n[3], c);
-        double* o = t1 + (yz * bx + cx) * 4;
+        if constexpr (CELL_FAST) {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) o[e] = c[e];
+            for (int e = 0; e < 4; ++e) t1[(e * NY * NZ + yz) * BX + cx] = c[e];
+        } else {
+            double2* o = reinterpret_cast<double2*>(t1 + (yz * BX + cx) * 4);
+            o[0] = make_double2(c[0], c[1]);
+            o[1] = make_double2(c[2], c[3]);
+        }
     }
     __syncthreads();
-    // y: t2[(((z by + cy) bx + cx) 4 + ex) 4 + ey]
-    for (int t = threadIdx.x; t < bx * by * NZ * 4; t += blockDim.x) {
-        const int ex = t & 3;
-        int r = t >> 2;
-        const int cx = r % bx;
-        r /= bx;
-        const int cy = r % by;
-        const int z = r / by;
-        const double* n = t1 + ((z * NY + cy) * bx + cx) * 4 + ex;
-        const int st = bx * 4;                      // next y
+    // y
+    for (int t = threadIdx.x; t < BX * BY * NZ * 4; t += blockDim.x) {
         double c[4];
-        cub4(n[0], n[st], n[2 * st], n[3 * st], c);
-        double* o = t2 + (((z * by + cy) * bx + cx) * 4 + ex) * 4;
+        if constexpr (CELL_FAST) {
+            const int cx = t % BX;
+            int r = t / BX;
+            const int cy = r % BY;
+            r /= BY;
+            const int z = r % NZ;
+            const int ex = r / NZ;
+            const double* n = t1 + ((ex * NZ + z) * NY + cy) * BX + cx;
+            cub4(n[0], n[BX], n[2 * BX], n[3 * BX], c);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) o[e] = c[e];
+            for (int e = 0; e < 4; ++e) t2[(((ex * 4 + e) * NZ + z) * BY + cy) * BX + cx] = c[e];
+        } else {
+            const int ex = t & 3;
+            int r = t >> 2;
+            const int cx = r % BX;
+            r /= BX;
+            const int cy = r % BY;
+            const int z = r / BY;
+            const double* n = t1 + ((z * NY + cy) * BX + cx) * 4 + ex;
+            constexpr int st = BX * 4;                  // next y
+            cub4(n[0], n[st], n[2 * st], n[3 * st], c);
+            double2* o = reinterpret_cast<double2*>(t2 + (((z * BY + cy) * BX + cx) * 4 + ex) * 4);
+            o[0] = make_double2(c[0], c[1]);
+            o[1] = make_double2(c[2], c[3]);
+        }
     }
     __syncthreads();
     // z: alpha_i, i = ex + 4 ey + 16 ez
-    for (int t = threadIdx.x; t < bx * by * bz * 16; t += blockDim.x) {
-        const int exy = t & 15;
-        int r = t >> 4;
-        const int cx = r % bx;
-        r /= bx;
-        const int cy = r % by;
-        const int cz = r / by;
+    constexpr int NC = BX * BY * BZ;
+    for (int t = threadIdx.x; t < NC * 16; t += blockDim.x) {
+        const int exy = CELL_FAST ? t / NC : (t & 15);
+        const int cell = CELL_FAST ? t % NC : (t >> 4);
+        const int cx = cell % BX, cy = (cell / BX) % BY, cz = cell / (BX * BY);
         const int ex = exy & 3, ey = exy >> 2;
-        const double* n = t2 + ((cz * by + cy) * bx + cx) * 16 + ex * 4 + ey;
-        const int st = by * bx * 16;                // next z
         double c[4];
-        cub4(n[0], n[st], n[2 * st], n[3 * st], c);
+        if constexpr (CELL_FAST) {
+            const double* n = t2 + (((ex * 4 + ey) * NZ + cz) * BY + cy) * BX + cx;
+            constexpr int st = BY * BX;                 // next z
+            cub4(n[0], n[st], n[2 * st], n[3 * st], c);
+        } else {
+            const double* n = t2 + ((cz * BY + cy) * BX + cx) * 16 + ex * 4 + ey;
+            constexpr int st = BY * BX * 16;            // next z
+            cub4(n[0], n[st], n[2 * st], n[3 * st], c);
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) out(cx, cy, cz, exy + 16 * e, c[e]);
     }

@@ -37,8 +37,8 @@ __global__ void __launch_bounds__(256)
 k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
     constexpr int NX = SX + 3, NY = SY + 3, NZ = 4;
     __shared__ double nodes[NX * NY * NZ];
-    __shared__ double t1[NY * NZ * SX * 4];
-    __shared__ double t2[NZ * SY * SX * 16];
+    __shared__ __align__(16) double t1[NY * NZ * SX * 4];
+    __shared__ __align__(16) double t2[NZ * SY * SX * 16];
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
     const int64_t cz = blockIdx.z;
     for (int k = 0; k < ninterp; ++k) {
@@ -52,7 +52,7 @@ k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
             nodes[i] = v;
         }
         __syncthreads();
-        arbs3::sep3_passes(nodes, t1, t2, SX, SY, 1, [&](int cx, int cy, int, int i, double a) {
+        arbs3::sep3_passes<SX, SY, 1, false>(nodes, t1, t2, [&](int cx, int cy, int, int i, double a) {
             const int64_t x = cx0 + cx, y = cy0 + cy;
             if (x < g.npos[0] && y < g.npos[1]) {
                 const int64_t cell = x + g.npos[0] * (y + g.npos[1] * cz);
@@ -86,8 +86,8 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
     constexpr int GX = SX + 1, GY = SY + 1;             // corner nodes 9 x 5 x 2 x 2
     __shared__ double n4[4][4][NY][NX];
     __shared__ double n3[4 * NY * NX];
-    __shared__ double t1[NY * 4 * SX * 4];
-    __shared__ double t2[4 * SY * SX * 16];
+    __shared__ __align__(16) double t1[NY * 4 * SX * 4];
+    __shared__ __align__(16) double t2[4 * SY * SX * 16];
     __shared__ double gc[2][2][GY][GX];
     __shared__ double cc[SX * SY][16];
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
@@ -133,7 +133,7 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
                 n3[i] = c[et];
             }
             __syncthreads();
-            arbs3::sep3_passes(n3, t1, t2, SX, SY, 1, [&](int cx, int cy, int, int i3, double a) {
+            arbs3::sep3_passes<SX, SY, 1, false>(n3, t1, t2, [&](int cx, int cy, int, int i3, double a) {
                 const int ex = i3 & 3, ey = (i3 >> 2) & 3, ez = i3 >> 4;
                 const int i = i3 + 64 * et;
                 if (ex && ey && ez && et) {

@@ -422,9 +422,14 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
 // per axis, into LDS, then the separable passes of arb_sep3.h, the routine
 // the table build uses, so the results are bitwise those of the table),
 // writes them into the same LDS block layout the staged kernel uses, and
-// evaluates the bucket's queries.  Dynamic LDS: ncell * cellstride * 16
-// bytes of coefficients, then the node tile and pass buffers (doubles).
-template <typename T, typename TO>
+// evaluates the bucket's queries.  The node tiles of all interpolants are
+// loaded into registers first (FUSED_REGS per thread, all loads in flight at
+// once), so the passes never wait on global memory.  Dynamic LDS: ncell *
+// cellstride * 16 bytes of coefficients, then one node tile and the pass
+// buffers (doubles).
+constexpr int FUSED_REGS = 8;
+
+template <typename T, typename TO, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
               int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
@@ -443,11 +448,10 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
             rem /= B.nb[a];
         }
     }
-    const int bx = B.b[0], by = B.b[1], bz = B.b[2];
-    const int NX = bx + 3, NY = by + 3, NZ = bz + 3;
+    constexpr int NX = BX + 3, NY = BY + 3, NT = NX * NY * (BZ + 3);
     double* nodes = reinterpret_cast<double*>(smem + B.ncell * B.cellstride);
-    double* t1 = nodes + arbs3::sep3_nodes(bx, by, bz);
-    double* t2 = t1 + arbs3::sep3_t1(bx, by, bz);
+    double* t1 = nodes + NT;
+    double* t2 = t1 + arbs3::sep3_t1(BX, BY, BZ);
     T* cf = reinterpret_cast<T*>(smem);
     const int cstr = B.cellstride * (16 / (int)sizeof(T));   // elements per LDS cell
     // this thread's first query, loaded before the coefficient work
@@ -461,22 +465,38 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 #pragma unroll
         for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
     }
+    // node tiles of every interpolant, x fastest: element e = k * NT + r of
+    // the flattened [interp][tile] space sits in register j of thread
+    // e - j * blockDim.x (the launcher checks ninterp * NT <= FUSED_REGS *
+    // blockDim.x).  Nodes past the grid edge read as 0 (only cells outside
+    // the grid use them); the loads are unconditional (index 0 when unused).
+    double nv[FUSED_REGS];
+    const int total = ninterp * NT;
+#pragma unroll
+    for (int j = 0; j < FUSED_REGS; ++j) {
+        const int e = threadIdx.x + j * blockDim.x;
+        const int k = e / NT, r = e - k * NT;
+        const int x = r % NX, y = (r / NX) % NY, z = r / (NX * NY);
+        const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
+        const bool in = e < total && nx < g.n[0] && ny < g.n[1] && nz < g.n[2];
+        const int64_t src = in ? (int64_t)k * g.nnodes + nx + g.nstride[1] * ny + g.nstride[2] * nz
+                               : 0;
+        const double v = vals[src];
+        nv[j] = in ? v : 0.0;
+    }
     for (int k = 0; k < ninterp; ++k) {
-        __syncthreads();                            // the previous passes are done
-        // node tile, x fastest; nodes past the grid edge read as 0 (only
-        // cells outside the grid use them)
-        for (int i = threadIdx.x; i < NX * NY * NZ; i += blockDim.x) {
-            const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
-            const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
-            double v = 0.0;
-            if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
-                v = vals[(int64_t)k * g.nnodes + nx + g.nstride[1] * ny + g.nstride[2] * nz];
-            nodes[i] = v;
+        // tile k into LDS: the x pass of interpolant k - 1, the last reader
+        // of `nodes`, ended at that interpolant's first pass barrier
+#pragma unroll
+        for (int j = 0; j < FUSED_REGS; ++j) {
+            const int e = threadIdx.x + j * blockDim.x - k * NT;
+            if (e >= 0 && e < NT) nodes[e] = nv[j];
         }
         __syncthreads();
-        arbs3::sep3_passes(nodes, t1, t2, bx, by, bz, [&](int cx, int cy, int cz, int i, double a) {
-            const int lc = cx | (cy << B.lsh[1]) | (cz << B.lsh[2]);
-            cf[lc * cstr + k * 64 + i] = (T)a;
+        // consecutive threads, consecutive cells: local cell cx + BX (cy + BY cz)
+        arbs3::sep3_passes<BX, BY, BZ, true>(nodes, t1, t2,
+                                             [&](int cx, int cy, int cz, int i, double a) {
+            cf[(cx + BX * (cy + BY * cz)) * cstr + k * 64 + i] = (T)a;
         });
     }
     __syncthreads();
@@ -601,11 +621,24 @@ hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t
     return hipGetLastError();
 }
 
+// Cell blocks k_eval_fused3 is instantiated for (the ladder the bucket
+// geometry is chosen from).
+#define ARB_FUSED_SHAPES(X) X(4, 4, 4) X(4, 4, 2) X(4, 2, 2) X(2, 2, 2) X(2, 2, 1) X(2, 1, 1) X(1, 1, 1)
+
 // LDS bytes of k_eval_fused3 for the bucket geometry B (0 when the path does
-// not apply: 3D only, within 64 KiB).
-size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B) {
+// not apply: 3D only, an instantiated block shape, within 64 KiB, node tiles
+// of all interpolants within the registers of `threads` threads).
+size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads) {
     if (g.ndim != 3) return 0;
     const int bx = B.b[0], by = B.b[1], bz = B.b[2];
+    bool shape = false;
+#define ARB_FUSED_IS(X, Y, Z) shape |= (bx == X && by == Y && bz == Z);
+    ARB_FUSED_SHAPES(ARB_FUSED_IS)
+#undef ARB_FUSED_IS
+    if (!shape) return 0;
+    // the local cell index must be cx + bx (cy + by cz), as the kernel writes it
+    if (B.lsh[0] != 0 || (1 << B.lsh[1]) != bx || (1 << B.lsh[2]) != bx * by) return 0;
+    if ((int64_t)ninterp * arbs3::sep3_nodes(bx, by, bz) > (int64_t)FUSED_REGS * threads) return 0;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16 +
                        sizeof(double) * (size_t)(arbs3::sep3_nodes(bx, by, bz) +
                                                  arbs3::sep3_t1(bx, by, bz) +
@@ -618,17 +651,23 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
-    const size_t lds = arb_fused_lds(g, B);
+    const size_t lds = arb_fused_lds(g, B, ninterp, block_threads);
     if (lds == 0) return hipErrorInvalidValue;
     dim3 grid((unsigned)B.nbt), block(block_threads);
-#define L_FUSED(T, TO)                                                                        \
-    hipLaunchKernelGGL((k_eval_fused3<T, TO>), grid, block, lds, s, g, B, vals, ninterp,      \
-                       has_comps, has_norm, q, n, ldq, o, offs, perm)
-    if (prec == ARB_PREC_FP64) L_FUSED(double, double);
-    else if (out_f64) L_FUSED(float, double);
-    else L_FUSED(float, float);
+#define L_FUSED(T, TO, X, Y, Z)                                                               \
+    hipLaunchKernelGGL((k_eval_fused3<T, TO, X, Y, Z>), grid, block, lds, s, g, B, vals,      \
+                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
+#define ARB_FUSED_CASE(X, Y, Z)                                                               \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
+        if (prec == ARB_PREC_FP64) L_FUSED(double, double, X, Y, Z);                          \
+        else if (out_f64) L_FUSED(float, double, X, Y, Z);                                    \
+        else L_FUSED(float, float, X, Y, Z);                                                  \
+        return hipGetLastError();                                                             \
+    }
+    ARB_FUSED_SHAPES(ARB_FUSED_CASE)
+#undef ARB_FUSED_CASE
 #undef L_FUSED
-    return hipGetLastError();
+    return hipErrorInvalidValue;
 }
 
 hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
