@@ -11,8 +11,11 @@ fallback follow the reference.
 
 Extensions (keyword arguments the reference does not have; its defaults are
 kept): `precision='fp64'` (default, plain FP64) or `'mixed'` (FP32 table and
-evaluation), `quiet=True` as a keyword, and `query_device(q)` for positions
-already in GPU memory (a torch tensor), which returns torch tensors.
+evaluation), `quiet=True` as a keyword, `boundary=None` (default, the
+reference's box) or `'neumann'` / `'dirichlet'` (one ghost node layer per
+face, so the whole sampled box is interpolated; the attributes then describe
+the extended grid), and `query_device(q)` for positions already in GPU memory
+(a torch tensor), which returns torch tensors.
 
 Every numeric result comes from libarbinterp.so; there is no CPU path.
 """
@@ -39,6 +42,9 @@ _MODE_CODE = {"scalar": N.ARB_MODE_SCALAR, "vector": N.ARB_MODE_VECTOR, "norm": 
               "both": N.ARB_MODE_BOTH, "invalid": N.ARB_MODE_VECTOR_NAN,
               "default": N.ARB_MODE_VECTOR_NAN}
 
+_BC_CODE = {None: N.ARB_BC_NONE, "none": N.ARB_BC_NONE, "neumann": N.ARB_BC_NEUMANN,
+            "dirichlet": N.ARB_BC_DIRICHLET}
+
 _AX = "xyzt"
 
 
@@ -63,9 +69,12 @@ class _Interp:
         d = self.NDIM
         self.eps = 10 * np.finfo(float).eps            # ARBInterp.py:11
         precision = kwargs.pop("precision", "fp64")
+        boundary = kwargs.pop("boundary", None)
         quiet = ("quiet" in args) or bool(kwargs.pop("quiet", False))
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if boundary not in _BC_CODE:
+            raise ValueError("boundary must be None, 'neumann' or 'dirichlet'")
         self.precision = precision
         torch = _torch()
         is_tensor = isinstance(field, torch.Tensor)
@@ -102,6 +111,10 @@ class _Interp:
                                 _stream_handle(torch), ctypes.byref(h))
         N.check(rc, "field")
         self._h = h
+        if _BC_CODE[boundary] != N.ARB_BC_NONE:
+            N.check(L.arb_field_set_boundary(h, _BC_CODE[boundary], _stream_handle(torch)),
+                    "boundary")
+        self.boundary = boundary
         info = N.FieldInfo()
         N.check(L.arb_field_info_get(h, ctypes.byref(info)))
         self._info = info

@@ -25,6 +25,10 @@ ARB_MODE_NORM = 2
 ARB_MODE_BOTH = 3
 ARB_MODE_VECTOR_NAN = 4
 
+ARB_BC_NONE = 0
+ARB_BC_NEUMANN = 1
+ARB_BC_DIRICHLET = 2
+
 ARB_PREC_FP64 = 0
 ARB_PREC_MIXED = 1
 
@@ -53,7 +57,7 @@ class FieldInfo(ctypes.Structure):
         ("n", c_i64 * 4), ("npos", c_i64 * 4), ("h", ctypes.c_double * 4),
         ("int_min", ctypes.c_double * 4), ("int_max", ctypes.c_double * 4),
         ("nc", c_i64), ("nnodes", c_i64), ("table_bytes", c_i64),
-        ("packed_width", c_i32), ("reserved", c_i32),
+        ("packed_width", c_i32), ("boundary", c_i32),
     ]
 
 
@@ -62,6 +66,7 @@ SIGNATURES = [
     ("arb_field_create", c_i32, [c_vp, c_i64, c_i32, c_i64, c_i32, c_i32, c_i32, c_vp,
                                  ctypes.POINTER(c_vp)]),
     ("arb_field_info_get", c_i32, [c_vp, ctypes.POINTER(FieldInfo)]),
+    ("arb_field_set_boundary", c_i32, [c_vp, c_i32, c_vp]),
     ("arb_build_coeffs", c_i32, [c_vp, c_vp]),
     ("arb_query", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     ("arb_query_point", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -170,6 +170,9 @@ struct arb_field {
     int32_t table_built = 0;
     int32_t sentinel_set = 0;   // table row nc written
     int32_t table_loaded = 0;   // rows came from arb_table_load
+    int32_t bc = ARB_BC_NONE;   // boundary extension (arb_field_set_boundary)
+    double edge_lo[ARB_MAXD] = {0, 0, 0, 0};   // first / last node coordinate per axis
+    double edge_hi[ARB_MAXD] = {0, 0, 0, 0};
     int device = 0;
     double* vals = nullptr;     // [ninterp][nnodes] FP64, canonical node order
     void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
@@ -425,6 +428,8 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
         g.h[a] = std::fabs(axes[a][0] - axes[a][1]);
         g.lo[a] = axes[a][1];
         g.hi[a] = axes[a][g.n[a] - 2];
+        f->edge_lo[a] = axes[a][0];
+        f->edge_hi[a] = axes[a][g.n[a] - 1];
     }
     g.nnodes = prod;
     // cell strides are 32-bit in the staging arithmetic of the query kernels
@@ -488,6 +493,53 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     out->nnodes = f->g.nnodes;
     out->table_bytes = f->table_bytes;
     out->packed_width = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
+    out->boundary = f->bc;
+    return ARB_OK;
+}
+
+int arb_field_set_boundary(arb_field* f, int32_t bc, void* stream) {
+    if (!f) return fail(ARB_EINVAL, "arb_field_set_boundary: NULL handle");
+    if (bc != ARB_BC_NONE && bc != ARB_BC_NEUMANN && bc != ARB_BC_DIRICHLET)
+        return fail(ARB_EINVAL, "unknown boundary rule %d", bc);
+    if (bc == f->bc) return ARB_OK;
+    if (f->bc != ARB_BC_NONE) return fail(ARB_EINVAL, "the field's boundary is already extended");
+    if (f->table_built || f->sentinel_set || f->table_loaded || f->B.nbt != 0 || f->bw_n != 0 ||
+        f->push_exec)
+        return fail(ARB_EINVAL, "set the boundary before the first table build or query");
+    hipStream_t s = (hipStream_t)stream;
+    const ArbGeom& go = f->g;
+    ArbGeom gn = go;
+    int64_t st = 1;
+    gn.nc = 1;
+    for (int a = 0; a < go.ndim; ++a) {
+        gn.n[a] = go.n[a] + 2;
+        gn.npos[a] = go.n[a] - 1;
+        gn.nstride[a] = st;
+        st *= gn.n[a];
+        gn.nc *= gn.npos[a];
+        gn.lo[a] = f->edge_lo[a];
+        gn.hi[a] = f->edge_hi[a];
+    }
+    gn.nnodes = st;
+    if (gn.nc >= ((int64_t)1 << 32))
+        return fail(ARB_EGRID, "grid too large: %lld cells (limit 2^32)", (long long)gn.nc);
+    const int K = 1 << (2 * go.ndim);
+    const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+    const int64_t tbytes = (int64_t)((gn.nc + 1) * f->ninterp * K * esz);
+    DevBuf nv, nt;
+    HIPCHK(nv.alloc(sizeof(double) * gn.nnodes * f->ninterp));
+    HIPCHK(nt.alloc((size_t)tbytes));
+    HIPCHK(arb_launch_pad_boundary(f->vals, go, nv.as<double>(), gn, f->ninterp, bc, s));
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipFree(f->vals);
+    (void)hipFree(f->table);
+    f->vals = nv.as<double>();
+    f->table = nt.p;
+    nv.p = nullptr;
+    nt.p = nullptr;
+    f->table_bytes = tbytes;
+    f->g = gn;
+    f->bc = bc;
     return ARB_OK;
 }
 
@@ -686,6 +738,7 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
 
 int arb_field_values(const arb_field* f, double* dst_dev, void* stream) {
     if (!f || !dst_dev) return fail(ARB_EINVAL, "arb_field_values: NULL argument");
+    if (f->bc) return fail(ARB_EINVAL, "node values are not available on a boundary-extended field");
     HIPCHK(hipMemcpyAsync(dst_dev, f->vals, sizeof(double) * f->g.nnodes * f->ninterp,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
     return ARB_OK;
@@ -693,6 +746,7 @@ int arb_field_values(const arb_field* f, double* dst_dev, void* stream) {
 
 int arb_field_set_values(arb_field* f, const double* src_dev, void* stream) {
     if (!f || !src_dev) return fail(ARB_EINVAL, "arb_field_set_values: NULL argument");
+    if (f->bc) return fail(ARB_EINVAL, "node values are not settable on a boundary-extended field");
     HIPCHK(hipMemcpyAsync(f->vals, src_dev, sizeof(double) * f->g.nnodes * f->ninterp,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
     f->table_built = 0;
@@ -703,6 +757,8 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
                        const double* gv_dev, const double* gg_dev, double* dq_dev,
                        double* dval_dev, void* stream) {
     if (!f) return fail(ARB_EINVAL, "arb_query_backward: NULL handle");
+    if (f->bc && dval_dev)
+        return fail(ARB_EINVAL, "node gradients are not available on a boundary-extended field");
     if (!f->has_norm) return fail(ARB_EINVAL, "backward needs a scalar interpolant (scalar, norm or both)");
     if (n < 0 || (n > 0 && (!q_dev || !gv_dev || !gg_dev || ldq < f->g.ndim)))
         return fail(ARB_EINVAL, "arb_query_backward: bad arguments");

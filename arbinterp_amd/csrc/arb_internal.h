@@ -226,6 +226,11 @@ hipError_t arb_launch_rank_scatter(const double* rows, int64_t nrows, int64_t ld
                                    int32_t mode, int32_t vcol0, double* vals, int32_t ninterp,
                                    int32_t* seen, hipStream_t s);
 hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* flag, hipStream_t s);
+// Extend the node values src ([ninterp][go.nnodes]) to the grid gn (one ghost
+// layer per face) in dst ([ninterp][gn.nnodes]) with boundary rule bc.
+hipError_t arb_launch_pad_boundary(const double* src, const ArbGeom& go, double* dst,
+                                   const ArbGeom& gn, int32_t ninterp, int32_t bc,
+                                   hipStream_t s);
 
 hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, int32_t ninterp,
                            void* table, hipStream_t s);

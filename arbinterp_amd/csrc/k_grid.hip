@@ -122,6 +122,57 @@ __global__ void k_check_seen(const int32_t* __restrict__ seen, int64_t nnodes, i
     if (seen[i] != 1) flag[0] = 1;
 }
 
+// ---------------------------------------------------------------------------
+// Boundary extension (opt-in; no reference counterpart: README.md:35 plans
+// boundary conditions of the Neumann or Dirichlet type).  One ghost node
+// layer per face: the extended grid has n + 2 nodes per axis, the original
+// node i at index i + 1.  Axes are extended one after another (x, y, z[, t]);
+// the faces of axis a span the already extended axes b < a and the original
+// nodes of the axes b > a, so edge and corner ghosts follow from the same
+// per-axis rule (indices of the extended axis, m = n + 2 nodes):
+//   Neumann   v(0) = v(2)               v(m-1) = v(m-3)
+//   Dirichlet v(0) = 2 v(1) - v(2)      v(m-1) = 2 v(m-2) - v(m-3)
+// i.e. an even (zero normal slope at the edge node) or odd (the edge node's
+// value held, one-sided slope) reflection about the edge node.
+// ---------------------------------------------------------------------------
+__global__ void k_pad_copy(const double* __restrict__ src, ArbGeom go, double* __restrict__ dst,
+                           ArbGeom gn, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int64_t k = i / go.nnodes;
+    int64_t r = i - k * go.nnodes;
+    int64_t o = k * gn.nnodes;
+    for (int a = 0; a < go.ndim; ++a) {
+        const int64_t c = r % go.n[a];
+        r /= go.n[a];
+        o += (c + 1) * gn.nstride[a];
+    }
+    dst[o] = src[i];
+}
+
+__global__ void k_pad_axis(double* __restrict__ v, ArbGeom gn, int32_t axis, int32_t bc,
+                           int64_t count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int64_t r = i >> 1;
+    const bool hi = (i & 1) != 0;
+    int64_t off = 0;
+    for (int b = 0; b < gn.ndim; ++b) {
+        if (b == axis) continue;
+        const int64_t ext = b < axis ? gn.n[b] : gn.n[b] - 2;
+        const int64_t c = r % ext;
+        r /= ext;
+        off += (b < axis ? c : c + 1) * gn.nstride[b];
+    }
+    off += r * gn.nnodes;                           // r: the interpolant
+    const int64_t st = gn.nstride[axis], m = gn.n[axis];
+    const int64_t ghost = off + (hi ? m - 1 : 0) * st;
+    const int64_t edge = off + (hi ? m - 2 : 1) * st;
+    const int64_t next = off + (hi ? m - 3 : 2) * st;
+    const double e = v[edge], nx = v[next];
+    v[ghost] = bc == ARB_BC_NEUMANN ? nx : 2.0 * e - nx;
+}
+
 inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -166,4 +217,24 @@ hipError_t arb_launch_check_seen(const int32_t* seen, int64_t nnodes, int32_t* f
     hipLaunchKernelGGL(k_check_seen, dim3(grid1(nnodes, 256)), dim3(256), 0, s, seen, nnodes,
                        flag);
     return hipGetLastError();
+}
+
+hipError_t arb_launch_pad_boundary(const double* src, const ArbGeom& go, double* dst,
+                                   const ArbGeom& gn, int32_t ninterp, int32_t bc,
+                                   hipStream_t s) {
+    const int64_t total = go.nnodes * ninterp;
+    hipLaunchKernelGGL(k_pad_copy, dim3(grid1(total, 256)), dim3(256), 0, s, src, go, dst, gn,
+                       total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int a = 0; a < gn.ndim; ++a) {
+        int64_t count = 2 * (int64_t)ninterp;
+        for (int b = 0; b < gn.ndim; ++b)
+            if (b != a) count *= b < a ? gn.n[b] : gn.n[b] - 2;
+        hipLaunchKernelGGL(k_pad_axis, dim3(grid1(count, 256)), dim3(256), 0, s, dst, gn, a, bc,
+                           count);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }

@@ -88,6 +88,13 @@ extern "C" {
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
 
+/* ---- boundary extension (arb_field_set_boundary) ----------------------- */
+#define ARB_BC_NONE      0  /* the reference's box: [2nd node, 2nd-last node]   */
+#define ARB_BC_NEUMANN   1  /* ghost nodes by even reflection: zero normal slope
+                               at the edge nodes                                 */
+#define ARB_BC_DIRICHLET 2  /* ghost nodes by odd reflection about the edge node
+                               (its value held, one-sided slope)                 */
+
 typedef struct arb_field arb_field;
 
 typedef struct arb_field_info {
@@ -108,7 +115,7 @@ typedef struct arb_field_info {
     int64_t nnodes;
     int64_t table_bytes;   /* bytes of the coefficient table                    */
     int32_t packed_width;  /* elements per row with ARB_Q_PACKED (4 or 8)       */
-    int32_t reserved;
+    int32_t boundary;      /* ARB_BC_* (arb_field_set_boundary)                 */
 } arb_field_info;
 
 /* Analyse a field and upload it (tricubic/quadcubic constructor minus the
@@ -125,6 +132,20 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
 /* Grid constants (the reference's public attributes nPos, hx.., xIntMin..,
  * nc: ARBInterp.py:545-568). */
 int arb_field_info_get(const arb_field* f, arb_field_info* out);
+
+/* Extend the field by one ghost node layer per face so that the whole
+ * sampled box [first node, last node] is interpolated (opt-in; no reference
+ * counterpart: README.md:35 notes that queries at the edge return NaN and
+ * plans Neumann or Dirichlet boundary conditions).  Every interpolated column
+ * (components and |B| alike) is extended on its own, axis after axis (x, y,
+ * z[, t]): Neumann v(-1) = v(1), v(n) = v(n-2); Dirichlet v(-1) = 2 v(0) -
+ * v(1), v(n) = 2 v(n-1) - v(n-2).  Afterwards the field is the reference
+ * algorithm on the extended grid: nPos = n - 1 cells per axis, IntMin /
+ * IntMax = first / last node, h unchanged, cells numbered on the extended
+ * grid.  Call it once, on a new field, before any table build or query;
+ * arb_field_values, arb_field_set_values and arb_query_backward then return
+ * ARB_EINVAL.  Synchronises `stream`. */
+int arb_field_set_boundary(arb_field* f, int32_t bc, void* stream);
 
 /* Fill the whole coefficient table (allCoeffs, ARBInterp.py:523-525,
  * :1260-1262, with calcCoefficients :570-603 / :1322-1355).  Idempotent. */

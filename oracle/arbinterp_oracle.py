@@ -24,6 +24,11 @@ Pinning (each function is checked by tests that do not retype its formula):
   * field_params  -- shuffle invariance; closed-form base indices; golden.
   * cell reduction/query -- edge battery (IntMax aliasing, NaN rows),
                      golden vectors written by tests/golden/make_golden.py.
+  * pad_nodes/extend_boundary (no reference counterpart) -- exact ghosts for
+                     fields even / odd about every face keep the edge cells
+                     as accurate as the interior (the other rule does not);
+                     Neumann zero normal slope; Dirichlet reproduces linear
+                     fields; interior unchanged; brute-force ghost indices.
 """
 from __future__ import annotations
 
@@ -33,7 +38,7 @@ import numpy as np
 __all__ = [
     "make_B", "make_D", "make_A", "deriv_blocks", "corner_centres",
     "FieldParams", "field_params", "neighbour_offsets", "tricubic", "quadcubic",
-    "coefficients_for_cells",
+    "coefficients_for_cells", "base_point_inds", "pad_nodes", "extend_boundary",
 ]
 
 
@@ -176,15 +181,21 @@ def field_params(field: np.ndarray, ndim: int) -> FieldParams:
     stride = [int(np.prod(n[:a])) for a in range(ndim)]
     int_min = [f[stride[a], a] for a in range(ndim)]
     int_max = [f[-2 * stride[a], a] for a in range(ndim)]
-    # basePointInds: node ids (i_a in [1, n_a-3]) in canonical order (:559-565)
-    base = np.zeros(1, dtype=np.int64) + sum(stride)
-    for a in range(ndim):
-        base = (base[None, :] + stride[a] * np.arange(n[a] - 3)[:, None]).reshape(-1)
-    base = np.sort(base)
+    base = base_point_inds(n, ndim)
     npos = [n[a] - 3 for a in range(ndim)]
     return FieldParams(field=f, n=n, npos=npos, h=h, int_min=int_min,
                        int_max=int_max, stride=stride, base=base, nc=len(base),
                        ndim=ndim)
+
+
+def base_point_inds(n, ndim: int) -> np.ndarray:
+    """basePointInds (ARBInterp.py:559-565): node ids (i_a in [1, n_a-3]) in
+    canonical order."""
+    stride = [int(np.prod(n[:a])) for a in range(ndim)]
+    base = np.zeros(1, dtype=np.int64) + sum(stride)
+    for a in range(ndim):
+        base = (base[None, :] + stride[a] * np.arange(n[a] - 3)[:, None]).reshape(-1)
+    return np.sort(base)
 
 
 def neighbour_offsets(n, ndim: int) -> np.ndarray:
@@ -253,10 +264,9 @@ class _Interp:
         self.kind, self.variant = kind, variant
         f = p.field
         self.interps = {}
-        # sentinel column nc: NaN except scalar mode (zero) and explicit
-        # 'vector' mode (zero at nc, NaN at nc+1) (ARBInterp.py:31,42-45,57,70,85,99)
-        ncol_tab = self.nc + 2 if kind == "vector" else self.nc + 1
-        sentinel = 0.0 if kind in ("scalar", "vector") else np.nan
+        # sentinel column nc (_init_tables): NaN except scalar mode (zero) and
+        # explicit 'vector' mode (zero at nc, NaN at nc+1)
+        # (ARBInterp.py:31,42-45,57,70,85,99)
         if variant in (1, 3):
             for j, name in enumerate("xyz"):
                 self.interps[name] = f[:, d + j]
@@ -265,6 +275,15 @@ class _Interp:
                 self.interps["n"] = f[:, d]
             else:
                 self.interps["n"] = np.linalg.norm(f[:, d:], axis=1)
+        self._init_tables()
+
+    def _init_tables(self):
+        """Coefficient tables (K, nc+1[+1]) with the sentinel columns, and the
+        fill mask."""
+        K = 4 ** self.NDIM
+        kind = self.kind
+        ncol_tab = self.nc + 2 if kind == "vector" else self.nc + 1
+        sentinel = 0.0 if kind in ("scalar", "vector") else np.nan
         self.tables = {}
         for k in self.interps:
             t = np.zeros((K, ncol_tab))
@@ -571,6 +590,57 @@ class quadcubic(_Interp):
     """Oracle mirror of `quadcubic` (ARBInterp.py:627-1381)."""
     NDIM = 4
     CHUNK = 1 << 13
+
+
+# --------------------------------------------------------------------------
+# Boundary extension (no reference counterpart: README.md:35 notes that edge
+# queries return NaN and plans Neumann or Dirichlet boundary conditions)
+# --------------------------------------------------------------------------
+
+def pad_nodes(values: np.ndarray, n, ndim: int, bc: str) -> np.ndarray:
+    """One ghost node per face of the node grid `values` (canonical order, x
+    fastest, axis lengths n), axis after axis (x, y, z[, t]):
+      neumann   v(-1) = v(1),            v(n) = v(n-2)
+      dirichlet v(-1) = 2 v(0) - v(1),   v(n) = 2 v(n-1) - v(n-2)
+    Returns the extended grid flattened in canonical order."""
+    g = values.reshape(tuple(reversed(list(n))))
+    for a in range(ndim):
+        g = np.moveaxis(g, ndim - 1 - a, 0)
+        if bc == "neumann":
+            lo, hi = g[1], g[-2]
+        elif bc == "dirichlet":
+            lo, hi = 2.0 * g[0] - g[1], 2.0 * g[-1] - g[-2]
+        else:
+            raise ValueError("unknown boundary rule %r" % (bc,))
+        g = np.concatenate([lo[None], g, hi[None]], axis=0)
+        g = np.moveaxis(g, 0, ndim - 1 - a)
+    return np.ascontiguousarray(g).reshape(-1)
+
+
+def extend_boundary(obj, bc: str):
+    """Extend an oracle interpolator `obj` (fresh, no cell filled) to the whole
+    sampled box: every interpolated column gets one ghost node per face
+    (pad_nodes), and the grid constants become those of the extended grid
+    with the original spacing: n + 2 nodes and n - 1 cells per axis, IntMin /
+    IntMax = the first / last node.  The reference algorithm then runs
+    unchanged on the extended grid."""
+    p = obj.p
+    d = obj.NDIM
+    n_old = list(p.n)
+    for k in list(obj.interps):
+        obj.interps[k] = pad_nodes(np.asarray(obj.interps[k], dtype=np.float64), n_old, d, bc)
+    n = [m + 2 for m in n_old]
+    p.n = n
+    p.npos = [m - 3 for m in n]
+    p.stride = [int(np.prod(n[:a])) for a in range(d)]
+    p.base = base_point_inds(n, d)
+    p.nc = len(p.base)
+    p.int_min = [p.field[0, a] for a in range(d)]
+    p.int_max = [p.field[-1, a] for a in range(d)]
+    obj.nc = p.nc
+    obj.basePointInds = p.base
+    obj._init_tables()
+    return obj
 
 
 # --------------------------------------------------------------------------
