@@ -193,10 +193,8 @@ struct arb_field {
     // radix-sort workspace: keys/values x2, digit histograms, scan temporaries
     uint32_t* rs_k[2] = {nullptr, nullptr};
     int32_t* rs_v[2] = {nullptr, nullptr};
-    int32_t* rs_hist = nullptr;
-    int32_t* rs_hscan = nullptr;
-    int32_t* rs_tmp = nullptr;
-    int32_t* rs_tmps = nullptr;
+    int32_t* rs_hist = nullptr;    // tile digit histograms
+    int32_t* rs_hscan = nullptr;   // their per-digit scans, then the digit totals
     int64_t rs_n = 0;
     // backward workspace
     int32_t* bw_key = nullptr;     // per query: cell (-1 outside)
@@ -289,20 +287,15 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         }
         if (f->rs_hist) (void)hipFree(f->rs_hist);
         if (f->rs_hscan) (void)hipFree(f->rs_hscan);
-        if (f->rs_tmp) (void)hipFree(f->rs_tmp);
-        if (f->rs_tmps) (void)hipFree(f->rs_tmps);
-        f->rs_hist = f->rs_hscan = f->rs_tmp = f->rs_tmps = nullptr;
+        f->rs_hist = f->rs_hscan = nullptr;
         f->rs_n = 0;
         const int64_t hl = 256 * arb_rs_ntiles(n);
-        const int64_t tl = arb_scan_tmp_elems(hl) + 1;
         for (int k = 0; k < 2; ++k) {
             HIPCHK(hipMalloc(&f->rs_k[k], sizeof(uint32_t) * n));
             HIPCHK(hipMalloc(&f->rs_v[k], sizeof(int32_t) * n));
         }
         HIPCHK(hipMalloc(&f->rs_hist, sizeof(int32_t) * hl));
-        HIPCHK(hipMalloc(&f->rs_hscan, sizeof(int32_t) * (hl + 1)));
-        HIPCHK(hipMalloc(&f->rs_tmp, sizeof(int32_t) * tl));
-        HIPCHK(hipMalloc(&f->rs_tmps, sizeof(int32_t) * tl));
+        HIPCHK(hipMalloc(&f->rs_hscan, sizeof(int32_t) * (hl + 256)));   // + digit totals
         f->rs_n = n;
     }
     if (f->B.nbt + 1 > f->ws_nb) {
@@ -647,12 +640,11 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             }
             HIPCHK(arb_launch_keys(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                    f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
-                                   f->rs_v[0], cellbits, s));
+                                   f->rs_v[0], cellbits, f->rs_hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
             HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
-                                         f->rs_hist, f->rs_hscan, f->rs_tmp, f->rs_tmps,
-                                         &in_first, s));
+                                         f->rs_hist, f->rs_hscan, 1, &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             const int w = in_first ? 0 : 1;
             HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, cellbits, s));
@@ -1076,8 +1068,6 @@ void arb_field_destroy(arb_field* f) {
     }
     if (f->rs_hist) (void)hipFree(f->rs_hist);
     if (f->rs_hscan) (void)hipFree(f->rs_hscan);
-    if (f->rs_tmp) (void)hipFree(f->rs_tmp);
-    if (f->rs_tmps) (void)hipFree(f->rs_tmps);
     if (f->bw_key) (void)hipFree(f->bw_key);
     if (f->bw_perm) (void)hipFree(f->bw_perm);
     if (f->bw_rec) (void)hipFree(f->bw_rec);

@@ -202,33 +202,75 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
 
 // Reduction + bucket key for the radix-sorted path (no atomics): key = bucket,
 // or nbt for rows evaluated here from the sentinel row; value = row index.
+// One workgroup of KEYS_THREADS threads per radix tile (ARB_RS_THREADS x
+// ARB_RS_ITEMS rows): it also counts the low 8 bits of its keys in LDS and
+// writes the tile's histogram, which is the first radix pass's upsweep.
+// Positions are loaded KEYS_BATCH rows at a time so their loads are in flight
+// together.
+constexpr int KEYS_THREADS = 256;
+constexpr int KEYS_ITEMS = ARB_RS_THREADS * ARB_RS_ITEMS / KEYS_THREADS;
+constexpr int KEYS_BATCH = 4;
+static_assert(KEYS_ITEMS % KEYS_BATCH == 0, "rows per thread must be a multiple of the batch");
+
 template <int D, typename T, typename TC, typename TO>
-__global__ void __launch_bounds__(256)
-k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
-       int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-       uint32_t* __restrict__ key, int32_t* __restrict__ val, int32_t cellbits) {
+__device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBuckets& B,
+                                               const TC* __restrict__ table, int32_t ninterp,
+                                               int32_t has_comps, int32_t has_norm,
+                                               const ArbOut& o, int64_t i, const double* qp,
+                                               int32_t cellbits) {
     constexpr int K = (D == 3) ? 64 : 256;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     double loc[D];
     int64_t ia[D];
     bool oob;
-    int64_t flat = arb_reduce_axes<D>(q + i * ldq, g, loc, ia, &oob);
+    const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
     if (o.cells) o.cells[i] = flat;
     if (oob && o.qmut) {
         for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
     }
-    val[i] = (int32_t)i;
     if (flat >= g.nc) {
         if (flat > g.nc) o.flag[0] = 1;
         eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
                                  o, i);
-        key[i] = (uint32_t)B.nbt << cellbits;
-        return;
+        return (uint32_t)B.nbt << cellbits;
     }
     int lc;
     const uint32_t b = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
-    key[i] = (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
+    return (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
+}
+
+template <int D, typename T, typename TC, typename TO>
+__global__ void __launch_bounds__(KEYS_THREADS)
+k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+       int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
+       uint32_t* __restrict__ key, int32_t* __restrict__ val, int32_t cellbits,
+       int32_t* __restrict__ hist, int64_t ntiles) {
+    __shared__ int32_t cnt[256];
+    cnt[threadIdx.x] = 0;                           // KEYS_THREADS == 256 == digits
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * (KEYS_THREADS * KEYS_ITEMS) + threadIdx.x;
+    for (int j0 = 0; j0 < KEYS_ITEMS; j0 += KEYS_BATCH) {
+        double qv[KEYS_BATCH][D];
+#pragma unroll
+        for (int u = 0; u < KEYS_BATCH; ++u) {
+            const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
+            const int64_t r = i < n ? i : 0;        // row 0 exists: n > 0
+#pragma unroll
+            for (int a = 0; a < D; ++a) qv[u][a] = q[r * ldq + a];
+        }
+#pragma unroll
+        for (int u = 0; u < KEYS_BATCH; ++u) {
+            const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
+            if (i < n) {
+                const uint32_t k = key_of_row<D, T, TC, TO>(g, B, table, ninterp, has_comps,
+                                                            has_norm, o, i, qv[u], cellbits);
+                key[i] = k;
+                val[i] = (int32_t)i;
+                atomicAdd(&cnt[k & 255u], 1);
+            }
+        }
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
 // Exclusive scan of nb counts into offs[0..nb] with one workgroup of 1024
@@ -690,12 +732,13 @@ hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, 
                            const void* table, int32_t ninterp, int32_t has_comps,
                            int32_t has_norm, const double* q, int64_t n, int64_t ldq,
                            const ArbOut& o, uint32_t* key, int32_t* val, int32_t cellbits,
-                           hipStream_t s) {
+                           int32_t* hist, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
-    hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3(grid1(n, 256)), dim3(256), 0, s, g, B,      \
-                       (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o, key, val,   \
-                       cellbits)
+    hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
+                       s, g, B, (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o,    \
+                       key, val, cellbits, hist, ntiles)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

@@ -3,22 +3,26 @@
 // bucket (block of cells) without device-scope atomics.
 //
 // One pass over digit d:
-//   upsweep   each tile of 4096 pairs counts its digits in LDS and writes
-//             hist[digit * ntiles + tile];
-//   scan      exclusive scan of hist (digit-major, so the result is the
-//             global start of every (digit, tile) run);
-//   downsweep each tile ranks its pairs stably (wave ballots match equal
-//             digits, per-wave LDS counters keep element order) and writes
-//             them to hist_scan[digit, tile] + rank.
-// Tiles are 256 threads x 16 items; items are striped inside each wave
-// (item j of lane l of wave w is element w*1024 + j*64 + l), which keeps
-// the ranking stable in element order.
+//   histogram  each tile of 4096 pairs counts its digits in LDS and writes
+//              hist[digit * ntiles + tile] (k_rs_upsweep; for the first pass
+//              the query kernel that makes the keys writes it);
+//   digit scan one workgroup per digit scans that digit's tile counts:
+//              hscan[digit * ntiles + tile] = pairs of the digit in earlier
+//              tiles, tot[digit] = the digit's total;
+//   downsweep  each tile adds the exclusive scan of tot (the start of every
+//              digit's run) to its hscan entries, ranks its pairs stably
+//              (wave ballots match equal digits, per-wave LDS counters keep
+//              element order) and writes them to that start + rank.
+// Tiles are ARB_RS_THREADS threads x 16 items; items are striped inside each
+// wave (item j of lane l of wave w is element w*1024 + j*64 + l), which keeps
+// the ranking stable in element order.  Threads below 256 also own a digit
+// each in the per-digit steps.
 #include "arb_internal.h"
 
 namespace {
 
-constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
+constexpr int RS_THREADS = ARB_RS_THREADS;
+constexpr int RS_ITEMS = ARB_RS_ITEMS;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096
 constexpr int RS_WAVES = RS_THREADS / 64;
 
@@ -26,7 +30,8 @@ __global__ void __launch_bounds__(RS_THREADS)
 k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* __restrict__ hist,
              int64_t ntiles) {
     __shared__ int32_t cnt[256];
-    cnt[threadIdx.x] = 0;
+    const bool dt = threadIdx.x < 256;              // this thread owns a digit
+    if (dt) cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll
@@ -35,21 +40,63 @@ k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* _
         if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1);
     }
     __syncthreads();
-    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+    if (dt) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// One workgroup per digit: hscan[d * ntiles + t] = sum of hist[d * ntiles + u]
+// over u < t, tot[d] = the sum over all tiles.  Chunks of RS_TILE counts,
+// RS_ITEMS consecutive ones per thread.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_digit_scan(const int32_t* __restrict__ hist, int64_t ntiles, int32_t* __restrict__ hscan,
+                int32_t* __restrict__ tot) {
+    __shared__ int32_t part[RS_THREADS];
+    const int64_t base = (int64_t)blockIdx.x * ntiles;
+    int32_t carry = 0;
+    for (int64_t c0 = 0; c0 < ntiles; c0 += RS_TILE) {
+        const int64_t t0 = c0 + (int64_t)threadIdx.x * RS_ITEMS;
+        int32_t v[RS_ITEMS];
+        int32_t sum = 0;
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            v[j] = (t0 + j < ntiles) ? hist[base + t0 + j] : 0;
+            sum += v[j];
+        }
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        for (int off = 1; off < RS_THREADS; off <<= 1) {
+            const int32_t x = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += x;
+            __syncthreads();
+        }
+        int32_t run = carry + part[threadIdx.x] - sum;
+#pragma unroll
+        for (int j = 0; j < RS_ITEMS; ++j) {
+            if (t0 + j < ntiles) hscan[base + t0 + j] = run;
+            run += v[j];
+        }
+        carry += part[RS_THREADS - 1];
+        __syncthreads();                            // part is reused by the next chunk
+    }
+    if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
-               const int32_t* __restrict__ hscan, int64_t ntiles) {
+               const int32_t* __restrict__ hscan, const int32_t* __restrict__ tot,
+               int64_t ntiles) {
     __shared__ int32_t wcnt[RS_WAVES][256];     // running digit counts per wave
     __shared__ int32_t gbase[256];              // global start of each digit's run
     __shared__ int32_t toff[256];               // tile-local start of each digit's run
     __shared__ uint32_t skey[RS_TILE];          // the tile's pairs in digit order
     __shared__ int32_t sval[RS_TILE];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool dt = threadIdx.x < 256;              // this thread owns digit threadIdx.x
+    const int dg0 = dt ? threadIdx.x : 0;
     for (int d = threadIdx.x; d < RS_WAVES * 256; d += RS_THREADS) (&wcnt[0][0])[d] = 0;
-    gbase[threadIdx.x] = hscan[(int64_t)threadIdx.x * ntiles + blockIdx.x];
+    const int32_t dtot = tot[dg0];
+    if (dt) gbase[dg0] = dtot;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -76,24 +123,36 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         if (ok && before == 0) wcnt[w][dg] = run + __popcll(peers);   // group leader
     }
     __syncthreads();
-    // exclusive prefix over the waves of this tile, per digit (thread = digit)
+    // exclusive prefix over the waves of this tile, per digit
     int32_t acc = 0;
+    if (dt) {
 #pragma unroll
-    for (int ww = 0; ww < RS_WAVES; ++ww) {
-        const int32_t c = wcnt[ww][threadIdx.x];
-        wcnt[ww][threadIdx.x] = acc;
-        acc += c;
+        for (int ww = 0; ww < RS_WAVES; ++ww) {
+            const int32_t c = wcnt[ww][dg0];
+            wcnt[ww][dg0] = acc;
+            acc += c;
+        }
+        toff[dg0] = acc;
     }
-    // exclusive scan of the tile's digit totals over the 256 digits
-    toff[threadIdx.x] = acc;
+    // exclusive scans over the 256 digits, side by side: the tile's digit
+    // totals (tile-local run starts) and the global digit totals (global run
+    // starts, to which the pairs of the digit in earlier tiles are added)
     __syncthreads();
     for (int off = 1; off < 256; off <<= 1) {
-        const int32_t x = threadIdx.x >= off ? toff[threadIdx.x - off] : 0;
+        const bool on = dt && dg0 >= off;
+        const int32_t x = on ? toff[dg0 - off] : 0;
+        const int32_t y = on ? gbase[dg0 - off] : 0;
         __syncthreads();
-        toff[threadIdx.x] += x;
+        if (dt) {
+            toff[dg0] += x;
+            gbase[dg0] += y;
+        }
         __syncthreads();
     }
-    toff[threadIdx.x] -= acc;
+    if (dt) {
+        toff[dg0] -= acc;
+        gbase[dg0] += hscan[(int64_t)dg0 * ntiles + blockIdx.x] - dtot;
+    }
     __syncthreads();
     // local reorder: digit-major, element order inside each digit
 #pragma unroll
@@ -206,25 +265,29 @@ hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, in
 int64_t arb_rs_ntiles(int64_t n) { return (int64_t)cdiv(n, RS_TILE); }
 
 // Sort (k0, v0) by the low `bits` bits of the keys.  Buffers: k1/v1 the same
-// size as k0/v0; hist and hscan 256 * ntiles ints; tmp/tmp_scan for the scan.
-// The sorted pairs end in (k0, v0) when the number of passes is even, else in
-// (k1, v1); *in_first reports which.
+// size as k0/v0; hist 256 * ntiles ints; hscan 256 * ntiles + 256 ints (the
+// per-tile digit prefixes, then the digit totals).  hist0_ready: hist already
+// holds the first pass's tile histograms (written with the keys).  The sorted
+// pairs end in (k0, v0) when the number of passes is even, else in (k1, v1);
+// *in_first reports which.
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
-                                 int bits, int32_t* hist, int32_t* hscan, int32_t* tmp,
-                                 int32_t* tmp_scan, int* in_first, hipStream_t s) {
+                                 int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
+                                 int* in_first, hipStream_t s) {
     const int64_t ntiles = arb_rs_ntiles(n);
+    int32_t* tot = hscan + 256 * ntiles;
     uint32_t* ka = k0;
     int32_t* va = v0;
     uint32_t* kb = k1;
     int32_t* vb = v1;
     int passes = 0;
     for (int shift = 0; shift < bits; shift += 8, ++passes) {
-        hipLaunchKernelGGL(k_rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n,
-                           shift, hist, ntiles);
-        hipError_t e = arb_launch_scan_long(hist, hscan, 256 * ntiles, tmp, tmp_scan, s);
-        if (e != hipSuccess) return e;
+        if (shift > 0 || !hist0_ready)
+            hipLaunchKernelGGL(k_rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n,
+                               shift, hist, ntiles);
+        hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
+                           tot);
         hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va,
-                           kb, vb, n, shift, hscan, ntiles);
+                           kb, vb, n, shift, hscan, tot, ntiles);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
     }
