@@ -640,11 +640,11 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             }
             HIPCHK(arb_launch_keys(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                    f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
-                                   f->rs_v[0], cellbits, f->rs_hist, s));
+                                   cellbits, f->rs_hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
             HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
-                                         f->rs_hist, f->rs_hscan, 1, &in_first, s));
+                                         f->rs_hist, f->rs_hscan, 1, 1, &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             const int w = in_first ? 0 : 1;
             HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, cellbits, s));

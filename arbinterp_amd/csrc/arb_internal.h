@@ -248,20 +248,21 @@ hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipS
 // longer digit runs of the larger tile make the scattered writes wider).
 constexpr int ARB_RS_THREADS = 512;
 constexpr int ARB_RS_ITEMS = 16;
-// Keys (bucket, or bucket and cell) and row indices of n queries, and the
-// tile histograms of the keys' low 8 bits (hist[digit * ntiles + tile]).
+// Keys (bucket, or bucket and cell) of n queries, and the tile histograms of
+// the keys' low 8 bits (hist[digit * ntiles + tile]).  The value of key i is
+// the row index i, implicit (the radix sort's index_vals).
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* table, int32_t ninterp, int32_t has_comps,
                            int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t* val, int32_t cellbits,
-                           int32_t* hist, hipStream_t s);
+                           const ArbOut& o, uint32_t* key, int32_t cellbits, int32_t* hist,
+                           hipStream_t s);
 int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
 int64_t arb_rs_ntiles(int64_t n);
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
-                                 int* in_first, hipStream_t s);
+                                 int index_vals, int* in_first, hipStream_t s);
 hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
                              int32_t shift, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,

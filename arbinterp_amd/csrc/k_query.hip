@@ -201,7 +201,8 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
 }
 
 // Reduction + bucket key for the radix-sorted path (no atomics): key = bucket,
-// or nbt for rows evaluated here from the sentinel row; value = row index.
+// or nbt for rows evaluated here from the sentinel row (the sort's value is
+// the row index, implicit).
 // One workgroup of KEYS_THREADS threads per radix tile (ARB_RS_THREADS x
 // ARB_RS_ITEMS rows): it also counts the low 8 bits of its keys in LDS and
 // writes the tile's histogram, which is the first radix pass's upsweep.
@@ -242,8 +243,8 @@ template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(KEYS_THREADS)
 k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-       uint32_t* __restrict__ key, int32_t* __restrict__ val, int32_t cellbits,
-       int32_t* __restrict__ hist, int64_t ntiles) {
+       uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
+       int64_t ntiles) {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;                           // KEYS_THREADS == 256 == digits
     __syncthreads();
@@ -264,7 +265,6 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, i
                 const uint32_t k = key_of_row<D, T, TC, TO>(g, B, table, ninterp, has_comps,
                                                             has_norm, o, i, qv[u], cellbits);
                 key[i] = k;
-                val[i] = (int32_t)i;
                 atomicAdd(&cnt[k & 255u], 1);
             }
         }
@@ -731,14 +731,14 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* table, int32_t ninterp, int32_t has_comps,
                            int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t* val, int32_t cellbits,
-                           int32_t* hist, hipStream_t s) {
+                           const ArbOut& o, uint32_t* key, int32_t cellbits, int32_t* hist,
+                           hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
                        s, g, B, (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o,    \
-                       key, val, cellbits, hist, ntiles)
+                       key, cellbits, hist, ntiles)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

@@ -108,7 +108,7 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         const int64_t e = base + j * 64 + lane;
         const bool ok = e < n;
         key[j] = ok ? kin[e] : 0xFFFFFFFFu;
-        val[j] = ok ? vin[e] : 0;
+        val[j] = ok ? (vin ? vin[e] : (int32_t)e) : 0;   // no vin: the value is the index
         const uint32_t dg = (key[j] >> shift) & 255u;
         uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -267,12 +267,13 @@ int64_t arb_rs_ntiles(int64_t n) { return (int64_t)cdiv(n, RS_TILE); }
 // Sort (k0, v0) by the low `bits` bits of the keys.  Buffers: k1/v1 the same
 // size as k0/v0; hist 256 * ntiles ints; hscan 256 * ntiles + 256 ints (the
 // per-tile digit prefixes, then the digit totals).  hist0_ready: hist already
-// holds the first pass's tile histograms (written with the keys).  The sorted
-// pairs end in (k0, v0) when the number of passes is even, else in (k1, v1);
-// *in_first reports which.
+// holds the first pass's tile histograms (written with the keys).
+// index_vals: v0 is not read; the value of pair i is i.  The sorted pairs end
+// in (k0, v0) when the number of passes is even, else in (k1, v1); *in_first
+// reports which.
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
-                                 int* in_first, hipStream_t s) {
+                                 int index_vals, int* in_first, hipStream_t s) {
     const int64_t ntiles = arb_rs_ntiles(n);
     int32_t* tot = hscan + 256 * ntiles;
     uint32_t* ka = k0;
@@ -286,7 +287,8 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
                                shift, hist, ntiles);
         hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
                            tot);
-        hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, va,
+        const int32_t* vin = (shift == 0 && index_vals) ? nullptr : va;
+        hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, vin,
                            kb, vb, n, shift, hscan, tot, ntiles);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
