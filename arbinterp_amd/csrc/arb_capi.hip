@@ -303,7 +303,7 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         if (f->ws_offs) (void)hipFree(f->ws_offs);
         f->ws_count = f->ws_offs = nullptr;
         f->ws_nb = 0;
-        HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * (f->B.nbt + 1)));
+        HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * arb_rs_first_elems(f->B.nbt)));
         HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (f->B.nbt + 1)));
         f->ws_nb = f->B.nbt + 1;
     }
@@ -643,12 +643,13 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                    cellbits, f->rs_hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
+            // the sort also leaves each bucket's first sorted position in
+            // ws_offs (ws_count as scratch)
             HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
-                                         f->rs_hist, f->rs_hscan, 1, 1, &in_first, s));
+                                         f->rs_hist, f->rs_hscan, 1, 1, f->ws_offs, f->B.nbt,
+                                         cellbits, f->ws_count, &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
-            const int w = in_first ? 0 : 1;
-            HIPCHK(arb_launch_bounds(f->rs_k[w], n, f->ws_offs, f->B.nbt, cellbits, s));
-            perm = f->rs_v[w];
+            perm = f->rs_v[in_first ? 0 : 1];
         }
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));
         if (flags & ARB_Q_SORTED_GLOBAL)

@@ -260,11 +260,11 @@ int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
 int64_t arb_rs_ntiles(int64_t n);
+int64_t arb_rs_first_elems(int64_t ngroups);   // scratch ints of arb_launch_radix_sort
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
-                                 int index_vals, int* in_first, hipStream_t s);
-hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
-                             int32_t shift, hipStream_t s);
+                                 int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
+                                 int32_t* first, int* in_first, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
                               int32_t* perm, int64_t n, hipStream_t s);
 size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads);

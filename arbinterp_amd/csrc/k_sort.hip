@@ -23,7 +23,7 @@ namespace {
 
 constexpr int RS_THREADS = ARB_RS_THREADS;
 constexpr int RS_ITEMS = ARB_RS_ITEMS;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 4096
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 8192
 constexpr int RS_WAVES = RS_THREADS / 64;
 
 __global__ void __launch_bounds__(RS_THREADS)
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(RS_THREADS)
 k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
                const int32_t* __restrict__ hscan, const int32_t* __restrict__ tot,
-               int64_t ntiles) {
+               int64_t ntiles, int32_t* __restrict__ first, int32_t fshift) {
     __shared__ int32_t wcnt[RS_WAVES][256];     // running digit counts per wave
     __shared__ int32_t gbase[256];              // global start of each digit's run
     __shared__ int32_t toff[256];               // tile-local start of each digit's run
@@ -165,7 +165,11 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         sval[lp] = val[j];
     }
     __syncthreads();
-    // write out: consecutive slots of one digit go to consecutive addresses
+    // write out: consecutive slots of one digit go to consecutive addresses.
+    // Last pass with `first`: the keys are not written; instead every slot
+    // that starts a run of equal (key >> fshift) inside its digit's run
+    // offers its position to first[key >> fshift] (the sorted order makes
+    // the minimum over tiles the group's first position).
     const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
     const int cnt = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
     for (int i = threadIdx.x; i < cnt; i += RS_THREADS) {
@@ -173,8 +177,78 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         const uint32_t dg = (k >> shift) & 255u;
         const int64_t pos = (int64_t)gbase[dg] + (i - toff[dg]);
         if (pos < 0 || pos >= n) continue;    // unreachable: a digit's run stays inside [0, n)
-        kout[pos] = k;
         vout[pos] = sval[i];
+        if (first) {
+            if (i == toff[dg] || (skey[i - 1] >> fshift) != (k >> fshift))
+                atomicMin(&first[k >> fshift], (int32_t)pos);
+        } else {
+            kout[pos] = k;
+        }
+    }
+}
+
+// offs[b] = min(first[b .. len-1]) for b in [0, len): the first sorted
+// position whose group is >= b (first[b] = n for an absent group).  Two
+// launches over blocks of SF_BLOCK entries (SF_ITEMS consecutive ones per
+// thread): the minimum of every block, then per block the suffix minimum,
+// seeded with the minimum of the blocks after it.
+constexpr int SF_THREADS = 256, SF_ITEMS = 16, SF_BLOCK = SF_THREADS * SF_ITEMS;
+
+__global__ void __launch_bounds__(SF_THREADS)
+k_block_min(const int32_t* __restrict__ first, int64_t len, int32_t* __restrict__ bmin) {
+    __shared__ int32_t part[SF_THREADS];
+    const int64_t b0 = (int64_t)blockIdx.x * SF_BLOCK + (int64_t)threadIdx.x * SF_ITEMS;
+    int32_t m = INT32_MAX;
+#pragma unroll
+    for (int j = 0; j < SF_ITEMS; ++j)
+        if (b0 + j < len) m = min(m, first[b0 + j]);
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = SF_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) part[threadIdx.x] = min(part[threadIdx.x], part[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bmin[blockIdx.x] = part[0];
+}
+
+__global__ void __launch_bounds__(SF_THREADS)
+k_suffix_min(const int32_t* __restrict__ first, int64_t len, const int32_t* __restrict__ bmin,
+             int32_t* __restrict__ offs) {
+    __shared__ int32_t part[SF_THREADS];
+    // minimum over the later blocks
+    int32_t carry = INT32_MAX;
+    for (int64_t k = blockIdx.x + 1 + threadIdx.x; k < gridDim.x; k += SF_THREADS)
+        carry = min(carry, bmin[k]);
+    part[threadIdx.x] = carry;
+    __syncthreads();
+    for (int off = SF_THREADS / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) part[threadIdx.x] = min(part[threadIdx.x], part[threadIdx.x + off]);
+        __syncthreads();
+    }
+    carry = part[0];
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * SF_BLOCK + (int64_t)threadIdx.x * SF_ITEMS;
+    int32_t v[SF_ITEMS];
+    int32_t m = INT32_MAX;
+#pragma unroll
+    for (int j = 0; j < SF_ITEMS; ++j) {
+        v[j] = (b0 + j < len) ? first[b0 + j] : INT32_MAX;
+        m = min(m, v[j]);
+    }
+    // inclusive suffix minimum over the threads of the block
+    part[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 1; off < SF_THREADS; off <<= 1) {
+        const int32_t x = threadIdx.x + off < SF_THREADS ? part[threadIdx.x + off] : INT32_MAX;
+        __syncthreads();
+        part[threadIdx.x] = min(part[threadIdx.x], x);
+        __syncthreads();
+    }
+    int32_t run = min(carry, threadIdx.x + 1 < SF_THREADS ? part[threadIdx.x + 1] : INT32_MAX);
+#pragma unroll
+    for (int j = SF_ITEMS - 1; j >= 0; --j) {
+        run = min(run, v[j]);
+        if (b0 + j < len) offs[b0 + j] = run;
     }
 }
 
@@ -230,16 +304,9 @@ k_sc_local(const int32_t* __restrict__ in, int64_t len, const int32_t* __restric
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[len] = bscan[gridDim.x];   // total
 }
 
-// offs[b] = first sorted position whose bucket (key >> shift) >= b, for b in
-// [0, nb]; keys are sorted, sentinel rows carry bucket nb and sort last.
-__global__ void k_bounds(const uint32_t* __restrict__ keys, int64_t n, int32_t* __restrict__ offs,
-                         int64_t nb, int32_t shift) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > n) return;
-    const int64_t kp = (s == 0) ? -1 : (int64_t)(keys[s - 1] >> shift);
-    int64_t k = (s == n) ? nb : (int64_t)(keys[s] >> shift);
-    if (k > nb) k = nb;
-    for (int64_t b = kp + 1; b <= k; ++b) offs[b] = (int32_t)s;
+__global__ void k_fill_first(int32_t* __restrict__ p, int64_t len, int32_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < len) p[i] = v;
 }
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
@@ -264,16 +331,24 @@ hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, in
 
 int64_t arb_rs_ntiles(int64_t n) { return (int64_t)cdiv(n, RS_TILE); }
 
+int64_t arb_rs_first_elems(int64_t ngroups) {
+    return ngroups + 1 + (int64_t)cdiv(ngroups + 1, SF_BLOCK);
+}
+
 // Sort (k0, v0) by the low `bits` bits of the keys.  Buffers: k1/v1 the same
 // size as k0/v0; hist 256 * ntiles ints; hscan 256 * ntiles + 256 ints (the
 // per-tile digit prefixes, then the digit totals).  hist0_ready: hist already
 // holds the first pass's tile histograms (written with the keys).
-// index_vals: v0 is not read; the value of pair i is i.  The sorted pairs end
-// in (k0, v0) when the number of passes is even, else in (k1, v1); *in_first
-// reports which.
+// index_vals: v0 is not read; the value of pair i is i.  The sorted values
+// end in v0 when the number of passes is even, else in v1; *in_first reports
+// which.  The sorted keys are not kept: instead offs[g] (g in [0, ngroups])
+// becomes the first sorted position whose key >> gshift is >= g, keys being
+// below (ngroups + 1) << gshift; first holds arb_rs_first_elems(ngroups)
+// ints of scratch.
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
-                                 int index_vals, int* in_first, hipStream_t s) {
+                                 int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
+                                 int32_t* first, int* in_first, hipStream_t s) {
     const int64_t ntiles = arb_rs_ntiles(n);
     int32_t* tot = hscan + 256 * ntiles;
     uint32_t* ka = k0;
@@ -281,7 +356,10 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
     uint32_t* kb = k1;
     int32_t* vb = v1;
     int passes = 0;
+    hipLaunchKernelGGL(k_fill_first, dim3(cdiv(ngroups + 1, 256)), dim3(256), 0, s, first,
+                       ngroups + 1, (int32_t)n);
     for (int shift = 0; shift < bits; shift += 8, ++passes) {
+        const bool last = shift + 8 >= bits;
         if (shift > 0 || !hist0_ready)
             hipLaunchKernelGGL(k_rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n,
                                shift, hist, ntiles);
@@ -289,16 +367,16 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
                            tot);
         const int32_t* vin = (shift == 0 && index_vals) ? nullptr : va;
         hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, vin,
-                           kb, vb, n, shift, hscan, tot, ntiles);
+                           kb, vb, n, shift, hscan, tot, ntiles, last ? first : nullptr, gshift);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
     }
+    const unsigned nsb = cdiv(ngroups + 1, SF_BLOCK);
+    int32_t* bmin = first + ngroups + 1;
+    hipLaunchKernelGGL(k_block_min, dim3(nsb), dim3(SF_THREADS), 0, s, first, ngroups + 1, bmin);
+    hipLaunchKernelGGL(k_suffix_min, dim3(nsb), dim3(SF_THREADS), 0, s, first, ngroups + 1, bmin,
+                       offs);
     *in_first = (passes % 2 == 0);
     return hipGetLastError();
 }
 
-hipError_t arb_launch_bounds(const uint32_t* keys, int64_t n, int32_t* offs, int64_t nb,
-                             int32_t shift, hipStream_t s) {
-    hipLaunchKernelGGL(k_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, s, keys, n, offs, nb, shift);
-    return hipGetLastError();
-}
