@@ -668,7 +668,7 @@ hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t
 #define ARB_FUSED_SHAPES(X) X(4, 4, 4) X(4, 4, 2) X(4, 2, 2) X(2, 2, 2) X(2, 2, 1) X(2, 1, 1) X(1, 1, 1)
 
 // LDS bytes of k_eval_fused3 for the bucket geometry B (0 when the path does
-// not apply: 3D only, an instantiated block shape, within 64 KiB, node tiles
+// not apply: 3D only, an instantiated block shape, within 160 KiB, node tiles
 // of all interpolants within the registers of `threads` threads).
 size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads) {
     if (g.ndim != 3) return 0;
@@ -685,7 +685,7 @@ size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int
                        sizeof(double) * (size_t)(arbs3::sep3_nodes(bx, by, bz) +
                                                  arbs3::sep3_t1(bx, by, bz) +
                                                  arbs3::sep3_t2(bx, by, bz));
-    return lds <= 64 * 1024 ? lds : 0;
+    return lds <= 160 * 1024 ? lds : 0;            // gfx950: 160 KiB of LDS per workgroup
 }
 
 hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t prec,

@@ -183,14 +183,20 @@ def test_deterministic_repeat(dev):
 
 def test_full_size_dipole_sample(dev):
     """128^3 vector field, mode 'both': 10^6 queries on the GPU; 20k of them
-    checked against the oracle; every row in range and finite."""
+    checked against the oracle; every row in range and finite.  At this size
+    the FP64 buckets are 4x4x2 cells (66.5 KB of LDS per workgroup), and the
+    fused kernel (another 14 KB) must give the same bits."""
     import torch
+    from arbinterp_amd import _native as NN
     rows, q = W.dipole3d(seed=2, nq=1_000_000)
     t = dev.tricubic(rows, "quiet", mode="both")
     qd = torch.from_numpy(q).cuda()
     c, nv, gr, cells = t.query_device(qd, cells=True)
+    fused = t.query_device(qd, flags=NN.ARB_Q_SORTED | NN.ARB_Q_FUSED)
     torch.cuda.synchronize()
     assert int(cells.max()) < t.nc and torch.isfinite(nv).all()
+    for a, b in zip((c, nv, gr), fused[:3]):
+        assert torch.equal(a, b)
     o = O.tricubic(rows, "quiet", mode="both")
     idx = np.random.default_rng(0).choice(len(q), 20_000, replace=False)
     co, no, go = o.Query(q[idx].copy())
