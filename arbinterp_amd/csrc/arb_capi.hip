@@ -205,7 +205,7 @@ struct arb_field {
     int32_t* bw_offs = nullptr;    // nc + 1
     int32_t* bw_cursor = nullptr;  // nc + 1
     int32_t* bw_tmp = nullptr;     // scan temporaries
-    double* bw_bbar = nullptr;     // K x nc (slot-major)
+    double* bw_bbar = nullptr;     // node-adjoint scratch: 3D block tiles, 4D K x nc
     // particle push graph
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t push_exec = nullptr;
@@ -780,7 +780,6 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     hipStream_t s = (hipStream_t)stream;
     int rc = arb_build_coeffs(f, stream);
     if (rc) return rc;
-    const int K = 1 << (2 * f->g.ndim);
     const int64_t nc = f->g.nc;
     if (dval_dev) {
         if (n > f->bw_n) {
@@ -801,7 +800,7 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             HIPCHK(hipMalloc(&f->bw_offs, sizeof(int32_t) * (nc + 1)));
             HIPCHK(hipMalloc(&f->bw_cursor, sizeof(int32_t) * (nc + 1)));
             HIPCHK(hipMalloc(&f->bw_tmp, sizeof(int32_t) * 2 * tl));
-            HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * nc * K));
+            HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * arb_bwd_scratch(f->g)));
         }
         HIPCHK(arb_launch_fill_i32(f->bw_count, nc + 1, 0, s));
         HIPCHK(arb_launch_fill_i32(f->bw_cursor, nc + 1, 0, s));

@@ -295,6 +295,8 @@ hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double*
                                 const int32_t* offs, const int32_t* perm, double* bbar,
                                 hipStream_t s);
 hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s);
+// Doubles of node-adjoint scratch (bbar of arb_launch_bwd_cells / _nodes).
+int64_t arb_bwd_scratch(const ArbGeom& g);
 size_t arb_batch_field_desc_size();
 void arb_batch_fill_desc(void* dst, const ArbGeom& g, const void* table, int64_t q0, int64_t nq,
                          int64_t item0);

@@ -6,12 +6,14 @@
 //   d/df_n = sum_i sum_j c_ij A_{j, l(n, cell_i)}    (U is linear in f)
 //            c_i = gv_i m(x_i) + sum_k gg_ik dm_k(x_i)/h_k
 // The node adjoint is deterministic: queries are grouped per cell (counting
-// sort), each cell's segment is put in query-index order, the per-cell
-// 64 (256)-vector alpha_bar is summed in that order, b_bar = A^T alpha_bar
-// (3D: FP64 matrix cores, 16-cell tiles), stored slot-major bbar[l][cell],
-// and every node gathers b_bar from its <= 4^d cells in a fixed slot order
-// (coalesced: neighbouring nodes read neighbouring cells).  No floating-point
-// atomics.
+// sort), each cell's segment is put in query-index order, and the per-cell
+// 64 (256)-vector alpha_bar is summed in that order.  3D: blocks of 4^3
+// cells apply the transpose of the separable coefficient passes to their
+// alpha_bar, giving partial gradients on the block's 7^3 node tile, and
+// every node sums the tiles holding it in a fixed block order.  4D:
+// b_bar = A^T alpha_bar per cell, stored slot-major bbar[l][cell], and every
+// node gathers b_bar from its <= 4^4 cells in a fixed slot order.  No
+// floating-point atomics.
 #include "arb_internal.h"
 
 namespace {
@@ -227,43 +229,61 @@ __device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __rest
     }
 }
 
-// 3D: a workgroup takes 64 consecutive cells.  Four threads per cell form
+// 3D: a workgroup takes a block of BB^3 cells.  Four threads per cell form
 // alpha_bar into LDS from the queries' records (k_bwd_points: u, v, w, gv,
 // gg / h): thread g of a cell sums, over the cell's queries in index order,
 // the entries ex + 4 ey + 16 ez with ex + 4 ey in 4g..4g+3,
 //   gv u^ex v^ey w^ez + gk_u (u^ex)' v^ey w^ez + gk_v u^ex (v^ey)' w^ez
 //                     + gk_w u^ex v^ey (w^ez)'      (gk = gg / h).
-// Then wave w computes rows w*16..w*16+15 of b_bar = A^T alpha_bar for the
-// four 16-cell column tiles on v_mfma_f64_16x16x4f64 (its A operands loaded
-// once) and writes bbar[l][cell] (16 consecutive cells per row: coalesced).
-typedef double d4 __attribute__((ext_vector_type(4)));
-
+// A = M (x) M (x) M (z slowest; M the 4x4 map of arb_sep3.h), so the node
+// adjoint of the block, A^T alpha_bar spread over the cells' neighbourhoods,
+// is the transpose of the separable passes: z, then y, then x, each a gather
+//   out(n) = sum over (c, o) with c + o = n of  sum_e M[e][o] in(c, e)
+// over the cells c of the block and offsets o = 0..3.  The result is the
+// block's node tile, (BB+3)^3 partial node gradients, written to
+// tiles[block][node].  Every sum runs in a fixed order (no atomics).
 __device__ __forceinline__ void pow3(double x, double* p, double* dp) {
     p[0] = 1.0; p[1] = x; p[2] = x * x; p[3] = p[2] * x;
     dp[0] = 0.0; dp[1] = 1.0; dp[2] = 2.0 * x; dp[3] = 3.0 * p[2];
 }
 
-constexpr int BWD_CELLS = 64;
+// sum_e M[e][o] a_e (column o of M applied to a; M as in arb_sep3.h)
+__device__ __forceinline__ double mcol(int o, double a0, double a1, double a2, double a3) {
+    switch (o) {
+        case 0: return fma(-0.5, a3, fma(-0.5, a1, a2));
+        case 1: return fma(1.5, a3, fma(-2.5, a2, a0));
+        case 2: return fma(-1.5, a3, fma(0.5, a1, 2.0 * a2));
+        default: return 0.5 * (a3 - a2);
+    }
+}
+
+constexpr int BB = 4;                              // block edge (cells)
+constexpr int BN = BB + 3;                         // node tile edge
+constexpr int BCELLS = BB * BB * BB;
+constexpr int BTILE = BN * BN * BN;
 
 __global__ void __launch_bounds__(256)
-k_bwd_cells3(ArbGeom g, const double* __restrict__ A, const double* __restrict__ rec,
-             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
-             double* __restrict__ bbar) {
-    __shared__ double ab[BWD_CELLS][65];           // alpha_bar[cell][k], padded
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * BWD_CELLS;
-    double aop[16];                                // A^T[w*16 + l%16][ks*4 + l/16]
-#pragma unroll
-    for (int ks = 0; ks < 16; ++ks) aop[ks] = A[(ks * 4 + (lane >> 4)) * 64 + w * 16 + (lane & 15)];
+k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restrict__ offs,
+             const int32_t* __restrict__ perm, int64_t nbx, int64_t nby,
+             double* __restrict__ tiles) {
+    // alpha_bar[cell][i] (padded); after the z pass the same memory holds
+    // t1[z][y][cx][ex] (47 KB in all: three workgroups per CU)
+    __shared__ double ab[BCELLS][65];
+    __shared__ double t2[BN][BB][BB][16];           // [z][cy][cx][ex + 4 ey]
+    static_assert(BN * BN * BB * 4 <= BCELLS * 65, "t1 must fit in alpha_bar's memory");
+    double (*t1)[BN][BB][4] = reinterpret_cast<double (*)[BN][BB][4]>(&ab[0][0]);
+    const int64_t blk = blockIdx.x;
+    const int64_t bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
     {
         const int cl = threadIdx.x >> 2, gq = threadIdx.x & 3;
-        const int64_t c = c0 + cl;
+        const int64_t x = bx * BB + (cl & 3), y = by * BB + ((cl >> 2) & 3), z = bz * BB + (cl >> 4);
         double acc[4][4];                          // [pair][ez]
 #pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
             for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
-        if (c < g.nc) {
+        if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
+            const int64_t c = x + g.npos[0] * (y + g.npos[1] * z);
             const int32_t s1 = offs[c + 1];
             for (int32_t s = offs[c]; s < s1; ++s) {
                 const int64_t i = perm[s];
@@ -292,20 +312,75 @@ k_bwd_cells3(ArbGeom g, const double* __restrict__ A, const double* __restrict__
             for (int ez = 0; ez < 4; ++ez) ab[cl][4 * gq + p + 16 * ez] = acc[p][ez];
     }
     __syncthreads();
+    // z: t2[z][cy][cx][exy] = sum_{cz + oz = z} sum_ez M[ez][oz] ab[cell][exy + 16 ez]
+    for (int t = threadIdx.x; t < BN * BB * BB * 16; t += 256) {
+        const int exy = t & 15, cx = (t >> 4) & 3, cy = (t >> 6) & 3, z = t >> 8;
+        double sum = 0.0;
 #pragma unroll
-    for (int ct = 0; ct < BWD_CELLS / 16; ++ct) {
-        d4 d = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int ks = 0; ks < 16; ++ks)
-            d = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[ks], ab[ct * 16 + (lane & 15)][ks * 4 + (lane >> 4)],
-                                                     d, 0, 0, 0);
-        const int64_t cell = c0 + ct * 16 + (lane & 15);
-        if (cell < g.nc) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                bbar[(int64_t)(w * 16 + (lane >> 4) + 4 * r) * g.nc + cell] = d[r];
+        for (int oz = 0; oz < 4; ++oz) {
+            const int cz = z - oz;
+            if (cz >= 0 && cz < BB) {
+                const double* a = &ab[cx + BB * (cy + BB * cz)][exy];
+                sum += mcol(oz, a[0], a[16], a[32], a[48]);
+            }
         }
+        t2[z][cy][cx][exy] = sum;
     }
+    __syncthreads();
+    // y: t1[z][y][cx][ex] = sum_{cy + oy = y} sum_ey M[ey][oy] t2[z][cy][cx][ex + 4 ey]
+    for (int t = threadIdx.x; t < BN * BN * BB * 4; t += 256) {
+        const int ex = t & 3, cx = (t >> 2) & 3, r = t >> 4, y = r % BN, z = r / BN;
+        double sum = 0.0;
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            const int cy = y - oy;
+            if (cy >= 0 && cy < BB) {
+                const double* a = &t2[z][cy][cx][ex];
+                sum += mcol(oy, a[0], a[4], a[8], a[12]);
+            }
+        }
+        t1[z][y][cx][ex] = sum;
+    }
+    __syncthreads();
+    // x: tile[z][y][x] = sum_{cx + ox = x} sum_ex M[ex][ox] t1[z][y][cx][ex]
+    double* out = tiles + blk * BTILE;
+    for (int t = threadIdx.x; t < BTILE; t += 256) {
+        const int x = t % BN, r = t / BN, y = r % BN, z = r / BN;
+        double sum = 0.0;
+#pragma unroll
+        for (int ox = 0; ox < 4; ++ox) {
+            const int cx = x - ox;
+            if (cx >= 0 && cx < BB) {
+                const double* a = &t1[z][y][cx][0];
+                sum += mcol(ox, a[0], a[1], a[2], a[3]);
+            }
+        }
+        out[t] = sum;
+    }
+}
+
+// Node gradient: the sum, over the blocks whose tile holds the node (at most
+// two per axis), of the tile entries, blocks in ascending z, y, x order.
+__global__ void __launch_bounds__(256)
+k_bwd_nodes3(ArbGeom g, const double* __restrict__ tiles, int64_t nbx, int64_t nby, int64_t nbz,
+             double* __restrict__ dval) {
+    const int64_t nidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (nidx >= g.nnodes) return;
+    const int64_t i = nidx % g.n[0], j = (nidx / g.n[0]) % g.n[1], k = nidx / (g.n[0] * g.n[1]);
+    // block b's tile holds nodes BB*b .. BB*b + BN - 1
+    auto lo = [](int64_t v) { const int64_t b = v - (BN - 1); return b <= 0 ? (int64_t)0 : (b + BB - 1) / BB; };
+    const int64_t x0 = lo(i), x1 = min(i / BB, nbx - 1);
+    const int64_t y0 = lo(j), y1 = min(j / BB, nby - 1);
+    const int64_t z0 = lo(k), z1 = min(k / BB, nbz - 1);
+    double sum = 0.0;
+    for (int64_t bz = z0; bz <= z1; ++bz)
+        for (int64_t by = y0; by <= y1; ++by)
+            for (int64_t bx = x0; bx <= x1; ++bx) {
+                const int64_t blk = bx + nbx * (by + nby * bz);
+                const int64_t t = (i - BB * bx) + BN * ((j - BB * by) + BN * (k - BB * bz));
+                sum += tiles[blk * BTILE + t];
+            }
+    dval[nidx] = sum;
 }
 
 // 4D: one wave per cell: alpha_bar (256) summed over the cell's queries in
@@ -409,22 +484,44 @@ hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, h
     return hipGetLastError();
 }
 
+// Blocks of the 3D node adjoint per axis, and the scratch it needs (doubles).
+void arb_bwd_blocks3(const ArbGeom& g, int64_t* nb3) {
+    for (int a = 0; a < 3; ++a) nb3[a] = (g.npos[a] + BB - 1) / BB;
+}
+
+int64_t arb_bwd_scratch(const ArbGeom& g) {
+    if (g.ndim == 3) {
+        int64_t nb3[3];
+        arb_bwd_blocks3(g, nb3);
+        return nb3[0] * nb3[1] * nb3[2] * BTILE;
+    }
+    return g.nc * 256;
+}
+
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
                                 const int32_t* offs, const int32_t* perm, double* bbar,
                                 hipStream_t s) {
-    if (g.ndim == 3)
-        hipLaunchKernelGGL(k_bwd_cells3, dim3(grid1(g.nc, BWD_CELLS)), dim3(256), 0, s, g, A, rec,
-                           offs, perm, bbar);
-    else
+    if (g.ndim == 3) {
+        int64_t nb3[3];
+        arb_bwd_blocks3(g, nb3);
+        hipLaunchKernelGGL(k_bwd_block3, dim3((unsigned)(nb3[0] * nb3[1] * nb3[2])), dim3(256), 0,
+                           s, g, rec, offs, perm, nb3[0], nb3[1], bbar);
+    } else {
         hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, 4)), dim3(256), 0, s, g, A, q, ldq, gv,
                            gg, offs, perm, bbar);
+    }
     return hipGetLastError();
 }
 
 hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s) {
     dim3 grid(grid1(g.nnodes, 256)), block(256);
-    if (g.ndim == 3) hipLaunchKernelGGL(k_bwd_nodes<3>, grid, block, 0, s, g, bbar, dval);
-    else hipLaunchKernelGGL(k_bwd_nodes<4>, grid, block, 0, s, g, bbar, dval);
+    if (g.ndim == 3) {
+        int64_t nb3[3];
+        arb_bwd_blocks3(g, nb3);
+        hipLaunchKernelGGL(k_bwd_nodes3, grid, block, 0, s, g, bbar, nb3[0], nb3[1], nb3[2], dval);
+    } else {
+        hipLaunchKernelGGL(k_bwd_nodes<4>, grid, block, 0, s, g, bbar, dval);
+    }
     return hipGetLastError();
 }
