@@ -246,26 +246,34 @@ static void choose_buckets(arb_field* f) {
     int dims[4] = {1, 1, 1, 1};
     if (f->user_block[0] > 0) {
         for (int a = 0; a < d; ++a) dims[a] = f->user_block[a];
-    } else if (d == 3) {
+    } else {
         const int big_cells = (80 * 1024) / (B.cellstride * 16);
-        int pick = -1, n3 = (int)(sizeof ladder3 / sizeof ladder3[0]);
-        for (int i = 0; i < n3 && pick < 0; ++i)
-            if (ladder3[i][0] * ladder3[i][1] * ladder3[i][2] <= max_cells) pick = i;
-        if (pick < 0) pick = n3 - 1;
-        if (bucket_count(f, ladder3[pick]) >= ((int64_t)1 << 16))
+        const int nl = d == 3 ? (int)(sizeof ladder3 / sizeof ladder3[0])
+                              : (int)(sizeof ladder4 / sizeof ladder4[0]);
+        auto entry = [&](int i, int* e) {
+            for (int a = 0; a < 4; ++a) e[a] = a < d ? (d == 3 ? ladder3[i][a] : ladder4[i][a]) : 1;
+        };
+        auto cells = [&](int i) {
+            int e[4];
+            entry(i, e);
+            return e[0] * e[1] * e[2] * e[3];
+        };
+        auto count = [&](int i) {
+            int e[4];
+            entry(i, e);
+            return bucket_count(f, e);
+        };
+        int pick = -1;
+        for (int i = 0; i < nl && pick < 0; ++i)
+            if (cells(i) <= max_cells) pick = i;
+        if (pick < 0) pick = nl - 1;
+        if (count(pick) >= ((int64_t)1 << 16))
             for (int i = pick - 1; i >= 0; --i)
-                if (ladder3[i][0] * ladder3[i][1] * ladder3[i][2] <= big_cells &&
-                    bucket_count(f, ladder3[i]) < ((int64_t)1 << 16)) {
+                if (cells(i) <= big_cells && count(i) < ((int64_t)1 << 16)) {
                     pick = i;
                     break;
                 }
-        for (int a = 0; a < 3; ++a) dims[a] = ladder3[pick][a];
-    } else {
-        for (auto& c : ladder4)
-            if (c[0] * c[1] * c[2] * c[3] <= max_cells) {
-                for (int a = 0; a < 4; ++a) dims[a] = c[a];
-                break;
-            }
+        entry(pick, dims);
     }
     B.ncell = 1;
     B.nbt = 1;
