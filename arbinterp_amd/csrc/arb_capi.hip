@@ -179,8 +179,10 @@ struct arb_field {
     int64_t table_bytes = 0;
     double* A = nullptr;        // K x K FP64
     int32_t* flag = nullptr;    // device status word
-    // bucketed query path
+    // bucketed query path: block geometry of the table kernels (B) and of the
+    // fused kernel (BF, without the larger blocks that only save a sort pass)
     ArbBuckets B{};
+    ArbBuckets BF{};
     int32_t user_block[ARB_MAXD] = {0, 0, 0, 0};
     int32_t eval_threads = 256;
     int32_t* ws_bkt = nullptr;  // per query: bucket (-1 = handled in place)
@@ -222,21 +224,20 @@ struct arb_field {
 
 // Choose the cell block of one bucket: the largest of a fixed ladder whose
 // coefficients (padded by 16 bytes per cell) fit in 48 KiB of LDS, unless the
-// caller set one.  When that leaves 2^16 buckets or more (a third 8-bit
-// radix pass), a larger block within 80 KiB that brings the count below 2^16
-// is taken instead: on the 128^3 FP64 field this trades 0.06 ms of
-// evaluation (fewer workgroups per CU) for a 0.08 ms sort pass.
+// caller set one.  With `upgrade`, when that leaves 2^16 buckets or more (a
+// third 8-bit radix pass), a larger block within 80 KiB that brings the
+// count below 2^16 is taken instead: on the 128^3 FP64 field this trades
+// 0.06 ms of evaluation (fewer workgroups per CU) for a 0.08 ms sort pass.
 static int64_t bucket_count(const arb_field* f, const int* dims) {
     int64_t nb = 1;
     for (int a = 0; a < f->g.ndim; ++a) nb *= (f->g.npos[a] + dims[a] - 1) / dims[a];
     return nb;
 }
 
-static void choose_buckets(arb_field* f) {
+static void choose_buckets(arb_field* f, ArbBuckets& B, bool upgrade) {
     const int d = f->g.ndim;
     const int K = 1 << (2 * d);
     const int esz = f->precision == ARB_PREC_FP64 ? 8 : 4;
-    ArbBuckets& B = f->B;
     B.cellvec = f->ninterp * K * esz / 16;
     B.cellstride = B.cellvec + 1;
     static const int ladder3[][3] = {{4, 4, 4}, {4, 4, 2}, {4, 2, 2}, {2, 2, 2}, {2, 2, 1}, {2, 1, 1}, {1, 1, 1}};
@@ -267,7 +268,7 @@ static void choose_buckets(arb_field* f) {
         for (int i = 0; i < nl && pick < 0; ++i)
             if (cells(i) <= max_cells) pick = i;
         if (pick < 0) pick = nl - 1;
-        if (count(pick) >= ((int64_t)1 << 16))
+        if (upgrade && count(pick) >= ((int64_t)1 << 16))
             for (int i = pick - 1; i >= 0; --i)
                 if (cells(i) <= big_cells && count(i) < ((int64_t)1 << 16)) {
                     pick = i;
@@ -294,7 +295,7 @@ static void choose_buckets(arb_field* f) {
     }
 }
 
-static int ensure_workspace(arb_field* f, int64_t n) {
+static int ensure_workspace(arb_field* f, int64_t n, int64_t nbt) {
     if (n > f->ws_n) {
         if (f->ws_bkt) (void)hipFree(f->ws_bkt);
         if (f->ws_rank) (void)hipFree(f->ws_rank);
@@ -326,14 +327,14 @@ static int ensure_workspace(arb_field* f, int64_t n) {
         HIPCHK(hipMalloc(&f->rs_hscan, sizeof(int32_t) * (hl + 256)));   // + digit totals
         f->rs_n = n;
     }
-    if (f->B.nbt + 1 > f->ws_nb) {
+    if (nbt + 1 > f->ws_nb) {
         if (f->ws_count) (void)hipFree(f->ws_count);
         if (f->ws_offs) (void)hipFree(f->ws_offs);
         f->ws_count = f->ws_offs = nullptr;
         f->ws_nb = 0;
-        HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * arb_rs_first_elems(f->B.nbt)));
-        HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (f->B.nbt + 1)));
-        f->ws_nb = f->B.nbt + 1;
+        HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * arb_rs_first_elems(nbt)));
+        HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (nbt + 1)));
+        f->ws_nb = nbt + 1;
     }
     return ARB_OK;
 }
@@ -604,13 +605,17 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
     if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
     if (n > INT32_MAX - 1) sorted = false;
-    if (sorted && f->B.nbt == 0) choose_buckets(f);
+    if (sorted && f->B.nbt == 0) choose_buckets(f, f->B, true);
     // 3D, bucketed, on request: coefficients formed in the query kernel from
     // node values, no table (ignored once a table has been loaded, or when
     // the block does not fit the kernel's LDS)
-    const bool fused = (flags & ARB_Q_FUSED) && sorted && f->g.ndim == 3 && !f->table_loaded &&
-                       !(flags & ARB_Q_SORTED_GLOBAL) &&
-                       arb_fused_lds(f->g, f->B, f->ninterp, f->eval_threads) > 0;
+    bool fused = false;
+    if ((flags & ARB_Q_FUSED) && sorted && f->g.ndim == 3 && !f->table_loaded &&
+        !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC))) {
+        if (f->BF.nbt == 0) choose_buckets(f, f->BF, false);
+        fused = arb_fused_lds(f->g, f->BF, f->ninterp, f->eval_threads) > 0;
+    }
+    const ArbBuckets& BB = fused ? f->BF : f->B;
     int rc = fused ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (n == 0) return ARB_OK;
@@ -641,7 +646,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(hipEventElapsedTime(&f->last_ms[3], f->ev[0], f->ev[4]));
         }
     } else {
-        rc = ensure_workspace(f, n);
+        rc = ensure_workspace(f, n, BB.nbt);
         if (rc) return rc;
         const int32_t* perm = f->ws_perm;
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
@@ -658,15 +663,15 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             // key = bucket (then the cell inside it with ARB_Q_SORT_CELL)
             int cellbits = 0;
             if (flags & ARB_Q_SORT_CELL)
-                while ((1 << cellbits) < f->B.ncell) ++cellbits;
+                while ((1 << cellbits) < BB.ncell) ++cellbits;
             int bits = 1;
-            while (((int64_t)1 << bits) <= f->B.nbt) ++bits;      // buckets are 0..nbt
+            while (((int64_t)1 << bits) <= BB.nbt) ++bits;        // buckets are 0..nbt
             bits += cellbits;
             if (bits > 32) {          // no room for the cell: order by bucket only
                 bits -= cellbits;
                 cellbits = 0;
             }
-            HIPCHK(arb_launch_keys(f->g, f->B, f->precision, out64, f->table, f->ninterp,
+            HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, f->table, f->ninterp,
                                    f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
                                    cellbits, f->rs_hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
@@ -674,7 +679,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             // the sort also leaves each bucket's first sorted position in
             // ws_offs (ws_count as scratch)
             HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
-                                         f->rs_hist, f->rs_hscan, 1, 1, f->ws_offs, f->B.nbt,
+                                         f->rs_hist, f->rs_hscan, 1, 1, f->ws_offs, BB.nbt,
                                          cellbits, f->ws_count, &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             perm = f->rs_v[in_first ? 0 : 1];
@@ -685,7 +690,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                         perm, s));
         else if (fused)
-            HIPCHK(arb_launch_eval_fused(f->g, f->B, f->precision, out64, f->vals, f->ninterp,
+            HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                          perm, f->eval_threads, s));
         else
@@ -1042,7 +1047,8 @@ int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads) {
         for (int a = 0; a < ARB_MAXD; ++a) f->user_block[a] = 0;
     }
     f->eval_threads = threads ? threads : 256;
-    choose_buckets(f);
+    choose_buckets(f, f->B, true);
+    f->BF.nbt = 0;                                  // chosen again at the next fused query
     return ARB_OK;
 }
 
