@@ -28,34 +28,47 @@ __global__ void k_sentinel(TO* table, int64_t nc, int32_t width, TO v) {
 inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // 3D, both precisions: alpha = (M (x) M (x) M) . F by separable passes
-// (arbs3::sep3_passes) over an 8 x 4 x 1 tile of cells per workgroup, in
-// FP64, stored as TO (FP64 table, or FP32 rounded once for mixed).
-constexpr int SX = 8, SY = 4;
+// (arbs3::sep3_passes) over an 8 x 4 x 4 tile of cells per workgroup, in
+// FP64, stored as TO (FP64 table, or FP32 rounded once for mixed).  The tile
+// reads 11 x 7 x 7 nodes per interpolant for 128 cells (4.2 per cell); the
+// loads are unconditional (index 0 for nodes past the grid edge, which only
+// cells outside the grid use) so that they issue back to back.
+constexpr int SX = 8, SY = 4, SZ3 = 4;
 
 template <typename TO>
 __global__ void __launch_bounds__(256)
 k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
-    constexpr int NX = SX + 3, NY = SY + 3, NZ = 4;
-    __shared__ double nodes[NX * NY * NZ];
+    constexpr int NX = SX + 3, NY = SY + 3, NZ = SZ3 + 3, NT = NX * NY * NZ;
+    constexpr int PER = (NT + 255) / 256;
+    __shared__ double nodes[NT];
     __shared__ __align__(16) double t1[NY * NZ * SX * 4];
     __shared__ __align__(16) double t2[NZ * SY * SX * 16];
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
-    const int64_t cz = blockIdx.z;
+    const int64_t cz0 = (int64_t)blockIdx.z * SZ3;
     for (int k = 0; k < ninterp; ++k) {
         const double* F = vals + (int64_t)k * g.nnodes;
-        __syncthreads();                            // the previous interpolant is done
-        for (int i = threadIdx.x; i < NX * NY * NZ; i += blockDim.x) {
+        double v[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = threadIdx.x + 256 * j;
             const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
-            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz + z;
-            double v = 0.0;
-            if (nx < g.n[0] && ny < g.n[1]) v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz];
-            nodes[i] = v;
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz0 + z;
+            const bool in = i < NT && nx < g.n[0] && ny < g.n[1] && nz < g.n[2];
+            const double t = F[in ? nx + g.nstride[1] * ny + g.nstride[2] * nz : 0];
+            v[j] = in ? t : 0.0;
+        }
+        __syncthreads();                            // the previous interpolant is done
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = threadIdx.x + 256 * j;
+            if (i < NT) nodes[i] = v[j];
         }
         __syncthreads();
-        arbs3::sep3_passes<SX, SY, 1, false>(nodes, t1, t2, [&](int cx, int cy, int, int i, double a) {
-            const int64_t x = cx0 + cx, y = cy0 + cy;
-            if (x < g.npos[0] && y < g.npos[1]) {
-                const int64_t cell = x + g.npos[0] * (y + g.npos[1] * cz);
+        arbs3::sep3_passes<SX, SY, SZ3, false>(nodes, t1, t2,
+                                              [&](int cx, int cy, int cz, int i, double a) {
+            const int64_t x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
+            if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
+                const int64_t cell = x + g.npos[0] * (y + g.npos[1] * z);
                 table[(cell * ninterp + k) * 64 + i] = (TO)a;
             }
         });
@@ -171,7 +184,7 @@ hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, i
     if (g.nc == 0) return hipSuccess;
     const dim3 block(256);
     if (g.ndim == 3) {
-        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)g.npos[2]);
+        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), cdiv(g.npos[2], SZ3));
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL(k_coef_sep3<double>, grid, block, 0, s, g, vals, ninterp, (double*)table);
         else
