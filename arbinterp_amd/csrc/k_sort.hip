@@ -3,7 +3,7 @@
 // bucket (block of cells) without device-scope atomics.
 //
 // One pass over digit d:
-//   histogram  each tile of 4096 pairs counts its digits in LDS and writes
+//   histogram  each tile of 8192 pairs counts its digits in LDS and writes
 //              hist[digit * ntiles + tile] (k_rs_upsweep; for the first pass
 //              the query kernel that makes the keys writes it);
 //   digit scan one workgroup per digit scans that digit's tile counts:
