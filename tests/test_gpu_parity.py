@@ -366,3 +366,31 @@ def test_single_point_matches_batch(dev, golden, precision):
                     (cls, mode, precision, i)
             assert t.queryInd == inds[i], (cls, mode, precision, i)
         t.close()
+
+
+@pytest.mark.parametrize("dim", [3, 4])
+def test_empty_query_batches(dev, golden, dim):
+    """A (0, d) batch returns empty outputs of the oracle's shapes (numpy's
+    own behaviour in the reference's rQuery), and the device path returns
+    empty tensors without launching anything."""
+    import torch
+    if dim == 3:
+        rows, _ = W.dipole3d(seed=2, n=(12, 11, 10), nq=10)
+        cls, ocls = dev.tricubic, O.tricubic
+    else:
+        rows = golden("vector4d_modes")["field"]
+        cls, ocls = dev.quadcubic, O.quadcubic
+    fields = [(rows[:, :dim + 1].copy(), {})] + [(rows, {"mode": m}) for m in ("both", "vector", "norm")]
+    for fld, kw in fields:
+        for prec in ("fp64", "mixed"):
+            t = cls(fld.copy(), "quiet", precision=prec, **kw)
+            o = ocls(fld.copy(), "quiet", **kw)
+            got = t.Query(np.zeros((0, dim)))
+            ref = o.Query(np.zeros((0, dim)))
+            got = got if isinstance(got, tuple) else (got,)
+            ref = ref if isinstance(ref, tuple) else (ref,)
+            assert [g.shape for g in got] == [r.shape for r in ref], (kw, prec)
+            assert t.queryInds.shape == (0,)
+            out = t.query_device(torch.zeros((0, dim), dtype=torch.float64, device="cuda"))
+            assert all(x is None or x.shape[0] == 0 for x in out)
+            t.close()
