@@ -437,14 +437,18 @@ k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has
         for (int a = 0; a < D; ++a) qp[a] = q[qq * ldq + a];
     }
     // stage the block: 8 independent 16-byte loads in flight per thread.  Loads
-    // and LDS stores are unconditional so that they issue back to back.
+    // and LDS stores are unconditional so that they issue back to back.  The
+    // table is read once per call, so the loads are nontemporal: the stream
+    // does not evict the positions, which the queries read back at random
+    // (the 128^3 mixed eval: 0.89 -> 0.80 ms).
     constexpr int BATCH = 8;
     for (int v0 = threadIdx.x; v0 < total; v0 += BATCH * blockDim.x) {
         v4u r[BATCH];
         int dst[BATCH];
 #pragma unroll
         for (int u = 0; u < BATCH; ++u)
-            r[u] = tv[stage_slot<D>(v0 + u * blockDim.x, total, B, cs, sb, &dst[u])];
+            r[u] = __builtin_nontemporal_load(
+                &tv[stage_slot<D>(v0 + u * blockDim.x, total, B, cs, sb, &dst[u])]);
 #pragma unroll
         for (int u = 0; u < BATCH; ++u) lds[dst[u]] = r[u];
     }

@@ -37,7 +37,7 @@ k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* _
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int64_t e = base + j * RS_THREADS + threadIdx.x;
-        if (e < n) atomicAdd(&cnt[(keys[e] >> shift) & 255u], 1);
+        if (e < n) atomicAdd(&cnt[(__builtin_nontemporal_load(&keys[e]) >> shift) & 255u], 1);
     }
     __syncthreads();
     if (dt) hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
@@ -107,8 +107,9 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int64_t e = base + j * 64 + lane;
         const bool ok = e < n;
-        key[j] = ok ? kin[e] : 0xFFFFFFFFu;
-        val[j] = ok ? (vin ? vin[e] : (int32_t)e) : 0;   // no vin: the value is the index
+        // pairs are read once: nontemporal, to keep the query positions cached
+        key[j] = ok ? __builtin_nontemporal_load(&kin[e]) : 0xFFFFFFFFu;
+        val[j] = ok ? (vin ? __builtin_nontemporal_load(&vin[e]) : (int32_t)e) : 0;   // no vin: the index
         const uint32_t dg = (key[j] >> shift) & 255u;
         uint64_t peers = __ballot(ok);
 #pragma unroll
