@@ -184,6 +184,13 @@ class _Interp:
         N.check(N.lib().arb_last_timings(self._h, ms))
         return [float(x) for x in ms]
 
+    def field_info(self):
+        """The library's current description of the field (arb_field_info):
+        grid constants, table state, bucket block, node-tile bytes."""
+        info = N.FieldInfo()
+        N.check(N.lib().arb_field_info_get(self._h, ctypes.byref(info)))
+        return info
+
     def set_tuning(self, block_cells=None, threads=0):
         """Cells per bucket block (one entry per axis, None = default) and
         threads per evaluation workgroup for the bucketed query path."""

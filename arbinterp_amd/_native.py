@@ -43,6 +43,7 @@ ARB_Q_GROUP_ATOMIC = 128
 ARB_Q_PACKED = 256
 ARB_Q_SORT_CELL = 512
 ARB_Q_FUSED = 1024
+ARB_Q_TABLE = 2048
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
@@ -57,7 +58,8 @@ class FieldInfo(ctypes.Structure):
         ("n", c_i64 * 4), ("npos", c_i64 * 4), ("h", ctypes.c_double * 4),
         ("int_min", ctypes.c_double * 4), ("int_max", ctypes.c_double * 4),
         ("nc", c_i64), ("nnodes", c_i64), ("table_bytes", c_i64),
-        ("packed_width", c_i32), ("boundary", c_i32),
+        ("packed_width", c_i32), ("boundary", c_i32), ("block", c_i32 * 4),
+        ("tiles_bytes", c_i64),
     ]
 
 

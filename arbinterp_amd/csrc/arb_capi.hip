@@ -175,6 +175,9 @@ struct arb_field {
     double edge_hi[ARB_MAXD] = {0, 0, 0, 0};
     int device = 0;
     double* vals = nullptr;     // [ninterp][nnodes] FP64, canonical node order
+    void* tiles = nullptr;      // node tiles of the buckets of B (node-form path)
+    int64_t tiles_bytes = 0;
+    int32_t tiles_ok = 0;       // tiles hold the current node values
     void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
     int64_t table_bytes = 0;
     double* A = nullptr;        // K x K FP64
@@ -516,6 +519,8 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     out->table_bytes = f->table_bytes;
     out->packed_width = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
     out->boundary = f->bc;
+    for (int a = 0; a < f->g.ndim; ++a) out->block[a] = f->B.nbt ? f->B.b[a] : 0;
+    out->tiles_bytes = f->tiles_bytes;
     return ARB_OK;
 }
 
@@ -615,9 +620,28 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (f->BF.nbt == 0) choose_buckets(f, f->BF, false);
         fused = arb_fused_lds(f->g, f->BF, f->ninterp, f->eval_threads) > 0;
     }
+    // mixed 3D 'both', bucketed (unless the table is asked for): evaluation
+    // from node tiles, no table
+    const bool nodeform = !fused && !(flags & ARB_Q_TABLE) && sorted &&
+                          f->precision == ARB_PREC_MIXED && !f->table_loaded &&
+                          !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC)) &&
+                          arb_nodes_ok(f->g, f->B, f->ninterp, f->has_comps, f->has_norm);
     const ArbBuckets& BB = fused ? f->BF : f->B;
-    int rc = fused ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
+    int rc = (fused || nodeform) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
+    if (nodeform && !f->tiles_ok) {
+        const int64_t tb = arb_tiles_bytes(f->B, nullptr);
+        if (tb != f->tiles_bytes) {
+            if (f->tiles) (void)hipFree(f->tiles);
+            f->tiles = nullptr;
+            f->tiles_bytes = 0;
+            HIPCHK(hipMalloc(&f->tiles, (size_t)tb));
+            f->tiles_bytes = tb;
+        }
+        HIPCHK(arb_launch_build_tiles(f->g, f->B, f->vals, f->tiles, s));
+        f->tiles_ok = 1;
+    }
+
     if (n == 0) return ARB_OK;
     if (flags & ARB_Q_CHECK) HIPCHK(hipMemsetAsync(f->flag + 1, 0, sizeof(int32_t), s));
     ArbOut o{};
@@ -689,6 +713,9 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                         perm, s));
+        else if (nodeform)
+            HIPCHK(arb_launch_eval_nodes(f->g, BB, out64, f->tiles, q_dev, n, ldq, o,
+                                         f->ws_offs, perm, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
@@ -776,6 +803,7 @@ int arb_field_set_values(arb_field* f, const double* src_dev, void* stream) {
     HIPCHK(hipMemcpyAsync(f->vals, src_dev, sizeof(double) * f->g.nnodes * f->ninterp,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
     f->table_built = 0;
+    f->tiles_ok = 0;
     return ARB_OK;
 }
 
@@ -1049,6 +1077,7 @@ int arb_set_tuning(arb_field* f, const int32_t* block_cells, int32_t threads) {
     f->eval_threads = threads ? threads : 256;
     choose_buckets(f, f->B, true);
     f->BF.nbt = 0;                                  // chosen again at the next fused query
+    f->tiles_ok = 0;                                // node tiles follow B
     return ARB_OK;
 }
 
@@ -1088,6 +1117,7 @@ int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_
 void arb_field_destroy(arb_field* f) {
     if (!f) return;
     if (f->vals) (void)hipFree(f->vals);
+    if (f->tiles) (void)hipFree(f->tiles);
     if (f->table) (void)hipFree(f->table);
     if (f->A) (void)hipFree(f->A);
     if (f->flag) (void)hipFree(f->flag);

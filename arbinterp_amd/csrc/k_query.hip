@@ -14,46 +14,11 @@ __device__ __forceinline__ void put(void* p, int64_t idx, T v) {
     reinterpret_cast<TO*>(p)[idx] = (TO)v;
 }
 
-// Evaluate the interpolants of one point from its cell's coefficients `a`
-// ([interp][K], in global memory or LDS) and write row i of the outputs.
-template <int D, typename T, typename TC, typename TO>
-__device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32_t has_norm,
-                                           const double* loc, const ArbGeom& g, const ArbOut& o,
-                                           int64_t i) {
-    constexpr int K = (D == 3) ? 64 : 256;
-    T x[D];
-#pragma unroll
-    for (int d = 0; d < D; ++d) x[d] = (T)loc[d];
-    TO rec[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r) rec[r] = (TO)0;
-    int k = 0, w = 0;
-    if (has_comps) {
-#pragma unroll 1
-        for (int c = 0; c < 3; ++c) {
-            T v;
-            if constexpr (D == 3) v = arb_val3<T, TC>(a + c * K, x[0], x[1], x[2]);
-            else v = arb_val4<T, TC>(a + c * K, x[0], x[1], x[2], x[3]);
-            rec[c] = (TO)v;
-        }
-        k = 3;
-        w = 3;
-    }
-    if (has_norm) {
-        T v, gr[D];
-        if constexpr (D == 3) arb_valgrad3<T, TC>(a + k * K, x[0], x[1], x[2], v, gr);
-        else arb_valgrad4<T, TC>(a + k * K, x[0], x[1], x[2], x[3], v, gr);
-        if (has_comps) {
-            rec[3] = (TO)v;
-#pragma unroll
-            for (int d = 0; d < D; ++d) rec[4 + d] = (TO)(gr[d] / (T)g.h[d]);
-        } else {
-            rec[0] = (TO)v;
-#pragma unroll
-            for (int d = 0; d < D; ++d) rec[1 + d] = (TO)(gr[d] / (T)g.h[d]);
-        }
-        w += 1 + D;
-    }
+// Write row i of the outputs from its record [comps(3)] [value, grad(D)]
+// (rec[0] = value when there are no components).
+template <int D, typename TO>
+__device__ __forceinline__ void store_rec(const TO* rec, int32_t has_comps, int32_t has_norm,
+                                          const ArbOut& o, int64_t i) {
     if (o.packed) {
         // one aligned record per row, written as full 16-byte vectors
         TO* dst = reinterpret_cast<TO*>(o.comps) + i * o.packed;
@@ -82,7 +47,47 @@ __device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32
 #pragma unroll
         for (int d = 0; d < D; ++d) put<TO>(o.grads, i * D + d, rec[b + 1 + d]);
     }
-    (void)w;
+}
+
+// Evaluate the interpolants of one point from its cell's coefficients `a`
+// ([interp][K], in global memory or LDS) and write row i of the outputs.
+template <int D, typename T, typename TC, typename TO>
+__device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32_t has_norm,
+                                           const double* loc, const ArbGeom& g, const ArbOut& o,
+                                           int64_t i) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    T x[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) x[d] = (T)loc[d];
+    TO rec[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) rec[r] = (TO)0;
+    int k = 0;
+    if (has_comps) {
+#pragma unroll 1
+        for (int c = 0; c < 3; ++c) {
+            T v;
+            if constexpr (D == 3) v = arb_val3<T, TC>(a + c * K, x[0], x[1], x[2]);
+            else v = arb_val4<T, TC>(a + c * K, x[0], x[1], x[2], x[3]);
+            rec[c] = (TO)v;
+        }
+        k = 3;
+    }
+    if (has_norm) {
+        T v, gr[D];
+        if constexpr (D == 3) arb_valgrad3<T, TC>(a + k * K, x[0], x[1], x[2], v, gr);
+        else arb_valgrad4<T, TC>(a + k * K, x[0], x[1], x[2], x[3], v, gr);
+        if (has_comps) {
+            rec[3] = (TO)v;
+#pragma unroll
+            for (int d = 0; d < D; ++d) rec[4 + d] = (TO)(gr[d] / (T)g.h[d]);
+        } else {
+            rec[0] = (TO)v;
+#pragma unroll
+            for (int d = 0; d < D; ++d) rec[1 + d] = (TO)(gr[d] / (T)g.h[d]);
+        }
+    }
+    store_rec<D, TO>(rec, has_comps, has_norm, o, i);
 }
 
 // Cell reduction of row i with the side outputs every path writes: queryInds,
@@ -556,6 +561,257 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Node form (mixed precision, 3D, mode 'both').  Since the coefficients are
+// (M x M x M) F, the interpolant of a cell is
+//   p(u, v, w) = sum_ijk wx_i(u) wy_j(v) wz_k(w) F(cx + i, cy + j, cz + k),
+// w_o(u) = sum_e M[e][o] u^e (the cubic Hermite weights with central-
+// difference slopes), so a query can be evaluated from its 4 x 4 x 4 nodes.
+//
+// Node tiles.  Per bucket, the block's node tile (cells + 3 per axis, x
+// fastest) is kept as FP32 differences from a polynomial model of each
+// interpolant around the tile's middle node m, with p = node - m:
+//   components: L(p) = F(m) + G . p,
+//   |B| (the interpolant whose gradient is output): Q(p) = L(p) + p^T H p / 2,
+// G and H the central first and second differences at m (FP64), together
+// with the models (TileHead).  Central differences are exact for quadratics,
+// so the weights reproduce every polynomial of degree <= 2 per axis
+// (sum_o w_o = 1, sum_o w_o (o - 1) = u, sum_o w_o (o - 1)^2 = u^2, and the
+// same for w' with 0, 1, 2u), and
+//   p = Q(query) + sum w (F - Q),  grad p = grad Q + sum w' (F - Q):
+// the model is evaluated in FP64 and only the differences, whose size is set
+// by the field's third derivatives across the tile (second for the
+// components), carry FP32 rounding.  (Rounding the node values themselves
+// to FP32 would leave an error of order 2^-24 |F| in the gradient, since the
+// gradient weights sum to zero.)  Each tile record holds all four
+// interpolants (16 bytes).
+// ---------------------------------------------------------------------------
+struct TileHead {
+    double f[4];             // F(m) per interpolant
+    double gx[4], gy[4], gz[4];   // central differences at m, per node step
+    double hxx, hyy, hzz, hxy, hxz, hyz;   // second differences of interpolant 3
+    int32_t m[3];            // m in tile coordinates
+    int32_t pad;
+};
+
+// The model of interpolant k at p (tile coordinates relative to m).
+__device__ __forceinline__ double tile_model(const TileHead& h, int k, double px, double py,
+                                            double pz) {
+    double v = fma(h.gz[k], pz, fma(h.gy[k], py, fma(h.gx[k], px, h.f[k])));
+    if (k == 3) {
+        const double q = fma(0.5 * h.hzz, pz * pz,
+                             fma(0.5 * h.hyy, py * py, 0.5 * h.hxx * (px * px)));
+        v += fma(h.hyz, py * pz, fma(h.hxz, px * pz, fma(h.hxy, px * py, q)));
+    }
+    return v;
+}
+
+template <int BX, int BY, int BZ>
+struct TileShape {
+    static constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
+    static constexpr int NT = NX * NY * NZ;
+    static constexpr int NTP = (NT + 15) / 16 * 16;      // records per tile in HBM
+    // LDS strides in records: rows of 8, and a plane pad that puts the two
+    // z layers of a 4 x 4 x 2 block in different bank groups
+    static constexpr int YS = (NX + 7) / 8 * 8, ZS = NY * YS + 4;
+};
+
+// Tile origin (first cell = first node) of bucket b, per axis.
+__device__ __forceinline__ void bucket_origin3(int64_t b, const ArbBuckets& B, int64_t* c0) {
+    int64_t rem = b;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        c0[a] = (rem % B.nb[a]) << B.bsh[a];
+        rem /= B.nb[a];
+    }
+}
+
+// One workgroup per bucket: the head, then every record of the tile.
+template <int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256)
+k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
+               float4* __restrict__ tiles, TileHead* __restrict__ heads) {
+    using S = TileShape<BX, BY, BZ>;
+    __shared__ TileHead h;
+    const int64_t b = blockIdx.x;
+    int64_t c0[3];
+    bucket_origin3(b, B, c0);
+    if (threadIdx.x < 4) {
+        const int k = threadIdx.x;
+        const int64_t nd[3] = {S::NX / 2, S::NY / 2, S::NZ / 2};
+        int64_t m[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            // middle node, kept one node inside the grid so that the central
+            // differences exist
+            m[a] = c0[a] + nd[a];
+            m[a] = m[a] < 1 ? 1 : (m[a] > g.n[a] - 2 ? g.n[a] - 2 : m[a]);
+        }
+        const double* v = vals + k * g.nnodes;
+        const int64_t mi = m[0] + g.nstride[1] * m[1] + g.nstride[2] * m[2];
+        const double f = v[mi];
+        h.f[k] = f;
+        h.gx[k] = 0.5 * (v[mi + 1] - v[mi - 1]);
+        h.gy[k] = 0.5 * (v[mi + g.nstride[1]] - v[mi - g.nstride[1]]);
+        h.gz[k] = 0.5 * (v[mi + g.nstride[2]] - v[mi - g.nstride[2]]);
+        if (k == 3) {
+            const int64_t sx = 1, sy = g.nstride[1], sz = g.nstride[2];
+            h.hxx = v[mi + sx] - 2.0 * f + v[mi - sx];
+            h.hyy = v[mi + sy] - 2.0 * f + v[mi - sy];
+            h.hzz = v[mi + sz] - 2.0 * f + v[mi - sz];
+            h.hxy = 0.25 * ((v[mi + sx + sy] - v[mi + sx - sy]) - (v[mi - sx + sy] - v[mi - sx - sy]));
+            h.hxz = 0.25 * ((v[mi + sx + sz] - v[mi + sx - sz]) - (v[mi - sx + sz] - v[mi - sx - sz]));
+            h.hyz = 0.25 * ((v[mi + sy + sz] - v[mi + sy - sz]) - (v[mi - sy + sz] - v[mi - sy - sz]));
+        }
+        if (k == 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) h.m[a] = (int32_t)(m[a] - c0[a]);
+            h.pad = 0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) heads[b] = h;
+    for (int e = threadIdx.x; e < S::NTP; e += blockDim.x) {
+        const int x = e % S::NX, y = (e / S::NX) % S::NY, z = e / (S::NX * S::NY);
+        const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
+        // records past the tile or nodes past the grid (used only by cells
+        // outside the grid) are 0
+        const bool in = e < S::NT && nx < g.n[0] && ny < g.n[1] && nz < g.n[2];
+        const int64_t src = in ? nx + g.nstride[1] * ny + g.nstride[2] * nz : 0;
+        const double px = x - h.m[0], py = y - h.m[1], pz = z - h.m[2];
+        float d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            d[k] = in ? (float)(vals[k * g.nnodes + src] - tile_model(h, k, px, py, pz)) : 0.f;
+        tiles[b * S::NTP + e] = make_float4(d[0], d[1], d[2], d[3]);
+    }
+}
+
+__device__ __forceinline__ void hermite_weights(float u, float* w, float* dw) {
+    w[0] = u * fmaf(u, fmaf(-0.5f, u, 1.0f), -0.5f);
+    w[1] = fmaf(u * u, fmaf(1.5f, u, -2.5f), 1.0f);
+    w[2] = u * fmaf(u, fmaf(-1.5f, u, 2.0f), 0.5f);
+    w[3] = (u * u) * fmaf(0.5f, u, -0.5f);
+    dw[0] = fmaf(u, fmaf(-1.5f, u, 2.0f), -0.5f);
+    dw[1] = u * fmaf(4.5f, u, -5.0f);
+    dw[2] = fmaf(u, fmaf(-4.5f, u, 4.0f), 0.5f);
+    dw[3] = u * fmaf(1.5f, u, -1.0f);
+}
+
+__device__ __forceinline__ float4 fma4(float a, float4 x, float4 y) {
+    return make_float4(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z), fmaf(a, x.w, y.w));
+}
+
+// Evaluate row qi (position qp) of bucket b from the node tile in LDS.
+template <typename TO, int BX, int BY, int BZ>
+__device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets& B,
+                                              const float4* tile, const TileHead& h,
+                                              const ArbOut& o, int64_t b, int64_t qi,
+                                              const double* qp) {
+    using S = TileShape<BX, BY, BZ>;
+    double loc[3];
+    int64_t ia[3];
+    bool oob;
+    const int64_t flat = arb_reduce_axes<3>(qp, g, loc, ia, &oob);
+    int lc;
+    const int64_t bb = bucket_of<3>(flat, ia, g, B, &lc);
+    if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
+    const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
+    float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
+    hermite_weights((float)loc[0], wx, dx);
+    hermite_weights((float)loc[1], wy, dy);
+    hermite_weights((float)loc[2], wz, dz);
+    const float4* base = tile + lz * S::ZS + ly * S::YS + lx;
+    float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float4 sy = make_float4(0.f, 0.f, 0.f, 0.f);
+        float sdx = 0.f, sdy = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4* r = base + k * S::ZS + j * S::YS;
+            const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            float4 sx = make_float4(0.f, 0.f, 0.f, 0.f);
+            sx = fma4(wx[0], r0, sx);
+            sx = fma4(wx[1], r1, sx);
+            sx = fma4(wx[2], r2, sx);
+            sx = fma4(wx[3], r3, sx);
+            float ddx = dx[0] * r0.w;
+            ddx = fmaf(dx[1], r1.w, ddx);
+            ddx = fmaf(dx[2], r2.w, ddx);
+            ddx = fmaf(dx[3], r3.w, ddx);
+            sy = fma4(wy[j], sx, sy);
+            sdx = fmaf(wy[j], ddx, sdx);
+            sdy = fmaf(dy[j], sx.w, sdy);
+        }
+        val = fma4(wz[k], sy, val);
+        gx = fmaf(wz[k], sdx, gx);
+        gy = fmaf(wz[k], sdy, gy);
+        gz = fmaf(dz[k], sy.w, gz);
+    }
+    // the linear model at the query, in tile coordinates (cell lx spans
+    // nodes lx + 1 and lx + 2)
+    const double px = (double)(lx + 1 - h.m[0]) + loc[0];
+    const double py = (double)(ly + 1 - h.m[1]) + loc[1];
+    const double pz = (double)(lz + 1 - h.m[2]) + loc[2];
+    const float vs[4] = {val.x, val.y, val.z, val.w};
+    TO rec[8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) rec[k] = (TO)(tile_model(h, k, px, py, pz) + (double)vs[k]);
+    const double mx = fma(h.hxz, pz, fma(h.hxy, py, fma(h.hxx, px, h.gx[3])));
+    const double my = fma(h.hyz, pz, fma(h.hyy, py, fma(h.hxy, px, h.gy[3])));
+    const double mz = fma(h.hzz, pz, fma(h.hyz, py, fma(h.hxz, px, h.gz[3])));
+    rec[4] = (TO)((mx + (double)gx) / g.h[0]);
+    rec[5] = (TO)((my + (double)gy) / g.h[1]);
+    rec[6] = (TO)((mz + (double)gz) / g.h[2]);
+    rec[7] = (TO)0;
+    store_rec<3, TO>(rec, 1, 1, o, qi);
+}
+
+template <typename TO, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256)
+k_eval_tiles3(ArbGeom g, ArbBuckets B, const float4* __restrict__ tiles,
+              const TileHead* __restrict__ heads, const double* __restrict__ q, int64_t n,
+              int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
+              const int32_t* __restrict__ perm) {
+    using S = TileShape<BX, BY, BZ>;
+    __shared__ float4 tile[S::NZ * S::ZS];
+    __shared__ TileHead h;
+    const int64_t b = blockIdx.x;
+    const int32_t start = offs[b], end = offs[b + 1];
+    if (start == end) return;                       // uniform over the workgroup
+    int32_t s = start + threadIdx.x;
+    int64_t qi = -1;
+    double qp[3];
+    if (s < end) {
+        const int64_t pi = perm[s];
+        qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
+        const int64_t qq = qi >= 0 ? qi : 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
+    }
+    // stage the tile (read once per call: nontemporal) and its head
+    const v4u* src = reinterpret_cast<const v4u*>(tiles + b * S::NTP);
+    v4u* dst = reinterpret_cast<v4u*>(tile);
+    for (int e = threadIdx.x; e < S::NT; e += blockDim.x) {
+        const int x = e % S::NX, y = (e / S::NX) % S::NY, z = e / (S::NX * S::NY);
+        dst[z * S::ZS + y * S::YS + x] = __builtin_nontemporal_load(&src[e]);
+    }
+    if (threadIdx.x < (int)(sizeof(TileHead) / 8))
+        reinterpret_cast<double*>(&h)[threadIdx.x] =
+            reinterpret_cast<const double*>(heads + b)[threadIdx.x];
+    __syncthreads();
+    if (qi >= 0) eval_row_tile<TO, BX, BY, BZ>(g, B, tile, h, o, b, qi, qp);
+    for (s += blockDim.x; s < end; s += blockDim.x) {
+        const int64_t pi = perm[s];
+        if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
+#pragma unroll
+        for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
+        eval_row_tile<TO, BX, BY, BZ>(g, B, tile, h, o, b, pi, qp);
+    }
+}
+
 // Sorted order, coefficients read from the table in global memory.
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
@@ -713,6 +969,71 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
     ARB_FUSED_SHAPES(ARB_FUSED_CASE)
 #undef ARB_FUSED_CASE
 #undef L_FUSED
+    return hipErrorInvalidValue;
+}
+
+// Whether the node-tile path applies: 3D 'both' (four interpolants,
+// components and a gradient), an instantiated block shape with the local
+// cell index cx + bx (cy + by cz).
+bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t has_comps,
+                  int32_t has_norm) {
+    if (g.ndim != 3 || ninterp != 4 || !has_comps || !has_norm) return false;
+    for (int a = 0; a < 3; ++a)
+        if (g.n[a] < 4) return false;
+    const int bx = B.b[0], by = B.b[1], bz = B.b[2];
+    bool shape = false;
+#define ARB_FUSED_IS(X, Y, Z) shape |= (bx == X && by == Y && bz == Z);
+    ARB_FUSED_SHAPES(ARB_FUSED_IS)
+#undef ARB_FUSED_IS
+    return shape && B.lsh[0] == 0 && (1 << B.lsh[1]) == bx && (1 << B.lsh[2]) == bx * by;
+}
+
+// Bytes of the node tiles of B: records, then the heads (16-byte aligned).
+int64_t arb_tiles_bytes(const ArbBuckets& B, int64_t* head_offset) {
+    const int64_t ntp = ((int64_t)(B.b[0] + 3) * (B.b[1] + 3) * (B.b[2] + 3) + 15) / 16 * 16;
+    const int64_t rec = B.nbt * ntp * 16;
+    if (head_offset) *head_offset = rec;
+    return rec + B.nbt * (int64_t)sizeof(TileHead);
+}
+
+hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, const double* vals,
+                                  void* tiles, hipStream_t s) {
+    int64_t hoff = 0;
+    arb_tiles_bytes(B, &hoff);
+    float4* rec = (float4*)tiles;
+    TileHead* heads = (TileHead*)((char*)tiles + hoff);
+#define ARB_TILES_CASE(X, Y, Z)                                                               \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
+        hipLaunchKernelGGL((k_build_tiles3<X, Y, Z>), dim3((unsigned)B.nbt), dim3(256), 0, s, g, \
+                           B, vals, rec, heads);                                              \
+        return hipGetLastError();                                                             \
+    }
+    ARB_FUSED_SHAPES(ARB_TILES_CASE)
+#undef ARB_TILES_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t out_f64,
+                                 const void* tiles, const double* q, int64_t n, int64_t ldq,
+                                 const ArbOut& o, const int32_t* offs, const int32_t* perm,
+                                 int32_t block_threads, hipStream_t s) {
+    int64_t hoff = 0;
+    arb_tiles_bytes(B, &hoff);
+    const float4* rec = (const float4*)tiles;
+    const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
+    dim3 grid((unsigned)B.nbt), block(block_threads);
+#define L_NODES(TO, X, Y, Z)                                                                  \
+    hipLaunchKernelGGL((k_eval_tiles3<TO, X, Y, Z>), grid, block, 0, s, g, B, rec, heads, q, n, \
+                       ldq, o, offs, perm)
+#define ARB_NODES_CASE(X, Y, Z)                                                               \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
+        if (out_f64) L_NODES(double, X, Y, Z);                                                \
+        else L_NODES(float, X, Y, Z);                                                         \
+        return hipGetLastError();                                                             \
+    }
+    ARB_FUSED_SHAPES(ARB_NODES_CASE)
+#undef ARB_NODES_CASE
+#undef L_NODES
     return hipErrorInvalidValue;
 }
 

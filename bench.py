@@ -29,7 +29,9 @@ sys.path.insert(0, ROOT)
 METRIC = "value+grad queries/s, mixed-prec vs FP64 plain on 1 GPU at ≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
 # evaluation kernel of the headline step (mixed, 3D, FP64 output records)
-HEAD_KERNEL = "k_eval_bucket<3, float, float, double>"
+HEAD_KERNEL = "k_eval_tiles3<double, 4, 4, 2>"
+# bytes of one node-tile head (TileHead: 22 doubles and 4 int32)
+TILE_HEAD_BYTES = 192
 
 
 def parse():
@@ -44,7 +46,7 @@ def parse():
     ap.add_argument("--err-sample", type=int, default=200_000)
     ap.add_argument("--flags", type=int, default=0, help="extra ARB_Q_* flags")
     ap.add_argument("--variants", default="", help="comma list of query paths to time "
-                    "(auto,direct,sorted_lds,sorted_fused,sorted_cell,sorted_atomic,sorted_global); default all")
+                    "(auto,direct,sorted_lds,sorted_table,sorted_fused,sorted_cell,sorted_atomic,sorted_global); default all")
     ap.add_argument("--timing-steps", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
@@ -120,6 +122,7 @@ def main():
     from arbinterp_amd import _native as NN
     variants = {"auto": 0, "direct": NN.ARB_Q_DIRECT, "sorted_lds": NN.ARB_Q_SORTED,
                 "sorted_fused": NN.ARB_Q_SORTED | NN.ARB_Q_FUSED,
+                "sorted_table": NN.ARB_Q_SORTED | NN.ARB_Q_TABLE,
                 "sorted_cell": NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL,
                 "sorted_atomic": NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC,
                 "sorted_global": NN.ARB_Q_SORTED_GLOBAL}
@@ -160,11 +163,27 @@ def main():
                 per[key] = {"ms_per_step": ms, "stage_ms": [round(x, 5) for x in st.tolist()]}
                 log("%s %-20s: %.3f ms/step  %.3e q/s  stages %s" % (prec, key, ms,
                                                                      args.nq / ms * 1e3, per[key]["stage_ms"]))
-        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=recs["p64"])
+        # the headline path's own outputs, for the accuracy check (the timing
+        # loops above leave the last variant's rows in the buffers)
+        rec = t.query_packed(qd, out=recs["p64"], out_f64=True)
+        torch.cuda.synchronize()
+        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=rec)
     nc = res["mixed"]["obj"].nc
     cells = res["mixed"]["out"][3]
     valid = cells[cells < nc]
     U = int(torch.unique(valid).numel())
+    # distinct buckets (blocks of cells) the mixed path touched, and the
+    # bytes of one node tile: records of (b + 3) nodes per axis, 16 bytes each
+    info = res["mixed"]["obj"].field_info()
+    blk = [int(info.block[a]) for a in range(3)]
+    npos = [int(info.npos[a]) for a in range(3)]
+    cx = valid % npos[0]
+    cy = (valid // npos[0]) % npos[1]
+    cz = valid // (npos[0] * npos[1])
+    nbx, nby = -(-npos[0] // blk[0]), -(-npos[1] // blk[1])
+    bk = (cx // blk[0]) + nbx * ((cy // blk[1]) + nby * (cz // blk[2]))
+    UB = int(torch.unique(bk).numel())
+    tile_bytes = (blk[0] + 3) * (blk[1] + 3) * (blk[2] + 3) * 16 + TILE_HEAD_BYTES
 
     # accuracy of the mixed path against the FP64 oracle on a sample
     from oracle import arbinterp_oracle as O
@@ -187,15 +206,16 @@ def main():
     log("errors vs oracle:", json.dumps(err))
 
     # The step is timed on the library's default path ("auto").  The roofline
-    # figure is for the evaluation kernel of that path: algorithmic bytes
-    # N*(24 + 7*s_o) + U*4*64*s_c over its HIP-event duration.
+    # figure is for the evaluation kernel of that path over its HIP-event
+    # duration.  Algorithmic bytes: mixed (node tiles) N*(24 + 7*8) + UB*tile
+    # bytes; FP64 (coefficient table) N*(24 + 7*8) + U*4*64*8.
     N = args.nq
     head = "auto" if "auto" in res["mixed"]["per"] else next(iter(res["mixed"]["per"]))
     ms = res["mixed"]["per"][head]["ms_per_step"]
     ms64 = res["fp64"]["per"][head]["ms_per_step"]
     # algorithmic output bytes are the 7 values per row, 8 bytes each in the
     # FP64 record (the record's pad word is overhead of the layout, not counted)
-    alg_bytes = N * (24 + 7 * 8) + U * 4 * 64 * 4
+    alg_bytes = N * (24 + 7 * 8) + UB * tile_bytes
     alg64 = N * (24 + 7 * 8) + U * 4 * 64 * 8
     ev_ms = res["mixed"]["per"][head]["stage_ms"][3] or ms
     ev64 = res["fp64"]["per"][head]["stage_ms"][3] or ms64
@@ -236,12 +256,15 @@ def main():
         "config": {
             "workload": "tricubic dipole field %dx%dx%d, h=(1.0,0.8,1.25) mm, mode='both', "
                         "%d interior FP64 positions resident in HBM" % (n + (N,)),
-            "precision": "mixed (FP64 cell reduction, FP32 table and evaluation)",
+            "precision": "mixed (FP64 cell reduction and tile models, FP32 node-tile "
+                         "differences and evaluation)",
             "outputs": "comps, |B|, grad|B| per row as one 64-byte record of FP64 values "
                        "(query_packed(out_f64=True); FP32 arithmetic, exact widening); the "
                        "32-byte FP32 record and the separate-array layouts are timed under "
                        "paths.*_p32 and paths.*_arrays",
             "queries": N, "cells": nc, "distinct_cells_touched": U,
+            "bucket_block": blk, "distinct_buckets_touched": UB,
+            "node_tile_bytes": tile_bytes,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -87,6 +87,13 @@ extern "C" {
                                results are bitwise the same                  */
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
+#define ARB_Q_TABLE  2048   /* bucketed path: stage each block's coefficients
+                               from the table even where the node tiles are
+                               the default (mixed precision, 3D, mode 'both':
+                               each query is evaluated from its 4x4x4 node
+                               values, no table is built or read; results
+                               agree with the table path to FP32 rounding,
+                               not bitwise)                                     */
 
 /* ---- boundary extension (arb_field_set_boundary) ----------------------- */
 #define ARB_BC_NONE      0  /* the reference's box: [2nd node, 2nd-last node]   */
@@ -116,6 +123,10 @@ typedef struct arb_field_info {
     int64_t table_bytes;   /* bytes of the coefficient table                    */
     int32_t packed_width;  /* elements per row with ARB_Q_PACKED (4 or 8)       */
     int32_t boundary;      /* ARB_BC_* (arb_field_set_boundary)                 */
+    int32_t block[4];      /* cells per axis of one bucket of the bucketed
+                              path (0 before its first use or arb_set_tuning) */
+    int64_t tiles_bytes;   /* bytes of the node tiles (mixed 3D 'both'
+                              bucketed path; 0 until its first query)         */
 } arb_field_info;
 
 /* Analyse a field and upload it (tricubic/quadcubic constructor minus the

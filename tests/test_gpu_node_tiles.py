@@ -1,0 +1,163 @@
+"""Node-tile evaluation (mixed precision, 3D, mode 'both', bucketed path).
+
+Each query is evaluated from its cell's 4x4x4 node values with the Hermite
+weights w_o(u) = sum_e M[e][o] u^e; a bucket's nodes are staged as FP32
+differences from an FP64 polynomial model of each interpolant (linear for
+the components, quadratic for |B|), which is added back per query in FP64.
+Pinned here by:
+
+  * the oracle (FP64 restatement of the reference), normwise 1e-6 as for
+    every mixed result, on a dipole field;
+  * exact reproduction: a field whose |B| is a quadratic polynomial is
+    reproduced to FP64 rounding (value and gradient), because the weights
+    reproduce quadratics and the model carries them in FP64; a linear
+    component likewise;
+  * invalidation: new node values (arb_field_set_values) rebuild the tiles
+    (doubling every node doubles every output, bit for bit);
+  * launch shapes: thread counts do not change the bits; other block shapes
+    agree with the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from arbinterp_amd import workloads as W
+from oracle import arbinterp_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOLMIX = 1e-6
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    from arbinterp_amd import ARBInterp as G
+    return G
+
+
+def _nodes_query(t, q, out_f64=True, **kw):
+    """Bucketed default path (node tiles for mixed 3D 'both')."""
+    import torch
+    from arbinterp_amd import _native as NN
+    qd = torch.from_numpy(np.ascontiguousarray(q)).cuda()
+    out = t.query_device(qd, out_f64=out_f64, cells=True, flags=NN.ARB_Q_SORTED, **kw)
+    torch.cuda.synchronize()
+    return [x.double().cpu().numpy() for x in out]
+
+
+@pytest.mark.parametrize("out_f64", [True, False])
+def test_node_tiles_match_oracle(dev, out_f64):
+    rows, q = W.dipole3d(seed=2, n=(48, 40, 36), nq=150_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    c, nv, g, cells = _nodes_query(t, q, out_f64=out_f64)
+    info = t.field_info()
+    assert info.tiles_bytes > 0 and all(info.block[a] > 0 for a in range(3))
+    o = O.tricubic(rows.copy(), "quiet", mode="both")
+    idx = np.random.default_rng(3).choice(len(q), 30_000, replace=False)
+    co, no, go = o.Query(q[idx].copy())
+    assert np.array_equal(cells[idx], o.queryInds)
+    sc, sn, sg = o.condition_scale(q[idx])
+    for got, ref, s, name in ((c[idx], co, sc, "comps"), (nv[idx, None], no, sn, "norm"),
+                              (g[idx], go, sg, "grad")):
+        mx, rms = O.rel_err_normwise(got, ref, s)
+        print("node tiles %s (%s outputs): max %.3e rms %.3e"
+              % (name, "FP64" if out_f64 else "FP32", mx, rms))
+        assert mx <= TOLMIX, (name, mx, rms)
+    t.close()
+
+
+def _quadratic_field(n=(26, 22, 19), h=(0.11, 0.09, 0.13), origin=(0.2, -0.4, 0.3)):
+    axes = W.grid_axes(n, h, origin)
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+
+    def quad(x, y, z):
+        return (3.0 + 0.4 * x - 0.3 * y + 0.25 * z + 0.15 * x * x + 0.1 * y * y + 0.2 * z * z
+                + 0.12 * x * y - 0.08 * x * z + 0.05 * y * z)
+
+    def grad(x, y, z):
+        return np.stack([0.4 + 0.3 * x + 0.12 * y - 0.08 * z,
+                         -0.3 + 0.2 * y + 0.12 * x + 0.05 * z,
+                         0.25 + 0.4 * z - 0.08 * x + 0.05 * y], axis=1)
+
+    def lin(x, y, z):
+        return 1.5 - 0.7 * x + 0.9 * y + 0.35 * z
+
+    rows = W.grid_rows(axes, [quad(X, Y, Z), np.zeros_like(X), lin(X, Y, Z)],
+                       np.random.default_rng(8))
+    return rows, axes, quad, grad, lin
+
+
+def test_node_tiles_reproduce_quadratics(dev):
+    """Components: B = (q, 0, l) with q quadratic and l linear.  The linear
+    component is reproduced to FP64 rounding (the components' model is
+    linear), the zero component is exactly zero, and q keeps its quadratic
+    part in FP32 differences (a few 2^-24 of that part).  |B|: with l = 0,
+    |B| = q (q > 0 on the box), and value and gradient are reproduced to
+    FP64 rounding (|B|'s model is quadratic)."""
+    rows, axes, quad, grad, lin = _quadratic_field()
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    lo, hi = t._lo, t._hi
+    rng = np.random.default_rng(4)
+    q = lo + (hi - lo) * rng.uniform(0.0, 1.0 - 1e-9, size=(120_000, 3))
+    c, nv, g, cells = _nodes_query(t, q)
+    x, y, z = q[:, 0], q[:, 1], q[:, 2]
+    ref_l = lin(x, y, z)
+    assert np.max(np.abs(c[:, 2] - ref_l)) <= 1e-13 * np.max(np.abs(ref_l))
+    assert np.all(c[:, 1] == 0.0)
+    ref_q = quad(x, y, z)
+    assert np.max(np.abs(c[:, 0] - ref_q)) <= 1e-6 * np.max(np.abs(ref_q))
+    t.close()
+
+    rows2 = rows.copy()
+    rows2[:, 5] = 0.0
+    t = dev.tricubic(rows2, "quiet", mode="both", precision="mixed")
+    c, nv, g, cells = _nodes_query(t, q)
+    assert np.max(np.abs(nv - ref_q)) <= 1e-13 * np.max(np.abs(ref_q))
+    ref_g = grad(x, y, z)
+    assert np.max(np.abs(g - ref_g)) <= 1e-12 * np.max(np.abs(ref_g))
+    t.close()
+
+
+def test_node_tiles_follow_new_values(dev):
+    import torch
+    from arbinterp_amd import _native as NN
+    rows, q = W.dipole3d(seed=5, n=(30, 27, 25), nq=100_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    a = _nodes_query(t, q)
+    info = t.field_info()
+    vals = torch.empty((info.ninterp, info.nnodes), dtype=torch.float64, device="cuda")
+    lib = NN.lib()
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    NN.check(lib.arb_field_values(t._h, ctypes.c_void_p(vals.data_ptr()), stream), "values")
+    vals *= 2.0
+    NN.check(lib.arb_field_set_values(t._h, ctypes.c_void_p(vals.data_ptr()), stream), "set")
+    b = _nodes_query(t, q)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(np.nan_to_num(2.0 * x, nan=5.0), np.nan_to_num(y, nan=5.0))
+    assert np.array_equal(a[3], b[3])
+    t.close()
+
+
+def test_node_tiles_launch_shapes(dev):
+    rows, q = W.dipole3d(seed=2, n=(33, 29, 27), nq=120_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    o = O.tricubic(rows.copy(), "quiet", mode="both")
+    idx = np.random.default_rng(6).choice(len(q), 15_000, replace=False)
+    ref = o.Query(q[idx].copy())
+    scales = o.condition_scale(q[idx])
+    for block in ((4, 4, 2), (2, 2, 2), (4, 4, 4), (2, 2, 1)):
+        outs = []
+        for threads in (64, 256):
+            t.set_tuning(block, threads)
+            outs.append(_nodes_query(t, q))
+            assert tuple(t.field_info().block[a] for a in range(3)) == block
+        for x, y in zip(*outs):
+            assert np.array_equal(np.nan_to_num(x, nan=5.0), np.nan_to_num(y, nan=5.0)), block
+        c, nv, g, _ = outs[0]
+        for got, r, s in ((c[idx], ref[0], scales[0]), (nv[idx, None], ref[1], scales[1]),
+                          (g[idx], ref[2], scales[2])):
+            mx, _ = O.rel_err_normwise(got, r, s)
+            assert mx <= TOLMIX, (block, mx)
+    t.close()

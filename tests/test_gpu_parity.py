@@ -222,7 +222,10 @@ def test_query_paths_bitwise_identical(dev, case):
     """Direct, bucketed-LDS and bucketed-global paths give identical bits,
     including rows outside the box, NaN rows, IntMax rows, partial blocks.
     For mixed 3D the fused path forms coefficients in the query kernel and
-    the others read the table: the same bits are required."""
+    the others read the table: the same bits are required.  Mixed 3D 'both'
+    evaluates bucketed queries from node tiles by default: those rows agree
+    with the table paths to FP32 rounding (1e-6 normwise against the oracle's
+    condition scale), and are the same bits in either bucketed order."""
     import torch
     from arbinterp_amd import _native as NN
     rng = np.random.default_rng(5)
@@ -260,6 +263,7 @@ def test_query_paths_bitwise_identical(dev, case):
     outs = {}
     for name, fl in (("direct", NN.ARB_Q_DIRECT), ("lds", NN.ARB_Q_SORTED),
                      ("cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL),
+                     ("table", NN.ARB_Q_SORTED | NN.ARB_Q_TABLE),
                      ("fused", NN.ARB_Q_SORTED | NN.ARB_Q_FUSED),
                      ("atomic", NN.ARB_Q_SORTED | NN.ARB_Q_GROUP_ATOMIC),
                      ("global", NN.ARB_Q_SORTED_GLOBAL)):
@@ -272,12 +276,28 @@ def test_query_paths_bitwise_identical(dev, case):
                 x.fill_(-7 if x.dtype == torch.int64 else 3.0e30)
         outs[name] = t.query_device(qd.clone(), cells=True, flags=fl, out=buf)
     torch.cuda.synchronize()
-    for name in ("lds", "cell", "fused", "atomic", "global"):
+    node_tiles = case == "3d_both_mixed"
+    for name in ("lds", "cell", "table", "fused", "atomic", "global"):
+        if node_tiles and name in ("lds", "cell"):
+            continue
         for a, b in zip(outs["direct"], outs[name]):
             if a is None:
                 assert b is None
                 continue
             assert _nan_equal(a, b), (case, name)
+    if node_tiles:
+        assert t.field_info().tiles_bytes > 0
+        for a, b in zip(outs["lds"], outs["cell"]):
+            assert _nan_equal(a, b), (case, "cell")
+        assert torch.equal(outs["lds"][3], outs["direct"][3])
+        o = O.tricubic(rows, "quiet", mode="both")
+        scales = o.condition_scale(q)
+        for k, s in enumerate(scales):
+            a = outs["direct"][k].double().cpu().numpy().reshape(len(q), -1)
+            b = outs["lds"][k].double().cpu().numpy().reshape(len(q), -1)
+            assert np.array_equal(np.isnan(a), np.isnan(b)), (case, k)
+            mx, _ = O.rel_err_normwise(b, a, s.reshape(len(q), -1))
+            assert mx <= TOLMIX, (case, k, mx)
 
 
 def test_coefficient_save_load_roundtrip(dev, tmp_path):
