@@ -611,9 +611,38 @@ struct TileShape {
     static constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
     static constexpr int NT = NX * NY * NZ;
     static constexpr int NTP = (NT + 15) / 16 * 16;      // records per tile in HBM
-    // LDS strides in records: rows of 8, and a plane pad that puts the two
-    // z layers of a 4 x 4 x 2 block in different bank groups
-    static constexpr int YS = (NX + 7) / 8 * 8, ZS = NY * YS + 4;
+    // LDS strides in 16-byte records.  A lane reads record
+    // (lx + i) + YS (ly + j) + ZS (lz + k); LDS serves 16-byte reads from 16
+    // bank groups, so the strides are padded (by < 16 records each) to
+    // spread the block's cells evenly over the groups: the first (YS, ZS)
+    // with the smallest largest group.
+    static constexpr int load(int ys, int zs) {
+        int cnt[16] = {};
+        int mx = 0;
+        for (int z = 0; z < BZ; ++z)
+            for (int y = 0; y < BY; ++y)
+                for (int x = 0; x < BX; ++x) {
+                    const int gi = (x + ys * y + zs * z) & 15;
+                    ++cnt[gi];
+                    mx = cnt[gi] > mx ? cnt[gi] : mx;
+                }
+        return mx;
+    }
+    static constexpr int pick(bool want_z) {
+        int by = NX, bz = NX * NY, best = 1 << 30;
+        for (int py = 0; py < 16; ++py)
+            for (int pz = 0; pz < 16; ++pz) {
+                const int ys = NX + py, zs = NY * ys + pz;
+                const int l = load(ys, zs);
+                if (l < best) {
+                    best = l;
+                    by = ys;
+                    bz = zs;
+                }
+            }
+        return want_z ? bz : by;
+    }
+    static constexpr int YS = pick(false), ZS = pick(true);
 };
 
 // Tile origin (first cell = first node) of bucket b, per axis.
@@ -717,6 +746,10 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
     const int64_t bb = bucket_of<3>(flat, ia, g, B, &lc);
     if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
     const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
+    // compiler barriers around the contraction: they keep the tile head's
+    // LDS loads (22 doubles) from being hoisted out of the query loop and
+    // held in registers through it (121 -> 76 VGPRs, 4 -> 6 waves per SIMD)
+    asm volatile("" ::: "memory");
     float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
     hermite_weights((float)loc[0], wx, dx);
     hermite_weights((float)loc[1], wy, dy);
@@ -750,8 +783,9 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
         gy = fmaf(wz[k], sdy, gy);
         gz = fmaf(dz[k], sy.w, gz);
     }
-    // the linear model at the query, in tile coordinates (cell lx spans
-    // nodes lx + 1 and lx + 2)
+    asm volatile("" ::: "memory");
+    // the models at the query, in tile coordinates (cell lx spans nodes
+    // lx + 1 and lx + 2)
     const double px = (double)(lx + 1 - h.m[0]) + loc[0];
     const double py = (double)(ly + 1 - h.m[1]) + loc[1];
     const double pz = (double)(lz + 1 - h.m[2]) + loc[2];
@@ -972,6 +1006,11 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
     return hipErrorInvalidValue;
 }
 
+// Cell blocks the node-tile kernels are instantiated for.
+#define ARB_NODE_SHAPES(X)                                                                    \
+    X(8, 8, 4) X(8, 4, 4) X(4, 4, 4) X(8, 4, 2) X(4, 4, 2) X(4, 2, 2) X(2, 2, 2) X(2, 2, 1)    \
+    X(2, 1, 1) X(1, 1, 1)
+
 // Whether the node-tile path applies: 3D 'both' (four interpolants,
 // components and a gradient), an instantiated block shape with the local
 // cell index cx + bx (cy + by cz).
@@ -982,9 +1021,9 @@ bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_
         if (g.n[a] < 4) return false;
     const int bx = B.b[0], by = B.b[1], bz = B.b[2];
     bool shape = false;
-#define ARB_FUSED_IS(X, Y, Z) shape |= (bx == X && by == Y && bz == Z);
-    ARB_FUSED_SHAPES(ARB_FUSED_IS)
-#undef ARB_FUSED_IS
+#define ARB_NODE_IS(X, Y, Z) shape |= (bx == X && by == Y && bz == Z);
+    ARB_NODE_SHAPES(ARB_NODE_IS)
+#undef ARB_NODE_IS
     return shape && B.lsh[0] == 0 && (1 << B.lsh[1]) == bx && (1 << B.lsh[2]) == bx * by;
 }
 
@@ -1008,7 +1047,7 @@ hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, const d
                            B, vals, rec, heads);                                              \
         return hipGetLastError();                                                             \
     }
-    ARB_FUSED_SHAPES(ARB_TILES_CASE)
+    ARB_NODE_SHAPES(ARB_TILES_CASE)
 #undef ARB_TILES_CASE
     return hipErrorInvalidValue;
 }
@@ -1031,7 +1070,7 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
         else L_NODES(float, X, Y, Z);                                                         \
         return hipGetLastError();                                                             \
     }
-    ARB_FUSED_SHAPES(ARB_NODES_CASE)
+    ARB_NODE_SHAPES(ARB_NODES_CASE)
 #undef ARB_NODES_CASE
 #undef L_NODES
     return hipErrorInvalidValue;

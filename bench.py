@@ -4,8 +4,9 @@ Workload: 128^3 vector field of 6 point dipoles (h = 1.0, 0.8, 1.25 mm, N x 7
 rows), mode='both' (components, |B| and grad|B|), 10^7 query positions
 uniform in the interpolation interior, already resident in HBM.  One step =
 one batch query of all 10^7 positions.  The headline path is mixed precision
-(FP64 cell reduction and FD stencils, FP32 coefficient table and evaluation);
-the plain FP64 path is timed beside it.
+(FP64 cell reduction and per-block node models, FP32 node-tile differences
+and evaluation); the plain FP64 path (FP64 coefficient table) is timed
+beside it.
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is not supported: this
 build runs on one GPU.
@@ -103,7 +104,7 @@ def main():
         qd = torch.from_numpy(q).cuda()
         t = tricubic(rows, "quiet", mode="both", precision="mixed")
         t.allCoeffs()
-        rec = t.query_packed(qd, out_f64=True)
+        rec = t.query_packed(qd, flags=args.flags, out_f64=True)
         for _ in range(args.steps):
             t.query_packed(qd, out=rec, flags=args.flags, out_f64=True)
         torch.cuda.synchronize()

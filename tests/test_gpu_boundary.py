@@ -88,6 +88,28 @@ def test_boundary_3d_matches_oracle(dev, bc, mode, precision):
 
 
 @pytest.mark.parametrize("bc", ["neumann", "dirichlet"])
+def test_boundary_node_tiles(dev, bc):
+    """Mixed 'both' batches on the bucketed path are evaluated from node
+    tiles of the extended grid (ghost layers included)."""
+    import torch
+    from arbinterp_amd import _native as NN
+    rows, axes = _rows3(3)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", boundary=bc, precision="mixed")
+    o = O.extend_boundary(O.tricubic(rows.copy(), "quiet", mode="both"), bc)
+    lo, hi = np.array(o.p.int_min), np.array(o.p.int_max)
+    q = _queries(lo, hi, np.random.default_rng(27), 80_000)
+    qd = torch.from_numpy(q).cuda()
+    c, nv, g, cells = t.query_device(qd, out_f64=True, cells=True, flags=NN.ARB_Q_SORTED)
+    torch.cuda.synchronize()
+    assert t.field_info().tiles_bytes > 0
+    got = (c.cpu().numpy(), nv.cpu().numpy()[:, None], g.cpu().numpy())
+    ref = o.Query(q.copy())
+    _check(got, ref, o.condition_scale(q), "mixed", (bc, "tiles"))
+    assert np.array_equal(cells.cpu().numpy(), o.queryInds)
+    t.close()
+
+
+@pytest.mark.parametrize("bc", ["neumann", "dirichlet"])
 def test_boundary_4d_matches_oracle(dev, golden, bc):
     fld = golden("vector4d_modes")["field"]
     for kw in ({"mode": "both"}, {}):
