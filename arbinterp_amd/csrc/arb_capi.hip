@@ -622,8 +622,12 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     }
     // mixed 3D 'both', bucketed (unless the table is asked for): evaluation
     // from node tiles, no table
+    // (not on Neumann-extended fields: the table's FP64-formed slopes make the
+    // normal slope on their faces exactly zero, node tiles only to 2^-24 of
+    // the tile differences)
     const bool nodeform = !fused && !(flags & ARB_Q_TABLE) && sorted &&
                           f->precision == ARB_PREC_MIXED && !f->table_loaded &&
+                          f->bc != ARB_BC_NEUMANN &&
                           !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC)) &&
                           arb_nodes_ok(f->g, f->B, f->ninterp, f->has_comps, f->has_norm);
     const ArbBuckets& BB = fused ? f->BF : f->B;
