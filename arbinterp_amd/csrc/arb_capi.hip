@@ -620,11 +620,10 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (f->BF.nbt == 0) choose_buckets(f, f->BF, false);
         fused = arb_fused_lds(f->g, f->BF, f->ninterp, f->eval_threads) > 0;
     }
-    // mixed 3D 'both', bucketed (unless the table is asked for): evaluation
-    // from node tiles, no table
-    // (not on Neumann-extended fields: the table's FP64-formed slopes make the
-    // normal slope on their faces exactly zero, node tiles only to 2^-24 of
-    // the tile differences)
+    // mixed 3D, bucketed (unless the table is asked for): evaluation from
+    // node tiles, no table.  Not on Neumann-extended fields: the table's
+    // FP64-formed slopes make the normal slope on their faces exactly zero,
+    // node tiles only to 2^-24 of the tile differences.
     const bool nodeform = !fused && !(flags & ARB_Q_TABLE) && sorted &&
                           f->precision == ARB_PREC_MIXED && !f->table_loaded &&
                           f->bc != ARB_BC_NEUMANN &&
@@ -634,7 +633,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     int rc = (fused || nodeform) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (nodeform && !f->tiles_ok) {
-        const int64_t tb = arb_tiles_bytes(f->B, nullptr);
+        const int64_t tb = arb_tiles_bytes(f->B, f->ninterp, nullptr);
         if (tb != f->tiles_bytes) {
             if (f->tiles) (void)hipFree(f->tiles);
             f->tiles = nullptr;
@@ -642,7 +641,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(hipMalloc(&f->tiles, (size_t)tb));
             f->tiles_bytes = tb;
         }
-        HIPCHK(arb_launch_build_tiles(f->g, f->B, f->vals, f->tiles, s));
+        HIPCHK(arb_launch_build_tiles(f->g, f->B, f->ninterp, f->vals, f->tiles, s));
         f->tiles_ok = 1;
     }
 
@@ -718,7 +717,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
                                         perm, s));
         else if (nodeform)
-            HIPCHK(arb_launch_eval_nodes(f->g, BB, out64, f->tiles, q_dev, n, ldq, o,
+            HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, q_dev, n, ldq, o,
                                          f->ws_offs, perm, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,

@@ -562,8 +562,8 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 }
 
 // ---------------------------------------------------------------------------
-// Node form (mixed precision, 3D, mode 'both').  Since the coefficients are
-// (M x M x M) F, the interpolant of a cell is
+// Node form (mixed precision, 3D).  Since the coefficients are (M x M x M) F,
+// the interpolant of a cell is
 //   p(u, v, w) = sum_ijk wx_i(u) wy_j(v) wz_k(w) F(cx + i, cy + j, cz + k),
 // w_o(u) = sum_e M[e][o] u^e (the cubic Hermite weights with central-
 // difference slopes), so a query can be evaluated from its 4 x 4 x 4 nodes.
@@ -571,58 +571,111 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 // Node tiles.  Per bucket, the block's node tile (cells + 3 per axis, x
 // fastest) is kept as FP32 differences from a polynomial model of each
 // interpolant around the tile's middle node m, with p = node - m:
-//   components: L(p) = F(m) + G . p,
-//   |B| (the interpolant whose gradient is output): Q(p) = L(p) + p^T H p / 2,
-// G and H the central first and second differences at m (FP64), together
-// with the models (TileHead).  Central differences are exact for quadratics,
-// so the weights reproduce every polynomial of degree <= 2 per axis
-// (sum_o w_o = 1, sum_o w_o (o - 1) = u, sum_o w_o (o - 1)^2 = u^2, and the
-// same for w' with 0, 1, 2u), and
-//   p = Q(query) + sum w (F - Q),  grad p = grad Q + sum w' (F - Q):
+//   L(p) = F(m) + G . p for all but the last interpolant,
+//   Q(p) = L(p) + p^T H p / 2 for the last one (|B| in mode 'both', the last
+//   component in 'vector'), and C(p) = Q(p) + T[p, p, p] / 6 for the one
+//   interpolant of scalar fields and mode 'norm' (whose values cross zero:
+//   the per-value error bound needs the smaller differences),
+// G, H and T the central first, second and third differences at m (FP64),
+// together with the models (TileHead).  Central differences are exact for
+// quadratics, so the weights reproduce every polynomial of degree <= 2 per
+// axis (sum_o w_o = 1, sum_o w_o (o - 1) = u, sum_o w_o (o - 1)^2 = u^2,
+// and the same for w' with 0, 1, 2u); of the cubic terms only x^3, y^3, z^3
+// are not, and their interpolant on a unit cell is x^3 + phi(u) with
+// phi(u) = u (u - 1) (2u - 1) (slopes 3x^2 + 1 from central differences).
+// So
+//   p = C(query) + sum_a (T_aaa / 6) phi(u_a) + sum w (F - C),
+//   grad p = grad C + (T_aaa / 6) phi'(u_a) + sum w' (F - C):
 // the model is evaluated in FP64 and only the differences, whose size is set
-// by the field's third derivatives across the tile (second for the
-// components), carry FP32 rounding.  (Rounding the node values themselves
-// to FP32 would leave an error of order 2^-24 |F| in the gradient, since the
-// gradient weights sum to zero.)  Each tile record holds all four
-// interpolants (16 bytes).
+// by the field's fourth derivatives across the tile (third for Q, second
+// for L), carry FP32 rounding.  (The cubic terms cost 'both' 50 registers
+// and 0.09 ms per 10^7 queries, for no gain its bound needs.)  (Rounding the node values themselves to
+// FP32 would leave an error of order 2^-24 |F| in the gradient, since the
+// gradient weights sum to zero.)  A tile record holds the NI interpolants of
+// one node: 16 bytes for NI = 4 ('both') and NI = 3 ('vector', fourth lane
+// 0), 4 bytes for NI = 1 (scalar fields, 'norm').
 // ---------------------------------------------------------------------------
 struct TileHead {
     double f[4];             // F(m) per interpolant
     double gx[4], gy[4], gz[4];   // central differences at m, per node step
-    double hxx, hyy, hzz, hxy, hxz, hyz;   // second differences of interpolant 3
+    double hxx, hyy, hzz, hxy, hxz, hyz;   // second differences, last interpolant
+    double txxx, tyyy, tzzz, txxy, txxz, txyy, tyyz, txzz, tyzz, txyz;   // third
     int32_t m[3];            // m in tile coordinates
     int32_t pad;
 };
 
-// The model of interpolant k at p (tile coordinates relative to m).
+// Whether the last interpolant's model has the cubic terms.
+template <int NI>
+constexpr bool tile_cubic() { return NI == 1; }
+
+// The model of interpolant k (of NI) at p (tile coordinates relative to m).
+template <int NI>
 __device__ __forceinline__ double tile_model(const TileHead& h, int k, double px, double py,
                                             double pz) {
     double v = fma(h.gz[k], pz, fma(h.gy[k], py, fma(h.gx[k], px, h.f[k])));
-    if (k == 3) {
+    if (k == NI - 1) {
         const double q = fma(0.5 * h.hzz, pz * pz,
                              fma(0.5 * h.hyy, py * py, 0.5 * h.hxx * (px * px)));
         v += fma(h.hyz, py * pz, fma(h.hxz, px * pz, fma(h.hxy, px * py, q)));
     }
+    if (tile_cubic<NI>() && k == NI - 1) {
+        const double px2 = px * px, py2 = py * py, pz2 = pz * pz;
+        double c = (h.txxx * px2 * px + h.tyyy * py2 * py + h.tzzz * pz2 * pz) * (1.0 / 6.0);
+        c = fma(0.5, h.txxy * px2 * py + h.txxz * px2 * pz + h.txyy * px * py2 +
+                         h.tyyz * py2 * pz + h.txzz * px * pz2 + h.tyzz * py * pz2, c);
+        v += fma(h.txyz, px * py * pz, c);
+    }
     return v;
 }
 
-template <int BX, int BY, int BZ>
+// Gradient (per node step) of the last interpolant's model at p.
+template <int NI>
+__device__ __forceinline__ void tile_model_grad(const TileHead& h, int k, double px, double py,
+                                                double pz, double* gm) {
+    gm[0] = fma(h.hxz, pz, fma(h.hxy, py, fma(h.hxx, px, h.gx[k])));
+    gm[1] = fma(h.hyz, pz, fma(h.hyy, py, fma(h.hxy, px, h.gy[k])));
+    gm[2] = fma(h.hzz, pz, fma(h.hyz, py, fma(h.hxz, px, h.gz[k])));
+    if constexpr (tile_cubic<NI>()) {
+        const double px2 = px * px, py2 = py * py, pz2 = pz * pz;
+        gm[0] += 0.5 * (h.txxx * px2 + h.txyy * py2 + h.txzz * pz2) + h.txxy * px * py +
+                 h.txxz * px * pz + h.txyz * py * pz;
+        gm[1] += 0.5 * (h.tyyy * py2 + h.txxy * px2 + h.tyzz * pz2) + h.txyy * px * py +
+                 h.tyyz * py * pz + h.txyz * px * pz;
+        gm[2] += 0.5 * (h.tzzz * pz2 + h.txxz * px2 + h.tyyz * py2) + h.txzz * px * pz +
+                 h.tyzz * py * pz + h.txyz * px * py;
+    }
+}
+
+template <int NI>
+struct TileRec {
+    using type = float4;
+    using bits = v4u;        // same bytes, for nontemporal loads
+};
+template <>
+struct TileRec<1> {
+    using type = float;
+    using bits = uint32_t;
+};
+
+template <int NI, int BX, int BY, int BZ>
 struct TileShape {
     static constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
     static constexpr int NT = NX * NY * NZ;
     static constexpr int NTP = (NT + 15) / 16 * 16;      // records per tile in HBM
-    // LDS strides in 16-byte records.  A lane reads record
-    // (lx + i) + YS (ly + j) + ZS (lz + k); LDS serves 16-byte reads from 16
-    // bank groups, so the strides are padded (by < 16 records each) to
+    // LDS strides in records.  A lane reads record
+    // (lx + i) + YS (ly + j) + ZS (lz + k); LDS serves a wave's reads from
+    // 64 banks of 4 bytes, i.e. 16 groups for 16-byte records and 64 for
+    // 4-byte ones, so the strides are padded (by < 16 records each) to
     // spread the block's cells evenly over the groups: the first (YS, ZS)
     // with the smallest largest group.
+    static constexpr int G = NI == 1 ? 64 : 16;
     static constexpr int load(int ys, int zs) {
-        int cnt[16] = {};
+        int cnt[64] = {};
         int mx = 0;
         for (int z = 0; z < BZ; ++z)
             for (int y = 0; y < BY; ++y)
                 for (int x = 0; x < BX; ++x) {
-                    const int gi = (x + ys * y + zs * z) & 15;
+                    const int gi = (x + ys * y + zs * z) & (G - 1);
                     ++cnt[gi];
                     mx = cnt[gi] > mx ? cnt[gi] : mx;
                 }
@@ -656,11 +709,11 @@ __device__ __forceinline__ void bucket_origin3(int64_t b, const ArbBuckets& B, i
 }
 
 // One workgroup per bucket: the head, then every record of the tile.
-template <int BX, int BY, int BZ>
+template <int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
-               float4* __restrict__ tiles, TileHead* __restrict__ heads) {
-    using S = TileShape<BX, BY, BZ>;
+               typename TileRec<NI>::type* __restrict__ tiles, TileHead* __restrict__ heads) {
+    using S = TileShape<NI, BX, BY, BZ>;
     __shared__ TileHead h;
     const int64_t b = blockIdx.x;
     int64_t c0[3];
@@ -673,24 +726,60 @@ k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
         for (int a = 0; a < 3; ++a) {
             // middle node, kept one node inside the grid so that the central
             // differences exist
+            // (two nodes inside where the axis has 5 nodes or more, for the
+            // third differences along it)
+            const int64_t in = g.n[a] >= 5 ? 2 : 1;
             m[a] = c0[a] + nd[a];
-            m[a] = m[a] < 1 ? 1 : (m[a] > g.n[a] - 2 ? g.n[a] - 2 : m[a]);
+            m[a] = m[a] < in ? in : (m[a] > g.n[a] - 1 - in ? g.n[a] - 1 - in : m[a]);
         }
-        const double* v = vals + k * g.nnodes;
         const int64_t mi = m[0] + g.nstride[1] * m[1] + g.nstride[2] * m[2];
-        const double f = v[mi];
-        h.f[k] = f;
-        h.gx[k] = 0.5 * (v[mi + 1] - v[mi - 1]);
-        h.gy[k] = 0.5 * (v[mi + g.nstride[1]] - v[mi - g.nstride[1]]);
-        h.gz[k] = 0.5 * (v[mi + g.nstride[2]] - v[mi - g.nstride[2]]);
-        if (k == 3) {
-            const int64_t sx = 1, sy = g.nstride[1], sz = g.nstride[2];
-            h.hxx = v[mi + sx] - 2.0 * f + v[mi - sx];
-            h.hyy = v[mi + sy] - 2.0 * f + v[mi - sy];
-            h.hzz = v[mi + sz] - 2.0 * f + v[mi - sz];
-            h.hxy = 0.25 * ((v[mi + sx + sy] - v[mi + sx - sy]) - (v[mi - sx + sy] - v[mi - sx - sy]));
-            h.hxz = 0.25 * ((v[mi + sx + sz] - v[mi + sx - sz]) - (v[mi - sx + sz] - v[mi - sx - sz]));
-            h.hyz = 0.25 * ((v[mi + sy + sz] - v[mi + sy - sz]) - (v[mi - sy + sz] - v[mi - sy - sz]));
+        if (k < NI) {
+            const double* v = vals + k * g.nnodes;
+            const double f = v[mi];
+            h.f[k] = f;
+            h.gx[k] = 0.5 * (v[mi + 1] - v[mi - 1]);
+            h.gy[k] = 0.5 * (v[mi + g.nstride[1]] - v[mi - g.nstride[1]]);
+            h.gz[k] = 0.5 * (v[mi + g.nstride[2]] - v[mi - g.nstride[2]]);
+            if (k == NI - 1) {
+                const int64_t sx = 1, sy = g.nstride[1], sz = g.nstride[2];
+                h.hxx = v[mi + sx] - 2.0 * f + v[mi - sx];
+                h.hyy = v[mi + sy] - 2.0 * f + v[mi - sy];
+                h.hzz = v[mi + sz] - 2.0 * f + v[mi - sz];
+                h.hxy = 0.25 * ((v[mi + sx + sy] - v[mi + sx - sy]) - (v[mi - sx + sy] - v[mi - sx - sy]));
+                h.hxz = 0.25 * ((v[mi + sx + sz] - v[mi + sx - sz]) - (v[mi - sx + sz] - v[mi - sx - sz]));
+                h.hyz = 0.25 * ((v[mi + sy + sz] - v[mi + sy - sz]) - (v[mi - sy + sz] - v[mi - sy - sz]));
+                // third differences: along one axis (5 nodes; 0 where the axis
+                // has only 4), and the central differences of the second ones
+                auto d3 = [&](int64_t st, int a) {
+                    return g.n[a] >= 5 ? 0.5 * ((v[mi + 2 * st] - v[mi - 2 * st]) -
+                                                2.0 * (v[mi + st] - v[mi - st]))
+                                       : 0.0;
+                };
+                auto d2d1 = [&](int64_t s2, int64_t s1) {   // d/d(s1) of d2/d(s2)^2
+                    return 0.5 * ((v[mi + s2 + s1] - 2.0 * v[mi + s1] + v[mi - s2 + s1]) -
+                                  (v[mi + s2 - s1] - 2.0 * v[mi - s1] + v[mi - s2 - s1]));
+                };
+                if constexpr (tile_cubic<NI>()) {
+                    h.txxx = d3(sx, 0);
+                    h.tyyy = d3(sy, 1);
+                    h.tzzz = d3(sz, 2);
+                    h.txxy = d2d1(sx, sy);
+                    h.txxz = d2d1(sx, sz);
+                    h.txyy = d2d1(sy, sx);
+                    h.tyyz = d2d1(sy, sz);
+                    h.txzz = d2d1(sz, sx);
+                    h.tyzz = d2d1(sz, sy);
+                    h.txyz = 0.125 * ((v[mi + sx + sy + sz] - v[mi + sx + sy - sz]) -
+                                      (v[mi + sx - sy + sz] - v[mi + sx - sy - sz]) -
+                                      (v[mi - sx + sy + sz] - v[mi - sx + sy - sz]) +
+                                      (v[mi - sx - sy + sz] - v[mi - sx - sy - sz]));
+                } else {
+                    h.txxx = h.tyyy = h.tzzz = h.txxy = h.txxz = 0.0;
+                    h.txyy = h.tyyz = h.txzz = h.tyzz = h.txyz = 0.0;
+                }
+            }
+        } else {
+            h.f[k] = h.gx[k] = h.gy[k] = h.gz[k] = 0.0;
         }
         if (k == 0) {
 #pragma unroll
@@ -708,11 +797,12 @@ k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
         const bool in = e < S::NT && nx < g.n[0] && ny < g.n[1] && nz < g.n[2];
         const int64_t src = in ? nx + g.nstride[1] * ny + g.nstride[2] * nz : 0;
         const double px = x - h.m[0], py = y - h.m[1], pz = z - h.m[2];
-        float d[4];
+        float d[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            d[k] = in ? (float)(vals[k * g.nnodes + src] - tile_model(h, k, px, py, pz)) : 0.f;
-        tiles[b * S::NTP + e] = make_float4(d[0], d[1], d[2], d[3]);
+        for (int k = 0; k < NI; ++k)
+            d[k] = in ? (float)(vals[k * g.nnodes + src] - tile_model<NI>(h, k, px, py, pz)) : 0.f;
+        if constexpr (NI == 1) tiles[b * S::NTP + e] = d[0];
+        else tiles[b * S::NTP + e] = make_float4(d[0], d[1], d[2], d[3]);
     }
 }
 
@@ -730,14 +820,21 @@ __device__ __forceinline__ void hermite_weights(float u, float* w, float* dw) {
 __device__ __forceinline__ float4 fma4(float a, float4 x, float4 y) {
     return make_float4(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y), fmaf(a, x.z, y.z), fmaf(a, x.w, y.w));
 }
+__device__ __forceinline__ float fma4(float a, float x, float y) { return fmaf(a, x, y); }
+__device__ __forceinline__ float last_of(float4 x) { return x.w; }
+__device__ __forceinline__ float last_of(float x) { return x; }
 
-// Evaluate row qi (position qp) of bucket b from the node tile in LDS.
-template <typename TO, int BX, int BY, int BZ>
+// Evaluate row qi (position qp) of bucket b from the node tile in LDS.  The
+// gradient is of the last interpolant (GRAD: modes with a value and
+// gradient output).
+template <typename TO, int NI, int BX, int BY, int BZ>
 __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets& B,
-                                              const float4* tile, const TileHead& h,
-                                              const ArbOut& o, int64_t b, int64_t qi,
-                                              const double* qp) {
-    using S = TileShape<BX, BY, BZ>;
+                                              const typename TileRec<NI>::type* tile,
+                                              const TileHead& h, const ArbOut& o, int64_t b,
+                                              int64_t qi, const double* qp) {
+    using S = TileShape<NI, BX, BY, BZ>;
+    using R = typename TileRec<NI>::type;
+    constexpr bool GRAD = NI != 3;
     double loc[3];
     int64_t ia[3];
     bool oob;
@@ -747,41 +844,45 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
     if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
     const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
     // compiler barriers around the contraction: they keep the tile head's
-    // LDS loads (22 doubles) from being hoisted out of the query loop and
+    // LDS loads (32 doubles) from being hoisted out of the query loop and
     // held in registers through it (121 -> 76 VGPRs, 4 -> 6 waves per SIMD)
     asm volatile("" ::: "memory");
     float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
     hermite_weights((float)loc[0], wx, dx);
     hermite_weights((float)loc[1], wy, dy);
     hermite_weights((float)loc[2], wz, dz);
-    const float4* base = tile + lz * S::ZS + ly * S::YS + lx;
-    float4 val = make_float4(0.f, 0.f, 0.f, 0.f);
+    const R* base = tile + lz * S::ZS + ly * S::YS + lx;
+    R val{};
     float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float4 sy = make_float4(0.f, 0.f, 0.f, 0.f);
+        R sy{};
         float sdx = 0.f, sdy = 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float4* r = base + k * S::ZS + j * S::YS;
-            const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-            float4 sx = make_float4(0.f, 0.f, 0.f, 0.f);
+            const R* r = base + k * S::ZS + j * S::YS;
+            const R r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            R sx{};
             sx = fma4(wx[0], r0, sx);
             sx = fma4(wx[1], r1, sx);
             sx = fma4(wx[2], r2, sx);
             sx = fma4(wx[3], r3, sx);
-            float ddx = dx[0] * r0.w;
-            ddx = fmaf(dx[1], r1.w, ddx);
-            ddx = fmaf(dx[2], r2.w, ddx);
-            ddx = fmaf(dx[3], r3.w, ddx);
             sy = fma4(wy[j], sx, sy);
-            sdx = fmaf(wy[j], ddx, sdx);
-            sdy = fmaf(dy[j], sx.w, sdy);
+            if constexpr (GRAD) {
+                float ddx = dx[0] * last_of(r0);
+                ddx = fmaf(dx[1], last_of(r1), ddx);
+                ddx = fmaf(dx[2], last_of(r2), ddx);
+                ddx = fmaf(dx[3], last_of(r3), ddx);
+                sdx = fmaf(wy[j], ddx, sdx);
+                sdy = fmaf(dy[j], last_of(sx), sdy);
+            }
         }
         val = fma4(wz[k], sy, val);
-        gx = fmaf(wz[k], sdx, gx);
-        gy = fmaf(wz[k], sdy, gy);
-        gz = fmaf(dz[k], sy.w, gz);
+        if constexpr (GRAD) {
+            gx = fmaf(wz[k], sdx, gx);
+            gy = fmaf(wz[k], sdy, gy);
+            gz = fmaf(dz[k], last_of(sy), gz);
+        }
     }
     asm volatile("" ::: "memory");
     // the models at the query, in tile coordinates (cell lx spans nodes
@@ -789,28 +890,56 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
     const double px = (double)(lx + 1 - h.m[0]) + loc[0];
     const double py = (double)(ly + 1 - h.m[1]) + loc[1];
     const double pz = (double)(lz + 1 - h.m[2]) + loc[2];
-    const float vs[4] = {val.x, val.y, val.z, val.w};
     TO rec[8];
+    float vs[4];
+    if constexpr (NI == 1) {
+        vs[0] = val;
+    } else {
+        vs[0] = val.x;
+        vs[1] = val.y;
+        vs[2] = val.z;
+        vs[3] = val.w;
+    }
+    // the interpolant of the last model's x^3, y^3, z^3 terms: phi(u) more
+    // than the terms themselves, phi(u) = u (u - 1) (2u - 1)
+    constexpr int L = NI - 1;
+    double corr = 0.0;
+    if constexpr (tile_cubic<NI>()) {
+        const double phix = loc[0] * (loc[0] - 1.0) * (2.0 * loc[0] - 1.0);
+        const double phiy = loc[1] * (loc[1] - 1.0) * (2.0 * loc[1] - 1.0);
+        const double phiz = loc[2] * (loc[2] - 1.0) * (2.0 * loc[2] - 1.0);
+        corr = (h.txxx * phix + h.tyyy * phiy + h.tzzz * phiz) * (1.0 / 6.0);
+    }
 #pragma unroll
-    for (int k = 0; k < 4; ++k) rec[k] = (TO)(tile_model(h, k, px, py, pz) + (double)vs[k]);
-    const double mx = fma(h.hxz, pz, fma(h.hxy, py, fma(h.hxx, px, h.gx[3])));
-    const double my = fma(h.hyz, pz, fma(h.hyy, py, fma(h.hxy, px, h.gy[3])));
-    const double mz = fma(h.hzz, pz, fma(h.hyz, py, fma(h.hxz, px, h.gz[3])));
-    rec[4] = (TO)((mx + (double)gx) / g.h[0]);
-    rec[5] = (TO)((my + (double)gy) / g.h[1]);
-    rec[6] = (TO)((mz + (double)gz) / g.h[2]);
-    rec[7] = (TO)0;
-    store_rec<3, TO>(rec, 1, 1, o, qi);
+    for (int k = 0; k < NI; ++k)
+        rec[k] = (TO)(tile_model<NI>(h, k, px, py, pz) + (k == L ? corr : 0.0) + (double)vs[k]);
+    if constexpr (GRAD) {
+        double gm[3];
+        tile_model_grad<NI>(h, L, px, py, pz, gm);
+        if constexpr (tile_cubic<NI>()) {
+            gm[0] += h.txxx * (1.0 / 6.0) * fma(6.0 * loc[0], loc[0] - 1.0, 1.0);
+            gm[1] += h.tyyy * (1.0 / 6.0) * fma(6.0 * loc[1], loc[1] - 1.0, 1.0);
+            gm[2] += h.tzzz * (1.0 / 6.0) * fma(6.0 * loc[2], loc[2] - 1.0, 1.0);
+        }
+        rec[NI] = (TO)((gm[0] + (double)gx) / g.h[0]);
+        rec[NI + 1] = (TO)((gm[1] + (double)gy) / g.h[1]);
+        rec[NI + 2] = (TO)((gm[2] + (double)gz) / g.h[2]);
+    }
+#pragma unroll
+    for (int r = GRAD ? NI + 3 : NI; r < 8; ++r) rec[r] = (TO)0;
+    // record layout [comps(3)] [value, grad(3)] (value first without comps)
+    store_rec<3, TO>(rec, NI != 1, GRAD, o, qi);
 }
 
-template <typename TO, int BX, int BY, int BZ>
+template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
-k_eval_tiles3(ArbGeom g, ArbBuckets B, const float4* __restrict__ tiles,
+k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
               const TileHead* __restrict__ heads, const double* __restrict__ q, int64_t n,
               int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
               const int32_t* __restrict__ perm) {
-    using S = TileShape<BX, BY, BZ>;
-    __shared__ float4 tile[S::NZ * S::ZS];
+    using S = TileShape<NI, BX, BY, BZ>;
+    using RB = typename TileRec<NI>::bits;
+    __shared__ typename TileRec<NI>::type tile[S::NZ * S::ZS];
     __shared__ TileHead h;
     const int64_t b = blockIdx.x;
     const int32_t start = offs[b], end = offs[b + 1];
@@ -826,8 +955,8 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const float4* __restrict__ tiles,
         for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
     }
     // stage the tile (read once per call: nontemporal) and its head
-    const v4u* src = reinterpret_cast<const v4u*>(tiles + b * S::NTP);
-    v4u* dst = reinterpret_cast<v4u*>(tile);
+    const RB* src = reinterpret_cast<const RB*>(tiles + b * S::NTP);
+    RB* dst = reinterpret_cast<RB*>(tile);
     for (int e = threadIdx.x; e < S::NT; e += blockDim.x) {
         const int x = e % S::NX, y = (e / S::NX) % S::NY, z = e / (S::NX * S::NY);
         dst[z * S::ZS + y * S::YS + x] = __builtin_nontemporal_load(&src[e]);
@@ -836,13 +965,13 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const float4* __restrict__ tiles,
         reinterpret_cast<double*>(&h)[threadIdx.x] =
             reinterpret_cast<const double*>(heads + b)[threadIdx.x];
     __syncthreads();
-    if (qi >= 0) eval_row_tile<TO, BX, BY, BZ>(g, B, tile, h, o, b, qi, qp);
+    if (qi >= 0) eval_row_tile<TO, NI, BX, BY, BZ>(g, B, tile, h, o, b, qi, qp);
     for (s += blockDim.x; s < end; s += blockDim.x) {
         const int64_t pi = perm[s];
         if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
 #pragma unroll
         for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
-        eval_row_tile<TO, BX, BY, BZ>(g, B, tile, h, o, b, pi, qp);
+        eval_row_tile<TO, NI, BX, BY, BZ>(g, B, tile, h, o, b, pi, qp);
     }
 }
 
@@ -1011,12 +1140,17 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
     X(8, 8, 4) X(8, 4, 4) X(4, 4, 4) X(8, 4, 2) X(4, 4, 2) X(4, 2, 2) X(2, 2, 2) X(2, 2, 1)    \
     X(2, 1, 1) X(1, 1, 1)
 
-// Whether the node-tile path applies: 3D 'both' (four interpolants,
-// components and a gradient), an instantiated block shape with the local
-// cell index cx + bx (cy + by cz).
+// Whether the node-tile path applies: 3D, mode 'both' (four interpolants,
+// components and a gradient), 'vector' (three, components) or scalar /
+// 'norm' (one, value and gradient); an instantiated block shape with the
+// local cell index cx + bx (cy + by cz).
 bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t has_comps,
                   int32_t has_norm) {
-    if (g.ndim != 3 || ninterp != 4 || !has_comps || !has_norm) return false;
+    if (g.ndim != 3) return false;
+    const bool mode_ok = (ninterp == 4 && has_comps && has_norm) ||
+                         (ninterp == 3 && has_comps && !has_norm) ||
+                         (ninterp == 1 && !has_comps && has_norm);
+    if (!mode_ok) return false;
     for (int a = 0; a < 3; ++a)
         if (g.n[a] < 4) return false;
     const int bx = B.b[0], by = B.b[1], bz = B.b[2];
@@ -1027,51 +1161,61 @@ bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_
     return shape && B.lsh[0] == 0 && (1 << B.lsh[1]) == bx && (1 << B.lsh[2]) == bx * by;
 }
 
-// Bytes of the node tiles of B: records, then the heads (16-byte aligned).
-int64_t arb_tiles_bytes(const ArbBuckets& B, int64_t* head_offset) {
+// Bytes of the node tiles of B for ninterp interpolants: records, then the
+// heads (16-byte aligned).
+int64_t arb_tiles_bytes(const ArbBuckets& B, int32_t ninterp, int64_t* head_offset) {
     const int64_t ntp = ((int64_t)(B.b[0] + 3) * (B.b[1] + 3) * (B.b[2] + 3) + 15) / 16 * 16;
-    const int64_t rec = B.nbt * ntp * 16;
+    const int64_t rec = B.nbt * ntp * (ninterp == 1 ? 4 : 16);
     if (head_offset) *head_offset = rec;
     return rec + B.nbt * (int64_t)sizeof(TileHead);
 }
 
-hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, const double* vals,
-                                  void* tiles, hipStream_t s) {
+hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                  const double* vals, void* tiles, hipStream_t s) {
     int64_t hoff = 0;
-    arb_tiles_bytes(B, &hoff);
-    float4* rec = (float4*)tiles;
+    arb_tiles_bytes(B, ninterp, &hoff);
     TileHead* heads = (TileHead*)((char*)tiles + hoff);
+#define L_TILES(NI, X, Y, Z)                                                                  \
+    hipLaunchKernelGGL((k_build_tiles3<NI, X, Y, Z>), dim3((unsigned)B.nbt), dim3(256), 0, s, g, \
+                       B, vals, (typename TileRec<NI>::type*)tiles, heads)
 #define ARB_TILES_CASE(X, Y, Z)                                                               \
     if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
-        hipLaunchKernelGGL((k_build_tiles3<X, Y, Z>), dim3((unsigned)B.nbt), dim3(256), 0, s, g, \
-                           B, vals, rec, heads);                                              \
+        if (ninterp == 4) L_TILES(4, X, Y, Z);                                                \
+        else if (ninterp == 3) L_TILES(3, X, Y, Z);                                           \
+        else L_TILES(1, X, Y, Z);                                                             \
         return hipGetLastError();                                                             \
     }
     ARB_NODE_SHAPES(ARB_TILES_CASE)
 #undef ARB_TILES_CASE
+#undef L_TILES
     return hipErrorInvalidValue;
 }
 
-hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t out_f64,
-                                 const void* tiles, const double* q, int64_t n, int64_t ldq,
-                                 const ArbOut& o, const int32_t* offs, const int32_t* perm,
-                                 int32_t block_threads, hipStream_t s) {
+hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                 int32_t out_f64, const void* tiles, const double* q, int64_t n,
+                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                 const int32_t* perm, int32_t block_threads, hipStream_t s) {
     int64_t hoff = 0;
-    arb_tiles_bytes(B, &hoff);
-    const float4* rec = (const float4*)tiles;
+    arb_tiles_bytes(B, ninterp, &hoff);
     const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
     dim3 grid((unsigned)B.nbt), block(block_threads);
-#define L_NODES(TO, X, Y, Z)                                                                  \
-    hipLaunchKernelGGL((k_eval_tiles3<TO, X, Y, Z>), grid, block, 0, s, g, B, rec, heads, q, n, \
-                       ldq, o, offs, perm)
+#define L_NODES(TO, NI, X, Y, Z)                                                              \
+    hipLaunchKernelGGL((k_eval_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,             \
+                       (const typename TileRec<NI>::type*)tiles, heads, q, n, ldq, o, offs,   \
+                       perm)
+#define L_NODES_NI(TO, X, Y, Z)                                                               \
+    if (ninterp == 4) L_NODES(TO, 4, X, Y, Z);                                                \
+    else if (ninterp == 3) L_NODES(TO, 3, X, Y, Z);                                           \
+    else L_NODES(TO, 1, X, Y, Z)
 #define ARB_NODES_CASE(X, Y, Z)                                                               \
     if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
-        if (out_f64) L_NODES(double, X, Y, Z);                                                \
-        else L_NODES(float, X, Y, Z);                                                         \
+        if (out_f64) L_NODES_NI(double, X, Y, Z);                                             \
+        else L_NODES_NI(float, X, Y, Z);                                                      \
         return hipGetLastError();                                                             \
     }
     ARB_NODE_SHAPES(ARB_NODES_CASE)
 #undef ARB_NODES_CASE
+#undef L_NODES_NI
 #undef L_NODES
     return hipErrorInvalidValue;
 }

@@ -222,8 +222,8 @@ def test_query_paths_bitwise_identical(dev, case):
     """Direct, bucketed-LDS and bucketed-global paths give identical bits,
     including rows outside the box, NaN rows, IntMax rows, partial blocks.
     For mixed 3D the fused path forms coefficients in the query kernel and
-    the others read the table: the same bits are required.  Mixed 3D 'both'
-    evaluates bucketed queries from node tiles by default: those rows agree
+    the others read the table: the same bits are required.  Mixed 3D fields
+    evaluate bucketed queries from node tiles by default: those rows agree
     with the table paths to FP32 rounding (1e-6 normwise against the oracle's
     condition scale), and are the same bits in either bucketed order."""
     import torch
@@ -276,7 +276,10 @@ def test_query_paths_bitwise_identical(dev, case):
                 x.fill_(-7 if x.dtype == torch.int64 else 3.0e30)
         outs[name] = t.query_device(qd.clone(), cells=True, flags=fl, out=buf)
     torch.cuda.synchronize()
-    node_tiles = case == "3d_both_mixed"
+    # mixed 3D fields evaluate bucketed queries from node tiles where the
+    # block shape has a node-tile kernel (not for the (2, 2, 8) tuning)
+    node_tiles = t.field_info().tiles_bytes > 0
+    assert node_tiles == (case in ("3d_both_mixed", "3d_scalar_mixed"))
     for name in ("lds", "cell", "table", "fused", "atomic", "global"):
         if node_tiles and name in ("lds", "cell"):
             continue
@@ -288,16 +291,41 @@ def test_query_paths_bitwise_identical(dev, case):
     if node_tiles:
         assert t.field_info().tiles_bytes > 0
         for a, b in zip(outs["lds"], outs["cell"]):
+            if a is None:
+                assert b is None
+                continue
             assert _nan_equal(a, b), (case, "cell")
         assert torch.equal(outs["lds"][3], outs["direct"][3])
-        o = O.tricubic(rows, "quiet", mode="both")
-        scales = o.condition_scale(q)
-        for k, s in enumerate(scales):
-            a = outs["direct"][k].double().cpu().numpy().reshape(len(q), -1)
-            b = outs["lds"][k].double().cpu().numpy().reshape(len(q), -1)
-            assert np.array_equal(np.isnan(a), np.isnan(b)), (case, k)
-            mx, _ = O.rel_err_normwise(b, a, s.reshape(len(q), -1))
-            assert mx <= TOLMIX, (case, k, mx)
+        # the same NaN rows as the table paths, and the oracle's values on
+        # every row inside the box: within 2e-7 of the condition scale (the
+        # FP32 bound), and under the floored metric within 1e-6 or the
+        # table path's own error, whichever is larger (near a zero of a scalar
+        # that varies fast across a cell neither FP32 form holds 1e-6 per
+        # value: table 2.5e-5, node tiles 3.3e-6 on this 23 x 19 x 14 grid)
+        for a, b in zip(outs["direct"][:3], outs["lds"][:3]):
+            if a is not None:
+                assert torch.equal(torch.isnan(a), torch.isnan(b)), case
+        cells = outs["direct"][3].cpu().numpy()
+        rows_in = np.where((cells < t.nc) & ~np.isnan(q).any(axis=1))[0]
+        o = O.tricubic(rows, "quiet", **kw)
+        ref = o.Query(q[rows_in].copy())
+        ref = ref if isinstance(ref, tuple) else (ref,)
+        scales = o.condition_scale(q[rows_in])
+        scales = scales if isinstance(scales, tuple) else (scales,)
+        got = [b for b in outs["lds"][:3] if b is not None]
+        tab = [b for b in outs["table"][:3] if b is not None]
+        assert len(got) == len(tab) == len(ref) == len(scales)
+        m = len(rows_in)
+        for k, (b, bt, r, s) in enumerate(zip(got, tab, ref, scales)):
+            b = b.double().cpu().numpy().reshape(len(q), -1)[rows_in]
+            bt = bt.double().cpu().numpy().reshape(len(q), -1)[rows_in]
+            r, s = r.reshape(m, -1), s.reshape(m, -1)
+            mx, _ = O.rel_err_normwise(b, r, s)
+            mt, _ = O.rel_err_normwise(bt, r, s)
+            ms, _ = O.rel_err_normwise(b, r, s, floor=1.0)
+            print("%s node tiles k=%d: %.3e (table %.3e), relative to S %.3e" % (case, k, mx, mt, ms))
+            assert ms <= 2e-7, (case, k, ms)
+            assert mx <= max(TOLMIX, mt), (case, k, mx, mt)
 
 
 def test_coefficient_save_load_roundtrip(dev, tmp_path):
