@@ -30,9 +30,9 @@ sys.path.insert(0, ROOT)
 METRIC = "value+grad queries/s, mixed-prec vs FP64 plain on 1 GPU at ≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
 # evaluation kernel of the headline step (mixed, 3D, FP64 output records)
-HEAD_KERNEL = "k_eval_tiles3<double, 4, 4, 2>"
-# bytes of one node-tile head (TileHead: 22 doubles and 4 int32)
-TILE_HEAD_BYTES = 192
+HEAD_KERNEL = "k_eval_tiles3<double, 4, 4, 4, 2>"
+# bytes of one node-tile head (TileHead: 32 doubles and 4 int32)
+TILE_HEAD_BYTES = 272
 
 
 def parse():

@@ -89,11 +89,10 @@ extern "C" {
                                cell inside the bucket (default: bucket only)    */
 #define ARB_Q_TABLE  2048   /* bucketed path: stage each block's coefficients
                                from the table even where the node tiles are
-                               the default (mixed precision, 3D, mode 'both':
-                               each query is evaluated from its 4x4x4 node
-                               values, no table is built or read; results
-                               agree with the table path to FP32 rounding,
-                               not bitwise)                                     */
+                               the default (mixed precision, 3D: each query is
+                               evaluated from its 4x4x4 node values, no table
+                               is built or read; results agree with the table
+                               path to FP32 rounding, not bitwise)            */
 
 /* ---- boundary extension (arb_field_set_boundary) ----------------------- */
 #define ARB_BC_NONE      0  /* the reference's box: [2nd node, 2nd-last node]   */
@@ -125,8 +124,8 @@ typedef struct arb_field_info {
     int32_t boundary;      /* ARB_BC_* (arb_field_set_boundary)                 */
     int32_t block[4];      /* cells per axis of one bucket of the bucketed
                               path (0 before its first use or arb_set_tuning) */
-    int64_t tiles_bytes;   /* bytes of the node tiles (mixed 3D 'both'
-                              bucketed path; 0 until its first query)         */
+    int64_t tiles_bytes;   /* bytes of the node tiles (mixed 3D bucketed
+                              path; 0 until its first query)                  */
 } arb_field_info;
 
 /* Analyse a field and upload it (tricubic/quadcubic constructor minus the
