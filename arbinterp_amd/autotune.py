@@ -10,12 +10,13 @@ on the caller's own positions (GPU, HIP events on the current stream):
     threads per workgroup.
 
 Outputs that must not depend on the launch shape are compared bit for bit:
-every table-staged candidate against the direct kernel, and the default
-bucketed candidates of one block against each other.  Where the default
-bucketed kernel is the node-tile one (mixed precision, 3D, mode 'both'),
-its FP32 rounding depends on the block (each block has its own tile
-models), so across blocks and against the direct kernel it is compared
-within a tolerance (`_close`).  A failed comparison raises.  The fastest
+every table-staged candidate against the first one, and the default
+candidates of one block against each other.  Where the default kernels are
+the node-tile ones (mixed precision, 3D), their FP32 rounding depends on
+the block (each block has its own tile models) and differs from the
+table's, so across blocks and families the outputs are compared with the
+direct kernel's within a tolerance (`_close`).  A failed comparison
+raises.  The fastest
 candidate is applied (`interp.set_tuning`, `interp.default_flags`) and
 remembered per (grid, mode, precision, query-count octave) for the process.
 
@@ -123,7 +124,7 @@ def tune_query(interp, q, reps=3, max_candidates=None, verify=True, use_cache=Tr
     if max_candidates:
         cands = cands[:max_candidates]
     table = []
-    first = {}                       # block -> outputs of its first default candidate
+    first = {}        # "table" / block -> outputs of its first candidate
     for path, b, th in cands:
         if path == "direct":
             interp.set_tuning(None, 0)
@@ -134,13 +135,14 @@ def tune_query(interp, q, reps=3, max_candidates=None, verify=True, use_cache=Tr
         fn = lambda: interp.query_device(q, flags=fl, out=out)
         ms = _time(fn, reps)
         if verify:
-            if path == "sorted" and b in first:
-                ok = _same(out, first[b])
-            elif path == "sorted":
-                ok = _close(out, ref)
-                first[b] = tuple(t.clone() if t is not None else None for t in out)
-            else:
+            key = "table" if path == "table" else b
+            if path == "direct":
                 ok = _same(out, ref)
+            elif key in first:
+                ok = _same(out, first[key])
+            else:
+                ok = _close(out, ref)
+                first[key] = tuple(t.clone() if t is not None else None for t in out)
             if not ok:
                 raise RuntimeError("autotune: launch shape %s %s %d changed the results"
                                    % (path, b, th))

@@ -594,6 +594,35 @@ int arb_build_coeffs(arb_field* f, void* stream) {
     return ARB_OK;
 }
 
+// Whether queries of the field are evaluated from node tiles: mixed 3D,
+// unless the table is asked for or has been loaded.  Not on Neumann-extended
+// fields: the table's FP64-formed slopes make the normal slope on their faces
+// exactly zero, node tiles only to 2^-24 of the tile differences.  Chooses
+// the bucket geometry if needed.
+static bool node_path(arb_field* f, int32_t flags) {
+    if (f->precision != ARB_PREC_MIXED || f->g.ndim != 3 || f->table_loaded ||
+        f->bc == ARB_BC_NEUMANN || (flags & ARB_Q_TABLE))
+        return false;
+    if (f->B.nbt == 0) choose_buckets(f, f->B, true);
+    return arb_nodes_ok(f->g, f->B, f->ninterp, f->has_comps, f->has_norm);
+}
+
+// The node tiles of the current node values and bucket geometry.
+static int ensure_tiles(arb_field* f, hipStream_t s) {
+    if (f->tiles_ok) return ARB_OK;
+    const int64_t tb = arb_tiles_bytes(f->B, f->ninterp, nullptr);
+    if (tb != f->tiles_bytes) {
+        if (f->tiles) (void)hipFree(f->tiles);
+        f->tiles = nullptr;
+        f->tiles_bytes = 0;
+        HIPCHK(hipMalloc(&f->tiles, (size_t)tb));
+        f->tiles_bytes = tb;
+    }
+    HIPCHK(arb_launch_build_tiles(f->g, f->B, f->ninterp, f->vals, f->tiles, s));
+    f->tiles_ok = 1;
+    return ARB_OK;
+}
+
 int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
               void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
               void* stream) {
@@ -620,29 +649,17 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (f->BF.nbt == 0) choose_buckets(f, f->BF, false);
         fused = arb_fused_lds(f->g, f->BF, f->ninterp, f->eval_threads) > 0;
     }
-    // mixed 3D, bucketed (unless the table is asked for): evaluation from
-    // node tiles, no table.  Not on Neumann-extended fields: the table's
-    // FP64-formed slopes make the normal slope on their faces exactly zero,
-    // node tiles only to 2^-24 of the tile differences.
-    const bool nodeform = !fused && !(flags & ARB_Q_TABLE) && sorted &&
-                          f->precision == ARB_PREC_MIXED && !f->table_loaded &&
-                          f->bc != ARB_BC_NEUMANN &&
-                          !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC)) &&
-                          arb_nodes_ok(f->g, f->B, f->ninterp, f->has_comps, f->has_norm);
+    // node tiles (mixed 3D): the direct path, and the bucketed path unless
+    // another grouping or the fused kernel is asked for; no table
+    const bool nodes = !fused && node_path(f, flags) &&
+                       !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC));
+    const bool nodeform = nodes && sorted;
     const ArbBuckets& BB = fused ? f->BF : f->B;
-    int rc = (fused || nodeform) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
+    int rc = (fused || nodes) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
-    if (nodeform && !f->tiles_ok) {
-        const int64_t tb = arb_tiles_bytes(f->B, f->ninterp, nullptr);
-        if (tb != f->tiles_bytes) {
-            if (f->tiles) (void)hipFree(f->tiles);
-            f->tiles = nullptr;
-            f->tiles_bytes = 0;
-            HIPCHK(hipMalloc(&f->tiles, (size_t)tb));
-            f->tiles_bytes = tb;
-        }
-        HIPCHK(arb_launch_build_tiles(f->g, f->B, f->ninterp, f->vals, f->tiles, s));
-        f->tiles_ok = 1;
+    if (nodes) {
+        rc = ensure_tiles(f, s);
+        if (rc) return rc;
     }
 
     if (n == 0) return ARB_OK;
@@ -665,8 +682,12 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     for (float& m : f->last_ms) m = 0.f;
     if (!sorted) {
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
-        HIPCHK(arb_launch_query_direct(f->g, f->precision, out64, f->table, f->ninterp,
-                                       f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
+        if (nodes)
+            HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, out64, f->tiles, f->table,
+                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
+        else
+            HIPCHK(arb_launch_query_direct(f->g, f->precision, out64, f->table, f->ninterp,
+                                           f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
         if (timing) {
             HIPCHK(hipEventRecord(f->ev[4], s));
             HIPCHK(hipEventSynchronize(f->ev[4]));
@@ -755,8 +776,13 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     if (f->has_norm && (!norm || !grad))
         return fail(ARB_EINVAL, "arb_query_point: norm/grad is NULL");
     hipStream_t s = (hipStream_t)stream;
-    int rc = arb_build_coeffs(f, stream);
+    const bool nodes = node_path(f, 0);
+    int rc = nodes ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
+    if (nodes) {
+        rc = ensure_tiles(f, s);
+        if (rc) return rc;
+    }
     if (!f->pin) {
         void* h = nullptr;
         HIPCHK(hipHostMalloc(&h, sizeof(double) * PIN_DOUBLES, hipHostMallocDefault));
@@ -777,8 +803,12 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     o.cells = reinterpret_cast<int64_t*>(f->pin_dev + PIN_CELL);
     o.ldq = d;
     o.flag = reinterpret_cast<int32_t*>(f->pin_dev + PIN_FLAG);
-    HIPCHK(arb_launch_query_direct(f->g, f->precision, 1, f->table, f->ninterp, f->has_comps,
-                                   f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
+    if (nodes)
+        HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, 1, f->tiles, f->table, f->has_comps,
+                                      f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
+    else
+        HIPCHK(arb_launch_query_direct(f->g, f->precision, 1, f->table, f->ninterp, f->has_comps,
+                                       f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
     HIPCHK(hipStreamSynchronize(s));
     if (f->has_comps)
         for (int c = 0; c < 3; ++c) comps[c] = h[PIN_COMPS + c];

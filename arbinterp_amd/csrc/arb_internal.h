@@ -278,6 +278,10 @@ bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_
 int64_t arb_tiles_bytes(const ArbBuckets& B, int32_t ninterp, int64_t* head_offset);
 hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                   const double* vals, void* tiles, hipStream_t s);
+hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                  int32_t out_f64, const void* tiles, const void* table,
+                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                  int64_t ldq, const ArbOut& o, hipStream_t s);
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                  int32_t out_f64, const void* tiles, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,

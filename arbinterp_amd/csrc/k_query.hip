@@ -824,24 +824,19 @@ __device__ __forceinline__ float fma4(float a, float x, float y) { return fmaf(a
 __device__ __forceinline__ float last_of(float4 x) { return x.w; }
 __device__ __forceinline__ float last_of(float x) { return x; }
 
-// Evaluate row qi (position qp) of bucket b from the node tile in LDS.  The
-// gradient is of the last interpolant (GRAD: modes with a value and
-// gradient output).
-template <typename TO, int NI, int BX, int BY, int BZ>
-__device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets& B,
-                                              const typename TileRec<NI>::type* tile,
-                                              const TileHead& h, const ArbOut& o, int64_t b,
-                                              int64_t qi, const double* qp) {
-    using S = TileShape<NI, BX, BY, BZ>;
+// Evaluate row qi at local cell lc of a block and local coordinates loc
+// from the block's node tile (records with row stride YS and plane stride
+// ZS, in LDS or in global memory) and its head.  The gradient is of the last
+// interpolant (GRAD: modes with a value and gradient output).  The same
+// operations in the same order wherever the tile lives, so every node-tile
+// path gives the same bits.
+template <typename TO, int NI, int BX, int BY, int BZ, int YS, int ZS>
+__device__ __forceinline__ void tile_eval(const ArbGeom& g, const ArbBuckets& B,
+                                          const typename TileRec<NI>::type* tile,
+                                          const TileHead& h, const ArbOut& o, int lc,
+                                          const double* loc, int64_t qi) {
     using R = typename TileRec<NI>::type;
     constexpr bool GRAD = NI != 3;
-    double loc[3];
-    int64_t ia[3];
-    bool oob;
-    const int64_t flat = arb_reduce_axes<3>(qp, g, loc, ia, &oob);
-    int lc;
-    const int64_t bb = bucket_of<3>(flat, ia, g, B, &lc);
-    if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
     const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
     // compiler barriers around the contraction: they keep the tile head's
     // LDS loads (32 doubles) from being hoisted out of the query loop and
@@ -851,7 +846,7 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
     hermite_weights((float)loc[0], wx, dx);
     hermite_weights((float)loc[1], wy, dy);
     hermite_weights((float)loc[2], wz, dz);
-    const R* base = tile + lz * S::ZS + ly * S::YS + lx;
+    const R* base = tile + lz * ZS + ly * YS + lx;
     R val{};
     float gx = 0.f, gy = 0.f, gz = 0.f;
 #pragma unroll
@@ -860,7 +855,7 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
         float sdx = 0.f, sdy = 0.f;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const R* r = base + k * S::ZS + j * S::YS;
+            const R* r = base + k * ZS + j * YS;
             const R r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
             R sx{};
             sx = fma4(wx[0], r0, sx);
@@ -929,6 +924,57 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
     for (int r = GRAD ? NI + 3 : NI; r < 8; ++r) rec[r] = (TO)0;
     // record layout [comps(3)] [value, grad(3)] (value first without comps)
     store_rec<3, TO>(rec, NI != 1, GRAD, o, qi);
+}
+
+
+// Evaluate row qi (position qp) of bucket b from the node tile in LDS.
+template <typename TO, int NI, int BX, int BY, int BZ>
+__device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets& B,
+                                              const typename TileRec<NI>::type* tile,
+                                              const TileHead& h, const ArbOut& o, int64_t b,
+                                              int64_t qi, const double* qp) {
+    using S = TileShape<NI, BX, BY, BZ>;
+    double loc[3];
+    int64_t ia[3];
+    bool oob;
+    const int64_t flat = arb_reduce_axes<3>(qp, g, loc, ia, &oob);
+    int lc;
+    const int64_t bb = bucket_of<3>(flat, ia, g, B, &lc);
+    if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
+    tile_eval<TO, NI, BX, BY, BZ, S::YS, S::ZS>(g, B, tile, h, o, lc, loc, qi);
+}
+
+// Lane-per-query node-tile kernel (small batches and single points): each
+// lane reads its cell's 64 records and its block's head from the tiles in
+// global memory.  Rows outside the box, or past the sentinel, are evaluated
+// from the table's sentinel row, as everywhere.
+template <typename TO, int NI, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256)
+k_query_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
+               const TileHead* __restrict__ heads, const float* __restrict__ table,
+               int32_t ninterp, int32_t has_comps, int32_t has_norm,
+               const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o) {
+    using S = TileShape<NI, BX, BY, BZ>;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[3];
+    int64_t ia[3];
+    bool oob;
+    const int64_t flat = arb_reduce_axes<3>(q + i * ldq, g, loc, ia, &oob);
+    if (o.cells) o.cells[i] = flat;
+    if (oob && o.qmut) {
+        for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
+    }
+    if (flat >= g.nc) {
+        if (flat > g.nc) o.flag[0] = 1;
+        eval_point<3, float, float, TO>(table + g.nc * (int64_t)ninterp * 64, has_comps, has_norm,
+                                        loc, g, o, i);
+        return;
+    }
+    int lc;
+    const int64_t b = bucket_of<3>(flat, ia, g, B, &lc);
+    tile_eval<TO, NI, BX, BY, BZ, S::NX, S::NX * S::NY>(g, B, tiles + b * S::NTP, heads[b], o,
+                                                        lc, loc, i);
 }
 
 template <typename TO, int NI, int BX, int BY, int BZ>
@@ -1217,6 +1263,36 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
 #undef ARB_NODES_CASE
 #undef L_NODES_NI
 #undef L_NODES
+    return hipErrorInvalidValue;
+}
+
+hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                  int32_t out_f64, const void* tiles, const void* table,
+                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                  int64_t ldq, const ArbOut& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    int64_t hoff = 0;
+    arb_tiles_bytes(B, ninterp, &hoff);
+    const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
+    dim3 grid(grid1(n, 256)), block(256);
+#define L_QT(TO, NI, X, Y, Z)                                                                 \
+    hipLaunchKernelGGL((k_query_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,            \
+                       (const typename TileRec<NI>::type*)tiles, heads, (const float*)table,  \
+                       ninterp, has_comps, has_norm, q, n, ldq, o)
+#define L_QT_NI(TO, X, Y, Z)                                                                  \
+    if (ninterp == 4) L_QT(TO, 4, X, Y, Z);                                                   \
+    else if (ninterp == 3) L_QT(TO, 3, X, Y, Z);                                              \
+    else L_QT(TO, 1, X, Y, Z)
+#define ARB_QT_CASE(X, Y, Z)                                                                  \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
+        if (out_f64) L_QT_NI(double, X, Y, Z);                                                \
+        else L_QT_NI(float, X, Y, Z);                                                         \
+        return hipGetLastError();                                                             \
+    }
+    ARB_NODE_SHAPES(ARB_QT_CASE)
+#undef ARB_QT_CASE
+#undef L_QT_NI
+#undef L_QT
     return hipErrorInvalidValue;
 }
 

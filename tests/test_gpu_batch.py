@@ -40,7 +40,10 @@ def test_batch_matches_single_launches(mods, precision):
     _, nb, gb = bt.query(qd, counts)
     off = np.concatenate([[0], np.cumsum(counts)])
     for k, t in enumerate(interps):
-        _, n1, g1 = t.query_device(qd[off[k]:off[k + 1]].contiguous(), flags=NN.ARB_Q_DIRECT)
+        # the batch kernel reads the coefficient tables: the same bits as the
+        # per-field table path (mixed 3D fields default to node tiles)
+        _, n1, g1 = t.query_device(qd[off[k]:off[k + 1]].contiguous(),
+                                   flags=NN.ARB_Q_DIRECT | NN.ARB_Q_TABLE)
         assert _eq(nb[off[k]:off[k + 1]], n1) and _eq(gb[off[k]:off[k + 1]], g1), k
 
 
