@@ -87,12 +87,13 @@ extern "C" {
                                results are bitwise the same                  */
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
-#define ARB_Q_TABLE  2048   /* bucketed path: stage each block's coefficients
-                               from the table even where the node tiles are
-                               the default (mixed precision, 3D: each query is
-                               evaluated from its 4x4x4 node values, no table
-                               is built or read; results agree with the table
-                               path to FP32 rounding, not bitwise)            */
+#define ARB_Q_TABLE  2048   /* read each cell's coefficients from the table
+                               even where the node tiles are the default
+                               (mixed precision, 3D, direct and bucketed
+                               paths: each query is evaluated from its 4x4x4
+                               node values, no table is built or read; results
+                               agree with the table path to FP32 rounding,
+                               not bitwise)                                   */
 
 /* ---- boundary extension (arb_field_set_boundary) ----------------------- */
 #define ARB_BC_NONE      0  /* the reference's box: [2nd node, 2nd-last node]   */
