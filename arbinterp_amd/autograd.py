@@ -44,7 +44,11 @@ class InterpFunction(torch.autograd.Function):
         N.check(N.lib().arb_field_set_values(interp._h, _ptr(vals), _stream_handle(torch)),
                 "set_values")
         interp._info.table_built = 0
-        _, U, G = interp.query_device(q, out_f64=True)
+        # the backward reads the coefficient table, so the forward reads it
+        # too (ARB_Q_TABLE; mixed 3D queries otherwise use node tiles): one
+        # build per set of values, and the gradients are exactly those of the
+        # values returned
+        _, U, G = interp.query_device(q, out_f64=True, flags=N.ARB_Q_TABLE)
         ctx.save_for_backward(q)
         ctx.interp = interp
         ctx.pos_dtype = positions.dtype
