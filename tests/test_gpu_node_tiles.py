@@ -195,3 +195,48 @@ def test_node_tiles_launch_shapes(dev):
             mx, _ = O.rel_err_normwise(got, r, s)
             assert mx <= TOLMIX, (block, mx)
     t.close()
+
+
+def test_node_tiles_side_outputs(dev):
+    """Rows outside the box, NaN rows and IntMax rows on the node-tile paths:
+    the caller's rows are NaN-filled as the reference does, cell indices are
+    the oracle's, the direct and bucketed node-tile paths give the same bits,
+    and a single point past the sentinel raises IndexError."""
+    import torch
+    from arbinterp_amd import _native as NN
+    rows, q = W.dipole3d(seed=7, n=(26, 23, 21), nq=20_000)
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    lo, hi = np.array(t._lo), np.array(t._hi)
+    rng = np.random.default_rng(12)
+    extra = rng.uniform(lo - 0.3 * (hi - lo), hi + 0.3 * (hi - lo), size=(2000, 3))
+    edge = np.tile((lo + hi) / 2, (4, 1))
+    edge[0, 0] = hi[0]
+    edge[1, 1] = hi[1]
+    edge[2, 0] = lo[0]
+    q = np.concatenate([q, extra, edge, np.full((2, 3), np.nan)])
+    q = q[rng.permutation(len(q))]
+    o = O.tricubic(rows.copy(), "quiet", mode="both")
+    qa = q.copy()
+    ref = o.Query(qa)                       # NaN-fills rows outside the box
+    outs = {}
+    for name, fl in (("direct", NN.ARB_Q_DIRECT), ("sorted", NN.ARB_Q_SORTED),
+                     ("cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL)):
+        qd = torch.from_numpy(np.concatenate([q, np.ones((len(q), 1))], axis=1)).cuda()
+        c, nv, g, cells = t.query_device(qd, nan_rows=True, cells=True, out_f64=True, flags=fl)
+        torch.cuda.synchronize()
+        qm = qd.cpu().numpy()
+        assert np.array_equal(np.isnan(qm[:, :3]), np.isnan(qa)), name
+        oob = np.isnan(qa).any(axis=1) & ~np.isnan(q).any(axis=1)
+        assert oob.sum() > 100 and np.all(np.isnan(qm[oob, 3])), name   # whole rows
+        assert np.array_equal(cells.cpu().numpy(), o.queryInds), name
+        outs[name] = [x.cpu().numpy() for x in (c, nv, g)]
+    assert t.field_info().tiles_bytes > 0
+    for name in ("sorted", "cell"):
+        for a, b in zip(outs["direct"], outs[name]):
+            assert np.array_equal(np.nan_to_num(a, nan=9.0), np.nan_to_num(b, nan=9.0)), name
+    for a, r in zip(outs["direct"], ref):
+        assert np.array_equal(np.isnan(a.reshape(len(q), -1)).any(axis=1),
+                              np.isnan(r.reshape(len(q), -1)).any(axis=1))
+    with pytest.raises(IndexError):           # the far corner, as a single point
+        t.Query(hi.copy())
+    t.close()
