@@ -240,3 +240,37 @@ def test_node_tiles_side_outputs(dev):
     with pytest.raises(IndexError):           # the far corner, as a single point
         t.Query(hi.copy())
     t.close()
+
+
+@pytest.mark.parametrize("mode", ["both", "scalar"])
+def test_node_tiles_tiny_grids(dev, mode):
+    """Grids of 4 and 5 nodes per axis (one and two cells): the tile model's
+    middle node is clamped into the grid and the third differences along an
+    axis of 4 nodes are left out; results still match the oracle."""
+    rng = np.random.default_rng(31)
+    for n in ((4, 5, 6), (5, 4, 4), (4, 4, 4)):
+        axes = W.grid_axes(n, (0.3, 0.25, 0.2), (0.1, -0.2, 0.4))
+        X, Y, Z = np.meshgrid(*axes, indexing="ij")
+        vals = [np.sin(2.0 * X + 0.5) * np.cos(1.5 * Y) + 0.3 * Z,
+                np.cos(X - Y) + 0.2, 0.4 * X * Y - np.sin(Z)]
+        if mode == "scalar":
+            vals = vals[:1]
+        rows = W.grid_rows(axes, vals, rng)
+        kw = {} if mode == "scalar" else {"mode": "both"}
+        t = dev.tricubic(rows.copy(), "quiet", precision="mixed", **kw)
+        o = O.tricubic(rows.copy(), "quiet", **kw)
+        lo, hi = np.array(t._lo), np.array(t._hi)
+        q = lo + (hi - lo) * rng.uniform(0.0, 1.0 - 1e-9, size=(3000, 3))
+        out = _nodes_query(t, q)
+        assert t.field_info().tiles_bytes > 0, n
+        ref = o.Query(q.copy())
+        ref = ref if isinstance(ref, tuple) else (ref,)
+        got = [x for x in out[:3] if x is not None]
+        scales = o.condition_scale(q)
+        scales = scales if isinstance(scales, tuple) else (scales,)
+        for gk, r, s in zip(got, ref, scales):
+            m = len(q)
+            mx, _ = O.rel_err_normwise(gk.reshape(m, -1), r.reshape(m, -1), s.reshape(m, -1),
+                                       floor=1.0)
+            assert mx <= 2e-7, (n, mode, mx)
+        t.close()
