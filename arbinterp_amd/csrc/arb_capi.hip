@@ -168,7 +168,7 @@ struct arb_field {
     ArbGeom g{};
     int32_t mode = 0, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
     int32_t table_built = 0;
-    int32_t sentinel_set = 0;   // table row nc written
+    int32_t sentinel_set = 0;   // sentinel row `sent` written
     int32_t table_loaded = 0;   // rows came from arb_table_load
     int32_t bc = ARB_BC_NONE;   // boundary extension (arb_field_set_boundary)
     double edge_lo[ARB_MAXD] = {0, 0, 0, 0};   // first / last node coordinate per axis
@@ -178,8 +178,9 @@ struct arb_field {
     void* tiles = nullptr;      // node tiles of the buckets of B (node-form path)
     int64_t tiles_bytes = 0;
     int32_t tiles_ok = 0;       // tiles hold the current node values
-    void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32
+    void* table = nullptr;      // [nc+1][ninterp][K]: FP64 or FP32, allocated on first build
     int64_t table_bytes = 0;
+    void* sent = nullptr;       // [ninterp][K] the sentinel row, for paths without the table
     double* A = nullptr;        // K x K FP64
     int32_t* flag = nullptr;    // device status word
     // bucketed query path: block geometry of the table kernels (B) and of the
@@ -489,8 +490,10 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     HIPCHK(hipMalloc(&f->A, sizeof(double) * K * K));
     HIPCHK(hipMemcpyAsync(f->A, A.data(), sizeof(double) * K * K, hipMemcpyHostToDevice, s));
     const size_t esz = precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+    // the table itself is allocated by its first build (node-tile queries
+    // never read it)
     f->table_bytes = (int64_t)((g.nc + 1) * f->ninterp * K * esz);
-    HIPCHK(hipMalloc(&f->table, (size_t)f->table_bytes));
+    HIPCHK(hipMalloc(&f->sent, esz * f->ninterp * K));
     HIPCHK(hipStreamSynchronize(s));
     guard.f = nullptr;
     *out = f;
@@ -553,33 +556,40 @@ int arb_field_set_boundary(arb_field* f, int32_t bc, void* stream) {
     const int K = 1 << (2 * go.ndim);
     const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
     const int64_t tbytes = (int64_t)((gn.nc + 1) * f->ninterp * K * esz);
-    DevBuf nv, nt;
+    DevBuf nv;
     HIPCHK(nv.alloc(sizeof(double) * gn.nnodes * f->ninterp));
-    HIPCHK(nt.alloc((size_t)tbytes));
     HIPCHK(arb_launch_pad_boundary(f->vals, go, nv.as<double>(), gn, f->ninterp, bc, s));
     HIPCHK(hipStreamSynchronize(s));
     (void)hipFree(f->vals);
-    (void)hipFree(f->table);
+    if (f->table) (void)hipFree(f->table);    // not built yet: allocated again by the build
     f->vals = nv.as<double>();
-    f->table = nt.p;
+    f->table = nullptr;
     nv.p = nullptr;
-    nt.p = nullptr;
     f->table_bytes = tbytes;
     f->g = gn;
     f->bc = bc;
     return ARB_OK;
 }
 
-// The sentinel row nc of the table, which rows outside the box are evaluated
-// from: zeros in scalar and vector mode, NaN otherwise
-// (ARBInterp.py:31,42-45,57,70,85,99).
+// The sentinel row, which rows outside the box are evaluated from: zeros in
+// scalar and vector mode, NaN otherwise (ARBInterp.py:31,42-45,57,70,85,99).
+// Written to `sent` (paths without the table) and, with the table, to its
+// row nc.
+static int sentinel_nan(const arb_field* f) {
+    return (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
+}
+
 static int ensure_sentinel(arb_field* f, hipStream_t s) {
     if (f->sentinel_set) return ARB_OK;
     const int K = 1 << (2 * f->g.ndim);
-    const int sentinel_nan = (f->mode == ARB_MODE_SCALAR || f->mode == ARB_MODE_VECTOR) ? 0 : 1;
-    HIPCHK(arb_launch_sentinel(f->table, f->precision != ARB_PREC_FP64, f->g.nc, f->ninterp, K,
-                               sentinel_nan, s));
+    HIPCHK(arb_launch_sentinel(f->sent, f->precision != ARB_PREC_FP64, 0, f->ninterp, K,
+                               sentinel_nan(f), s));
     f->sentinel_set = 1;
+    return ARB_OK;
+}
+
+static int ensure_table_alloc(arb_field* f) {
+    if (!f->table) HIPCHK(hipMalloc(&f->table, (size_t)f->table_bytes));
     return ARB_OK;
 }
 
@@ -587,8 +597,13 @@ int arb_build_coeffs(arb_field* f, void* stream) {
     if (!f) return fail(ARB_EINVAL, "arb_build_coeffs: NULL handle");
     if (f->table_built) return ARB_OK;
     hipStream_t s = (hipStream_t)stream;
+    int rc = ensure_table_alloc(f);
+    if (rc) return rc;
     HIPCHK(arb_launch_coef(f->g, f->precision, f->vals, f->ninterp, f->table, s));
-    int rc = ensure_sentinel(f, s);
+    const int K = 1 << (2 * f->g.ndim);
+    HIPCHK(arb_launch_sentinel(f->table, f->precision != ARB_PREC_FP64, f->g.nc, f->ninterp, K,
+                               sentinel_nan(f), s));
+    rc = ensure_sentinel(f, s);
     if (rc) return rc;
     f->table_built = 1;
     return ARB_OK;
@@ -637,7 +652,12 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         return fail(ARB_EINVAL, "arb_query: norms/grads is NULL");
     hipStream_t s = (hipStream_t)stream;
     bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
-    if (!(flags & ARB_Q_DIRECT) && !sorted) sorted = n >= 65536 && 2 * n >= f->g.nc;
+    // automatic choice: the bucketed path from about one query per four cells
+    // with node tiles (it stages ~128 bytes per cell), from one per two cells
+    // with the table (1 KB per cell in mixed 'both'); measured crossovers on
+    // the 128^3 field: 0.5 M and 1-2 M queries
+    if (!(flags & ARB_Q_DIRECT) && !sorted)
+        sorted = n >= 65536 && (node_path(f, flags) ? 4 * n : 2 * n) >= f->g.nc;
     if (n > INT32_MAX - 1) sorted = false;
     if (sorted && f->B.nbt == 0) choose_buckets(f, f->B, true);
     // 3D, bucketed, on request: coefficients formed in the query kernel from
@@ -683,7 +703,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     if (!sorted) {
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
         if (nodes)
-            HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, out64, f->tiles, f->table,
+            HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, out64, f->tiles, f->sent,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
         else
             HIPCHK(arb_launch_query_direct(f->g, f->precision, out64, f->table, f->ninterp,
@@ -719,9 +739,17 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                 bits -= cellbits;
                 cellbits = 0;
             }
-            HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, f->table, f->ninterp,
-                                   f->has_comps, f->has_norm, q_dev, n, ldq, o, f->rs_k[0],
-                                   cellbits, f->rs_hist, s));
+            // rows outside the box: the table's row nc on the table paths
+            // (a loaded table brings its own), the sentinel row otherwise
+            const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
+            const void* sentrow =
+                (fused || nodes) ? f->sent
+                                 : (const void*)((const char*)f->table +
+                                                 esz * (size_t)f->g.nc * f->ninterp *
+                                                     ((size_t)1 << (2 * f->g.ndim)));
+            HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, sentrow, f->has_comps,
+                                   f->has_norm, q_dev, n, ldq, o, f->rs_k[0], cellbits,
+                                   f->rs_hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
             // the sort also leaves each bucket's first sorted position in
@@ -804,7 +832,7 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     o.ldq = d;
     o.flag = reinterpret_cast<int32_t*>(f->pin_dev + PIN_FLAG);
     if (nodes)
-        HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, 1, f->tiles, f->table, f->has_comps,
+        HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, 1, f->tiles, f->sent, f->has_comps,
                                       f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
     else
         HIPCHK(arb_launch_query_direct(f->g, f->precision, 1, f->table, f->ninterp, f->has_comps,
@@ -1135,14 +1163,17 @@ int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_
     const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
     const size_t row = esz * (size_t)f->ninterp * K;
     // rows not loaded keep the built coefficients; a partial load overlays them
+    int rc = ensure_table_alloc(f);
+    if (rc) return rc;
+    rc = ensure_sentinel(f, (hipStream_t)stream);
+    if (rc) return rc;
     if (!(cell0 == 0 && ncells == f->g.nc + 1)) {
-        int rc = arb_build_coeffs(f, stream);
+        rc = arb_build_coeffs(f, stream);
         if (rc) return rc;
     }
     HIPCHK(hipMemcpyAsync((char*)f->table + row * cell0, src_dev, row * ncells,
                           hipMemcpyDeviceToDevice, (hipStream_t)stream));
     f->table_built = 1;
-    f->sentinel_set = 1;
     f->table_loaded = 1;      // the table is authoritative: queries read it
     return ARB_OK;
 }
@@ -1152,6 +1183,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->vals) (void)hipFree(f->vals);
     if (f->tiles) (void)hipFree(f->tiles);
     if (f->table) (void)hipFree(f->table);
+    if (f->sent) (void)hipFree(f->sent);
     if (f->A) (void)hipFree(f->A);
     if (f->flag) (void)hipFree(f->flag);
     if (f->ws_bkt) (void)hipFree(f->ws_bkt);

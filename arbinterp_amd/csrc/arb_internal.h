@@ -250,12 +250,12 @@ constexpr int ARB_RS_THREADS = 512;
 constexpr int ARB_RS_ITEMS = 16;
 // Keys (bucket, or bucket and cell) of n queries, and the tile histograms of
 // the keys' low 8 bits (hist[digit * ntiles + tile]).  The value of key i is
-// the row index i, implicit (the radix sort's index_vals).
+// the row index i, implicit (the radix sort's index_vals).  Rows outside the
+// box are evaluated here from the sentinel row `sent` (ninterp x K values).
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
-                           const void* table, int32_t ninterp, int32_t has_comps,
-                           int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t cellbits, int32_t* hist,
-                           hipStream_t s);
+                           const void* sent, int32_t has_comps, int32_t has_norm,
+                           const double* q, int64_t n, int64_t ldq, const ArbOut& o,
+                           uint32_t* key, int32_t cellbits, int32_t* hist, hipStream_t s);
 int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
@@ -279,7 +279,7 @@ int64_t arb_tiles_bytes(const ArbBuckets& B, int32_t ninterp, int64_t* head_offs
 hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                   const double* vals, void* tiles, hipStream_t s);
 hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
-                                  int32_t out_f64, const void* tiles, const void* table,
+                                  int32_t out_f64, const void* tiles, const void* sent,
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, hipStream_t s);
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,

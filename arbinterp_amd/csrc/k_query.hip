@@ -220,11 +220,9 @@ static_assert(KEYS_ITEMS % KEYS_BATCH == 0, "rows per thread must be a multiple 
 
 template <int D, typename T, typename TC, typename TO>
 __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBuckets& B,
-                                               const TC* __restrict__ table, int32_t ninterp,
-                                               int32_t has_comps, int32_t has_norm,
-                                               const ArbOut& o, int64_t i, const double* qp,
-                                               int32_t cellbits) {
-    constexpr int K = (D == 3) ? 64 : 256;
+                                               const TC* __restrict__ sent, int32_t has_comps,
+                                               int32_t has_norm, const ArbOut& o, int64_t i,
+                                               const double* qp, int32_t cellbits) {
     double loc[D];
     int64_t ia[D];
     bool oob;
@@ -235,8 +233,7 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
     }
     if (flat >= g.nc) {
         if (flat > g.nc) o.flag[0] = 1;
-        eval_point<D, T, TC, TO>(table + g.nc * (int64_t)ninterp * K, has_comps, has_norm, loc, g,
-                                 o, i);
+        eval_point<D, T, TC, TO>(sent, has_comps, has_norm, loc, g, o, i);
         return (uint32_t)B.nbt << cellbits;
     }
     int lc;
@@ -246,7 +243,7 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
 
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(KEYS_THREADS)
-k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, int32_t has_comps,
+k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
        uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
        int64_t ntiles) {
@@ -267,8 +264,8 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp, i
         for (int u = 0; u < KEYS_BATCH; ++u) {
             const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
             if (i < n) {
-                const uint32_t k = key_of_row<D, T, TC, TO>(g, B, table, ninterp, has_comps,
-                                                            has_norm, o, i, qv[u], cellbits);
+                const uint32_t k = key_of_row<D, T, TC, TO>(g, B, sent, has_comps, has_norm, o,
+                                                            i, qv[u], cellbits);
                 key[i] = k;
                 atomicAdd(&cnt[k & 255u], 1);
             }
@@ -947,13 +944,13 @@ __device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets
 // Lane-per-query node-tile kernel (small batches and single points): each
 // lane reads its cell's 64 records and its block's head from the tiles in
 // global memory.  Rows outside the box, or past the sentinel, are evaluated
-// from the table's sentinel row, as everywhere.
+// from the sentinel row `sent`, as everywhere.
 template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_query_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
-               const TileHead* __restrict__ heads, const float* __restrict__ table,
-               int32_t ninterp, int32_t has_comps, int32_t has_norm,
-               const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o) {
+               const TileHead* __restrict__ heads, const float* __restrict__ sent,
+               int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
+               int64_t ldq, ArbOut o) {
     using S = TileShape<NI, BX, BY, BZ>;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -967,8 +964,7 @@ k_query_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __rest
     }
     if (flat >= g.nc) {
         if (flat > g.nc) o.flag[0] = 1;
-        eval_point<3, float, float, TO>(table + g.nc * (int64_t)ninterp * 64, has_comps, has_norm,
-                                        loc, g, o, i);
+        eval_point<3, float, float, TO>(sent, has_comps, has_norm, loc, g, o, i);
         return;
     }
     int lc;
@@ -1267,7 +1263,7 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
 }
 
 hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
-                                  int32_t out_f64, const void* tiles, const void* table,
+                                  int32_t out_f64, const void* tiles, const void* sent,
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -1277,8 +1273,8 @@ hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t
     dim3 grid(grid1(n, 256)), block(256);
 #define L_QT(TO, NI, X, Y, Z)                                                                 \
     hipLaunchKernelGGL((k_query_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,            \
-                       (const typename TileRec<NI>::type*)tiles, heads, (const float*)table,  \
-                       ninterp, has_comps, has_norm, q, n, ldq, o)
+                       (const typename TileRec<NI>::type*)tiles, heads, (const float*)sent,   \
+                       has_comps, has_norm, q, n, ldq, o)
 #define L_QT_NI(TO, X, Y, Z)                                                                  \
     if (ninterp == 4) L_QT(TO, 4, X, Y, Z);                                                   \
     else if (ninterp == 3) L_QT(TO, 3, X, Y, Z);                                              \
@@ -1313,16 +1309,15 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 }
 
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
-                           const void* table, int32_t ninterp, int32_t has_comps,
-                           int32_t has_norm, const double* q, int64_t n, int64_t ldq,
-                           const ArbOut& o, uint32_t* key, int32_t cellbits, int32_t* hist,
-                           hipStream_t s) {
+                           const void* sent, int32_t has_comps, int32_t has_norm,
+                           const double* q, int64_t n, int64_t ldq, const ArbOut& o,
+                           uint32_t* key, int32_t cellbits, int32_t* hist, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
-                       s, g, B, (const TC*)table, ninterp, has_comps, has_norm, q, n, ldq, o,    \
-                       key, cellbits, hist, ntiles)
+                       s, g, B, (const TC*)sent, has_comps, has_norm, q, n, ldq, o, key,         \
+                       cellbits, hist, ntiles)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

@@ -274,3 +274,29 @@ def test_node_tiles_tiny_grids(dev, mode):
                                        floor=1.0)
             assert mx <= 2e-7, (n, mode, mx)
         t.close()
+
+
+def test_node_tiles_need_no_table(dev):
+    """Node-tile queries leave the coefficient table unallocated: the
+    device memory the field takes through creation and a batch of queries
+    is well below the table's size; asking for the table (allCoeffs) then
+    allocates it."""
+    import torch
+    rows, q = W.dipole3d(seed=2, n=(96, 96, 96), nq=300_000)
+    qd = torch.from_numpy(q).cuda()
+    torch.cuda.synchronize()
+    free0, _ = torch.cuda.mem_get_info()
+    t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
+    out = t.query_device(qd)
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    info = t.field_info()
+    assert info.tiles_bytes > 0
+    table = int(info.table_bytes)                   # 93^3 cells x 1 KB: about 0.8 GB
+    assert free0 - free1 < table // 2, (free0 - free1, table)
+    t.allCoeffs()
+    torch.cuda.synchronize()
+    free2, _ = torch.cuda.mem_get_info()
+    assert free1 - free2 >= table * 9 // 10, (free1 - free2, table)
+    del out
+    t.close()
