@@ -82,9 +82,9 @@ extern "C" {
                                padded; norms_dev/grads_dev are not used        */
 #define ARB_Q_FUSED  1024   /* 3D bucketed path: form each block's
                                coefficients in the query kernel from node
-                               values instead of staging them from the table
-                               (default).  No table is built or read; the
-                               results are bitwise the same                  */
+                               values instead of staging them from the table.
+                               No table is built or read; the results are
+                               bitwise those of the table-reading paths      */
 #define ARB_Q_SORT_CELL 512  /* bucketed path: order queries by bucket, then by
                                cell inside the bucket (default: bucket only)    */
 #define ARB_Q_TABLE  2048   /* read each cell's coefficients from the table
