@@ -192,6 +192,8 @@ struct arb_field {
     int32_t* ws_bkt = nullptr;  // per query: bucket (-1 = handled in place)
     int32_t* ws_rank = nullptr; // per query: rank inside its bucket
     int32_t* ws_perm = nullptr; // bucket-sorted query indices
+    void* ws_rec = nullptr;     // packed output records of a bucketed call (split afterwards)
+    int64_t ws_rec_bytes = 0;
     int64_t ws_n = 0;
     int32_t* ws_count = nullptr;
     int32_t* ws_offs = nullptr;
@@ -339,6 +341,18 @@ static int ensure_workspace(arb_field* f, int64_t n, int64_t nbt) {
         HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * arb_rs_first_elems(nbt)));
         HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (nbt + 1)));
         f->ws_nb = nbt + 1;
+    }
+    return ARB_OK;
+}
+
+// Room for `bytes` of packed output records, made on first use.
+static int ensure_records(arb_field* f, int64_t bytes) {
+    if (bytes > f->ws_rec_bytes) {
+        if (f->ws_rec) (void)hipFree(f->ws_rec);
+        f->ws_rec = nullptr;
+        f->ws_rec_bytes = 0;
+        HIPCHK(hipMalloc(&f->ws_rec, (size_t)bytes));
+        f->ws_rec_bytes = bytes;
     }
     return ARB_OK;
 }
@@ -716,6 +730,22 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     } else {
         rc = ensure_workspace(f, n, BB.nbt);
         if (rc) return rc;
+        // separate output arrays: the bucketed kernels write each row as one
+        // packed record into the workspace, and k_unpack splits the records
+        // into the arrays in row order (random partial-line writes to three
+        // arrays cost more than the extra ordered pass)
+        const bool split = !packed;
+        if (split) {
+            const int32_t w = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
+            const bool wide = f->precision == ARB_PREC_FP64 || out64;
+            const int64_t esz = wide ? (int64_t)sizeof(double) : (int64_t)sizeof(float);
+            rc = ensure_records(f, n * w * esz);
+            if (rc) return rc;
+            o.comps = f->ws_rec;
+            o.norms = nullptr;
+            o.grads = nullptr;
+            o.packed = w;
+        }
         const int32_t* perm = f->ws_perm;
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
         if (flags & ARB_Q_GROUP_ATOMIC) {
@@ -776,6 +806,10 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
                                           f->ws_offs, perm, f->eval_threads, s));
+        if (split)
+            HIPCHK(arb_launch_unpack(f->g.ndim, f->precision == ARB_PREC_FP64 || out64,
+                                     f->has_comps, f->has_norm, f->ws_rec, comps_dev, norms_dev,
+                                     grads_dev, n, s));
         if (timing) {
             HIPCHK(hipEventRecord(f->ev[4], s));
             HIPCHK(hipEventSynchronize(f->ev[4]));
@@ -1189,6 +1223,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->ws_bkt) (void)hipFree(f->ws_bkt);
     if (f->ws_rank) (void)hipFree(f->ws_rank);
     if (f->ws_perm) (void)hipFree(f->ws_perm);
+    if (f->ws_rec) (void)hipFree(f->ws_rec);
     if (f->ws_count) (void)hipFree(f->ws_count);
     if (f->ws_offs) (void)hipFree(f->ws_offs);
     for (int k = 0; k < 2; ++k) {

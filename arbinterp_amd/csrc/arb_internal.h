@@ -244,6 +244,9 @@ hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec
                              const ArbOut& o, int32_t* bkt, int32_t* rank, int32_t* count,
                              hipStream_t s);
 hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipStream_t s);
+hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, int32_t has_norm,
+                             const void* rec, void* comps, void* norms, void* grads, int64_t n,
+                             hipStream_t s);
 // Radix-sort tiles: ARB_RS_THREADS threads x ARB_RS_ITEMS pairs (8192; the
 // longer digit runs of the larger tile make the scattered writes wider).
 constexpr int ARB_RS_THREADS = 512;

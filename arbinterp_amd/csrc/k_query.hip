@@ -312,6 +312,33 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count
     if (t == 0) offs[nb] = carry;
 }
 
+// Split packed records (one aligned record of `width` elements per row, as
+// the bucketed kernels write them) into the caller's separate comps / norms /
+// grads arrays.  Row i is thread i, so both the record reads and the array
+// writes are contiguous over a wave; the bucketed kernels' random writes then
+// cost one full record per row instead of three partial-line writes.
+template <typename TO, int D>
+__global__ void __launch_bounds__(256)
+k_unpack(const TO* __restrict__ rec, int32_t width, int32_t has_comps, int32_t has_norm,
+         TO* __restrict__ comps, TO* __restrict__ norms, TO* __restrict__ grads, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    TO r[8];
+    const TO* src = rec + i * width;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) r[c] = c < width ? src[c] : TO(0);
+    if (has_comps) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) comps[i * 3 + c] = r[c];
+    }
+    if (has_norm) {
+        const int b = has_comps ? 3 : 0;
+        norms[i] = r[b];
+#pragma unroll
+        for (int d = 0; d < D; ++d) grads[i * D + d] = r[b + 1 + d];
+    }
+}
+
 // Scatter: query index into bucket order.
 __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
                           const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
@@ -1096,6 +1123,25 @@ hipError_t arb_launch_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_BUCKET);
     else ARB_DISPATCH(4, prec, out_f64, L_BUCKET);
 #undef L_BUCKET
+    return hipGetLastError();
+}
+
+hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, int32_t has_norm,
+                             const void* rec, void* comps, void* norms, void* grads, int64_t n,
+                             hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int32_t w = arb_packed_width(has_comps, has_norm, ndim);
+#define L_UNPACK(TO, DD)                                                                       \
+    hipLaunchKernelGGL((k_unpack<TO, DD>), dim3(grid1(n, 256)), dim3(256), 0, s, (const TO*)rec, \
+                       w, has_comps, has_norm, (TO*)comps, (TO*)norms, (TO*)grads, n)
+    if (out_f64) {
+        if (ndim == 3) L_UNPACK(double, 3);
+        else L_UNPACK(double, 4);
+    } else {
+        if (ndim == 3) L_UNPACK(float, 3);
+        else L_UNPACK(float, 4);
+    }
+#undef L_UNPACK
     return hipGetLastError();
 }
 

@@ -374,6 +374,18 @@ def test_packed_records_match_arrays(dev, case):
                 assert b is None
                 continue
             assert _nan_equal(a, b.contiguous()), (case, fl)
+    # the bucketed path splits its packed workspace records into the arrays:
+    # the same bits as the direct path, also for a later, smaller call that
+    # reuses the workspace, and with FP64 outputs of a mixed field
+    for f64 in ((False, True) if "mixed" in case else (False,)):
+        for m in (len(qd), 40_000):
+            a = t.query_device(qd[:m], flags=NN.ARB_Q_DIRECT, out_f64=f64)
+            b = t.query_device(qd[:m], flags=NN.ARB_Q_SORTED, out_f64=f64)
+            for x, y in zip(a, b):
+                if x is None:
+                    assert y is None
+                    continue
+                assert _nan_equal(x, y), (case, f64, m)
 
 
 def _point_cases(golden):
