@@ -15,6 +15,10 @@
  *     gives a thread-local message for the last failing call.
  *   - A handle is not thread-safe (neither is the reference object: Query
  *     mutates queryInds and the coefficient cache, ARBInterp.py:368-377).
+ *     Its queries share the handle's device workspaces (sort buffers, packed
+ *     output records, node tiles), so calls on one handle issued to
+ *     different streams must not run at the same time: order the streams,
+ *     or use one stream per handle.
  *   - All layouts are row-major.  Node/cell order is x fastest, then y, z (t):
  *     node = ix + nx*(iy + ny*(iz [+ nz*it]))  (ARBInterp.py:530-532,1266-1269);
  *     cell = cx + npx*(cy + npy*(cz [+ npz*ct])), npx = nx-3 (ARBInterp.py:368,1088).
