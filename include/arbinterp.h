@@ -60,11 +60,13 @@ extern "C" {
                                NaN (ARBInterp.py:75-102)                         */
 
 /* ---- precision --------------------------------------------------------- */
-#define ARB_PREC_FP64   0   /* plain: FP64 everywhere; A = inv(B).D GEMM on
-                               FP64 MFMA; FP64 table; FP64 evaluation            */
-#define ARB_PREC_MIXED  1   /* FP64 cell reduction and FD stencils; inv(B).b as a
-                               split BF16x3 MFMA GEMM with FP32 accumulation;
-                               FP32 table; FP32 evaluation                       */
+#define ARB_PREC_FP64   0   /* plain: FP64 everywhere (separable coefficient
+                               passes, table, evaluation)                       */
+#define ARB_PREC_MIXED  1   /* FP64 cell reduction and coefficient passes; 3D
+                               queries evaluate FP32 node-tile differences from
+                               an FP64 per-block model; the table (4D, the
+                               batch kernel, ARB_Q_TABLE) is formed in FP64 and
+                               stored in FP32; FP32 evaluation                   */
 
 /* ---- query flags ------------------------------------------------------- */
 #define ARB_Q_NAN_ROWS  1   /* write NaN over every column of rows with a
