@@ -734,7 +734,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         // write each row as one packed record into the workspace, and
         // k_unpack splits the records into the arrays in row order (three
         // random partial-line writes per row cost more than the extra ordered
-        // pass).  Measured on 10^7 'both' rows: 0.93 -> 0.70 ms.  With fewer
+        // pass).  Measured on 10^7 'both' rows: 0.93 -> 0.68 ms.  With fewer
         // arrays, or in 4D where the evaluation hides the writes, the split
         // gains nothing (2e6 4D rows: eval time unchanged, +33 us of split).
         const bool split =

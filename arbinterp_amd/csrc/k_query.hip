@@ -317,25 +317,45 @@ __global__ void __launch_bounds__(1024) k_scan(const int32_t* __restrict__ count
 // grads arrays.  Row i is thread i, so both the record reads and the array
 // writes are contiguous over a wave; the bucketed kernels' random writes then
 // cost one full record per row instead of three partial-line writes.
-template <typename TO, int D>
+template <typename TO, int D, int W>
 __global__ void __launch_bounds__(256)
-k_unpack(const TO* __restrict__ rec, int32_t width, int32_t has_comps, int32_t has_norm,
+k_unpack(const TO* __restrict__ rec, int32_t has_comps, int32_t has_norm,
          TO* __restrict__ comps, TO* __restrict__ norms, TO* __restrict__ grads, int64_t n) {
+    static_assert(W == 4 || W == 8, "records are 4 or 8 elements");
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    TO r[8];
-    const TO* src = rec + i * width;
+    // the record is 16-byte aligned (W * sizeof(TO) is 16, 32 or 64 bytes)
+    TO r[W];
+    using V = typename std::conditional<sizeof(TO) == 4, float4, double2>::type;
+    constexpr int PER = 16 / sizeof(TO);
+    const V* src = reinterpret_cast<const V*>(rec + i * W);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) r[c] = c < width ? src[c] : TO(0);
+    for (int v = 0; v < W / PER; ++v) {
+        const V x = src[v];
+        r[v * PER] = x.x;
+        r[v * PER + 1] = x.y;
+        if constexpr (PER == 4) {
+            r[v * PER + 2] = x.z;
+            r[v * PER + 3] = x.w;
+        }
+    }
+    // constant indices in each branch keep r in registers; an index past the
+    // record (possible only in combinations the width rules out) reads 0
+    auto at = [&](int k) { return k < W ? r[k < W ? k : 0] : TO(0); };
     if (has_comps) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) comps[i * 3 + c] = r[c];
+        for (int c = 0; c < 3; ++c) comps[i * 3 + c] = at(c);
     }
     if (has_norm) {
-        const int b = has_comps ? 3 : 0;
-        norms[i] = r[b];
+        if (has_comps) {
+            norms[i] = at(3);
 #pragma unroll
-        for (int d = 0; d < D; ++d) grads[i * D + d] = r[b + 1 + d];
+            for (int d = 0; d < D; ++d) grads[i * D + d] = at(4 + d);
+        } else {
+            norms[i] = at(0);
+#pragma unroll
+            for (int d = 0; d < D; ++d) grads[i * D + d] = at(1 + d);
+        }
     }
 }
 
@@ -1131,16 +1151,21 @@ hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, i
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int32_t w = arb_packed_width(has_comps, has_norm, ndim);
-#define L_UNPACK(TO, DD)                                                                       \
-    hipLaunchKernelGGL((k_unpack<TO, DD>), dim3(grid1(n, 256)), dim3(256), 0, s, (const TO*)rec, \
-                       w, has_comps, has_norm, (TO*)comps, (TO*)norms, (TO*)grads, n)
+#define L_UNPACK(TO, DD, WW)                                                                   \
+    hipLaunchKernelGGL((k_unpack<TO, DD, WW>), dim3(grid1(n, 256)), dim3(256), 0, s,             \
+                       (const TO*)rec, has_comps, has_norm, (TO*)comps, (TO*)norms,             \
+                       (TO*)grads, n)
+#define L_UNPACK_W(TO, DD)                                                                     \
+    if (w == 4) L_UNPACK(TO, DD, 4);                                                           \
+    else L_UNPACK(TO, DD, 8)
     if (out_f64) {
-        if (ndim == 3) L_UNPACK(double, 3);
-        else L_UNPACK(double, 4);
+        if (ndim == 3) L_UNPACK_W(double, 3);
+        else L_UNPACK_W(double, 4);
     } else {
-        if (ndim == 3) L_UNPACK(float, 3);
-        else L_UNPACK(float, 4);
+        if (ndim == 3) L_UNPACK_W(float, 3);
+        else L_UNPACK_W(float, 4);
     }
+#undef L_UNPACK_W
 #undef L_UNPACK
     return hipGetLastError();
 }
