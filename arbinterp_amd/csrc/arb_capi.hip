@@ -730,18 +730,19 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     } else {
         rc = ensure_workspace(f, n, BB.nbt);
         if (rc) return rc;
-        // three separate output arrays of a 3D field: the bucketed kernels
-        // write each row as one packed record into the workspace, and
-        // k_unpack splits the records into the arrays in row order (three
-        // random partial-line writes per row cost more than the extra ordered
-        // pass).  Measured on 10^7 'both' rows: 0.93 -> 0.68 ms.  With fewer
-        // arrays, or in 4D where the evaluation hides the writes, the split
-        // gains nothing (2e6 4D rows: eval time unchanged, +33 us of split).
-        const bool split =
-            !packed && f->g.ndim == 3 && f->has_comps && f->has_norm;
+        // separate output arrays of a 3D field: the bucketed kernels write
+        // each row as one packed record into the workspace, and k_unpack
+        // splits the records into the arrays in row order (random
+        // partial-line writes to several arrays per row cost more than the
+        // extra ordered pass).  Measured on 10^7 rows of the 128^3 field:
+        // 'both' 0.93 -> 0.68 ms; FP64 'norm' / scalar rows (norms and
+        // grads) 1.24 ms as arrays against 0.96 ms as records.  FP32 norms
+        // and grads, comps alone, and 4D (whose evaluation hides the writes:
+        // 2e6 rows, eval time unchanged, +33 us of split) gain nothing.
+        const bool wide = f->precision == ARB_PREC_FP64 || out64;
+        const bool split = !packed && f->g.ndim == 3 && f->has_norm && (f->has_comps || wide);
         if (split) {
             const int32_t w = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
-            const bool wide = f->precision == ARB_PREC_FP64 || out64;
             const int64_t esz = wide ? (int64_t)sizeof(double) : (int64_t)sizeof(float);
             rc = ensure_records(f, n * w * esz);
             if (rc) return rc;
