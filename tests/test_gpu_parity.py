@@ -347,8 +347,8 @@ def test_coefficient_save_load_roundtrip(dev, tmp_path):
             other.load_coefficients(path)
 
 
-@pytest.mark.parametrize("case", ["3d_both_mixed", "3d_scalar_fp64", "3d_vector", "4d_both_fp64",
-                                  "4d_scalar_mixed"])
+@pytest.mark.parametrize("case", ["3d_both_mixed", "3d_scalar_fp64", "3d_scalar_mixed", "3d_vector",
+                                  "4d_both_fp64", "4d_scalar_mixed"])
 def test_packed_records_match_arrays(dev, case):
     import torch
     from arbinterp_amd import _native as NN
