@@ -25,6 +25,34 @@ __device__ __forceinline__ void cub4(double n0, double n1, double n2, double n3,
     c[3] = fma(0.5, n3, fma(-1.5, n2, fma(1.5, n1, -0.5 * n0)));
 }
 
+// Non-finite node values.  The reference forms a cell's coefficients as
+// alpha = A . F[stencil] (np.dot, ARBInterp.py:599-602, :1351-1354); A has a
+// zero in every column and 0 * NaN = 0 * Inf = NaN, so a NaN or +-Inf value
+// anywhere in a cell's 4^d stencil makes every coefficient of the cell NaN
+// (or +-Inf), and with it every output of the cell; no other cell changes.
+// cub4 skips M's zero entries, which would leave some of those coefficients
+// finite, so the builds mark such cells and store NaN for all of their
+// coefficients.  (For +-Inf values the reference has NaN or +-Inf per
+// coefficient; the outputs are NaN either way.)
+//
+// cell_bad3: per cell of a BX x BY x BZ tile (node tile laid out as in
+// sep3_passes), whether its 4 x 4 x 4 stencil holds a non-finite value.
+// Block-wide; the callers take this path only for a tile that holds one
+// (64 LDS reads per cell).
+template <int BX, int BY, int BZ>
+__device__ void cell_bad3(const double* nodes, unsigned char* bad) {
+    constexpr int NX = BX + 3, NY = BY + 3;
+    for (int c = threadIdx.x; c < BX * BY * BZ; c += blockDim.x) {
+        const int cx = c % BX, cy = (c / BX) % BY, cz = c / (BX * BY);
+        bool b = false;
+        for (int z = 0; z < 4; ++z)
+            for (int y = 0; y < 4; ++y)
+                for (int x = 0; x < 4; ++x)
+                    b |= !isfinite(nodes[((cz + z) * NY + cy + y) * NX + cx + x]);
+        bad[c] = b ? 1 : 0;
+    }
+}
+
 // LDS doubles the passes need for a tile of bx x by x bz cells: node tile,
 // x-pass and y-pass results.
 __host__ __device__ inline int sep3_nodes(int bx, int by, int bz) {

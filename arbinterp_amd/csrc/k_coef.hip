@@ -43,6 +43,7 @@ k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
     __shared__ double nodes[NT];
     __shared__ __align__(16) double t1[NY * NZ * SX * 4];
     __shared__ __align__(16) double t2[NZ * SY * SX * 16];
+    __shared__ unsigned char cbad[SX * SY * SZ3];       // cells with a non-finite stencil
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
     const int64_t cz0 = (int64_t)blockIdx.z * SZ3;
     for (int k = 0; k < ninterp; ++k) {
@@ -58,17 +59,26 @@ k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
             v[j] = in ? t : 0.0;
         }
         __syncthreads();                            // the previous interpolant is done
+        int nonfinite = 0;
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
             const int i = threadIdx.x + 256 * j;
-            if (i < NT) nodes[i] = v[j];
+            if (i < NT) {
+                nodes[i] = v[j];
+                nonfinite |= !isfinite(v[j]);
+            }
         }
-        __syncthreads();
+        const bool anybad = __syncthreads_or(nonfinite) != 0;
+        if (anybad) {
+            arbs3::cell_bad3<SX, SY, SZ3>(nodes, cbad);
+            __syncthreads();
+        }
         arbs3::sep3_passes<SX, SY, SZ3, false>(nodes, t1, t2,
                                               [&](int cx, int cy, int cz, int i, double a) {
             const int64_t x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
             if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
                 const int64_t cell = x + g.npos[0] * (y + g.npos[1] * z);
+                if (anybad && cbad[cx + SX * (cy + SY * cz)]) a = NAN;
                 table[(cell * ninterp + k) * 64 + i] = (TO)a;
             }
         });
@@ -103,11 +113,13 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
     __shared__ __align__(16) double t2[4 * SY * SX * 16];
     __shared__ double gc[2][2][GY][GX];
     __shared__ double cc[SX * SY][16];
+    __shared__ unsigned char cbad[SX * SY];             // cells with a non-finite stencil
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
     const int64_t cz = blockIdx.z % g.npos[2], ct = blockIdx.z / g.npos[2];
     for (int k = 0; k < ninterp; ++k) {
         const double* F = vals + (int64_t)k * g.nnodes;
         __syncthreads();                            // the previous interpolant is done
+        int nonfinite = 0;
         for (int i = threadIdx.x; i < 4 * 4 * NY * NX; i += blockDim.x) {
             const int x = i % NX, y = (i / NX) % NY, z = (i / (NX * NY)) & 3, t = i / (NX * NY * 4);
             const int64_t nx = cx0 + x, ny = cy0 + y;
@@ -115,8 +127,22 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
             if (nx < g.n[0] && ny < g.n[1])
                 v = F[nx + g.nstride[1] * ny + g.nstride[2] * (cz + z) + g.nstride[3] * (ct + t)];
             (&n4[0][0][0][0])[i] = v;
+            nonfinite |= !isfinite(v);
         }
-        __syncthreads();
+        const bool anybad = __syncthreads_or(nonfinite) != 0;
+        if (anybad) {
+            // per cell: a non-finite value among its 4^4 stencil nodes
+            for (int c = threadIdx.x; c < SX * SY; c += blockDim.x) {
+                const int cx = c % SX, cy = c / SX;
+                bool b = false;
+                for (int t = 0; t < 4; ++t)
+                    for (int z = 0; z < 4; ++z)
+                        for (int y = 0; y < 4; ++y)
+                            for (int x = 0; x < 4; ++x) b |= !isfinite(n4[t][z][cy + y][cx + x]);
+                cbad[c] = b ? 1 : 0;
+            }
+            __syncthreads();
+        }
         // xyzt central differences at the corner nodes (tile node + 1)
         for (int i = threadIdx.x; i < 2 * 2 * GY * GX; i += blockDim.x) {
             const int gx = i % GX, gy = (i / GX) % GY, gz = (i / (GX * GY)) & 1, gt = i / (GX * GY * 2);
@@ -166,6 +192,7 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
                 const int64_t x = cx0 + cx, y = cy0 + cy;
                 if (x < g.npos[0] && y < g.npos[1]) {
                     const int64_t cell = x + g.npos[0] * (y + g.npos[1] * (cz + g.npos[2] * ct));
+                    if (anybad && cbad[cx + SX * cy]) a = NAN;
                     table[(cell * ninterp + k) * 256 + i] = (TO)a;
                 }
             });

@@ -579,19 +579,31 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
         const double v = vals[src];
         nv[j] = in ? v : 0.0;
     }
+    __shared__ unsigned char cbad[BX * BY * BZ];        // cells with a non-finite stencil
     for (int k = 0; k < ninterp; ++k) {
         // tile k into LDS: the x pass of interpolant k - 1, the last reader
-        // of `nodes`, ended at that interpolant's first pass barrier
+        // of `nodes`, ended at that interpolant's first pass barrier (so did
+        // its last reader of cbad, the z pass's stores)
+        int nonfinite = 0;
 #pragma unroll
         for (int j = 0; j < FUSED_REGS; ++j) {
             const int e = threadIdx.x + j * blockDim.x - k * NT;
-            if (e >= 0 && e < NT) nodes[e] = nv[j];
+            if (e >= 0 && e < NT) {
+                nodes[e] = nv[j];
+                nonfinite |= !isfinite(nv[j]);
+            }
         }
-        __syncthreads();
+        const bool anybad = __syncthreads_or(nonfinite) != 0;
+        if (anybad) {
+            arbs3::cell_bad3<BX, BY, BZ>(nodes, cbad);
+            __syncthreads();
+        }
         // consecutive threads, consecutive cells: local cell cx + BX (cy + BY cz)
         arbs3::sep3_passes<BX, BY, BZ, true>(nodes, t1, t2,
                                              [&](int cx, int cy, int cz, int i, double a) {
-            cf[(cx + BX * (cy + BY * cz)) * cstr + k * 64 + i] = (T)a;
+            const int lc = cx + BX * (cy + BY * cz);
+            if (anybad && cbad[lc]) a = NAN;
+            cf[lc * cstr + k * 64 + i] = (T)a;
         });
     }
     __syncthreads();
@@ -752,84 +764,155 @@ __device__ __forceinline__ void bucket_origin3(int64_t b, const ArbBuckets& B, i
     }
 }
 
+// The model of interpolant k (values v = that interpolant's node values)
+// around grid node mi: F(mi) and the central first differences, and for the
+// last interpolant the second (and, with the cubic model, third) differences.
+// Returns whether every number of it is finite.
+template <int NI>
+__device__ bool tile_head_at(const ArbGeom& g, const double* __restrict__ v, int64_t mi, int k,
+                             TileHead& h) {
+    const int64_t sx = 1, sy = g.nstride[1], sz = g.nstride[2];
+    const double f = v[mi];
+    h.f[k] = f;
+    h.gx[k] = 0.5 * (v[mi + sx] - v[mi - sx]);
+    h.gy[k] = 0.5 * (v[mi + sy] - v[mi - sy]);
+    h.gz[k] = 0.5 * (v[mi + sz] - v[mi - sz]);
+    bool ok = isfinite(f) && isfinite(h.gx[k]) && isfinite(h.gy[k]) && isfinite(h.gz[k]);
+    if (k != NI - 1) return ok;
+    h.hxx = v[mi + sx] - 2.0 * f + v[mi - sx];
+    h.hyy = v[mi + sy] - 2.0 * f + v[mi - sy];
+    h.hzz = v[mi + sz] - 2.0 * f + v[mi - sz];
+    h.hxy = 0.25 * ((v[mi + sx + sy] - v[mi + sx - sy]) - (v[mi - sx + sy] - v[mi - sx - sy]));
+    h.hxz = 0.25 * ((v[mi + sx + sz] - v[mi + sx - sz]) - (v[mi - sx + sz] - v[mi - sx - sz]));
+    h.hyz = 0.25 * ((v[mi + sy + sz] - v[mi + sy - sz]) - (v[mi - sy + sz] - v[mi - sy - sz]));
+    ok = ok && isfinite(h.hxx) && isfinite(h.hyy) && isfinite(h.hzz) && isfinite(h.hxy) &&
+         isfinite(h.hxz) && isfinite(h.hyz);
+    if constexpr (tile_cubic<NI>()) {
+        // third differences: along one axis (5 nodes; 0 where the axis has
+        // only 4), and the central differences of the second ones
+        auto d3 = [&](int64_t st, int a) {
+            return g.n[a] >= 5 ? 0.5 * ((v[mi + 2 * st] - v[mi - 2 * st]) -
+                                        2.0 * (v[mi + st] - v[mi - st]))
+                               : 0.0;
+        };
+        auto d2d1 = [&](int64_t s2, int64_t s1) {   // d/d(s1) of d2/d(s2)^2
+            return 0.5 * ((v[mi + s2 + s1] - 2.0 * v[mi + s1] + v[mi - s2 + s1]) -
+                          (v[mi + s2 - s1] - 2.0 * v[mi - s1] + v[mi - s2 - s1]));
+        };
+        h.txxx = d3(sx, 0);
+        h.tyyy = d3(sy, 1);
+        h.tzzz = d3(sz, 2);
+        h.txxy = d2d1(sx, sy);
+        h.txxz = d2d1(sx, sz);
+        h.txyy = d2d1(sy, sx);
+        h.tyyz = d2d1(sy, sz);
+        h.txzz = d2d1(sz, sx);
+        h.tyzz = d2d1(sz, sy);
+        h.txyz = 0.125 * ((v[mi + sx + sy + sz] - v[mi + sx + sy - sz]) -
+                          (v[mi + sx - sy + sz] - v[mi + sx - sy - sz]) -
+                          (v[mi - sx + sy + sz] - v[mi - sx + sy - sz]) +
+                          (v[mi - sx - sy + sz] - v[mi - sx - sy - sz]));
+        ok = ok && isfinite(h.txxx) && isfinite(h.tyyy) && isfinite(h.tzzz) &&
+             isfinite(h.txxy) && isfinite(h.txxz) && isfinite(h.txyy) && isfinite(h.tyyz) &&
+             isfinite(h.txzz) && isfinite(h.tyzz) && isfinite(h.txyz);
+    } else {
+        h.txxx = h.tyyy = h.tzzz = h.txxy = h.txxz = 0.0;
+        h.txyy = h.tyyz = h.txzz = h.tyzz = h.txyz = 0.0;
+    }
+    return ok;
+}
+
+// A zero model for interpolant k (its records then hold the node values).
+template <int NI>
+__device__ void tile_head_zero(TileHead& h, int k) {
+    h.f[k] = h.gx[k] = h.gy[k] = h.gz[k] = 0.0;
+    if (k == NI - 1) {
+        h.hxx = h.hyy = h.hzz = h.hxy = h.hxz = h.hyz = 0.0;
+        h.txxx = h.tyyy = h.tzzz = h.txxy = h.txxz = 0.0;
+        h.txyy = h.tyyz = h.txzz = h.tyzz = h.txyz = 0.0;
+    }
+}
+
 // One workgroup per bucket: the head, then every record of the tile.
+//
+// Non-finite node values.  In the reference a NaN or +-Inf node value makes
+// every coefficient of every cell whose 4x4x4 stencil holds it NaN (alpha =
+// A . F, ARBInterp.py:599-602, and A has a zero in every column), and no
+// other cell.  Here such a node's record is NaN, so exactly the cells whose
+// 64 records include it evaluate to NaN (every record enters every output of
+// its interpolant).  The model must stay finite, or it would spread the NaN
+// to the whole bucket: when the model around the middle node m is not
+// finite, the tile node nearest to m (then lowest index) whose model is
+// finite for every interpolant becomes the centre; if there is none, each
+// interpolant whose model at m is not finite gets a zero model.
 template <int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
                typename TileRec<NI>::type* __restrict__ tiles, TileHead* __restrict__ heads) {
     using S = TileShape<NI, BX, BY, BZ>;
     __shared__ TileHead h;
+    __shared__ int best;
     const int64_t b = blockIdx.x;
     int64_t c0[3];
     bucket_origin3(b, B, c0);
-    if (threadIdx.x < 4) {
-        const int k = threadIdx.x;
-        const int64_t nd[3] = {S::NX / 2, S::NY / 2, S::NZ / 2};
-        int64_t m[3];
+    // the centre range per axis: one node inside the grid so that the central
+    // differences exist (two where the axis has 5 nodes or more, for the third
+    // differences along it); the default centre is the tile's middle node
+    int64_t lo[3], hi[3], m[3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            // middle node, kept one node inside the grid so that the central
-            // differences exist
-            // (two nodes inside where the axis has 5 nodes or more, for the
-            // third differences along it)
-            const int64_t in = g.n[a] >= 5 ? 2 : 1;
-            m[a] = c0[a] + nd[a];
-            m[a] = m[a] < in ? in : (m[a] > g.n[a] - 1 - in ? g.n[a] - 1 - in : m[a]);
-        }
-        const int64_t mi = m[0] + g.nstride[1] * m[1] + g.nstride[2] * m[2];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t in = g.n[a] >= 5 ? 2 : 1;
+        lo[a] = in;
+        hi[a] = g.n[a] - 1 - in;
+        const int64_t nd = (a == 0 ? S::NX : a == 1 ? S::NY : S::NZ) / 2;
+        m[a] = c0[a] + nd;
+        m[a] = m[a] < lo[a] ? lo[a] : (m[a] > hi[a] ? hi[a] : m[a]);
+    }
+    const int k = threadIdx.x;
+    bool bad = false;
+    if (k < 4) {
         if (k < NI) {
-            const double* v = vals + k * g.nnodes;
-            const double f = v[mi];
-            h.f[k] = f;
-            h.gx[k] = 0.5 * (v[mi + 1] - v[mi - 1]);
-            h.gy[k] = 0.5 * (v[mi + g.nstride[1]] - v[mi - g.nstride[1]]);
-            h.gz[k] = 0.5 * (v[mi + g.nstride[2]] - v[mi - g.nstride[2]]);
-            if (k == NI - 1) {
-                const int64_t sx = 1, sy = g.nstride[1], sz = g.nstride[2];
-                h.hxx = v[mi + sx] - 2.0 * f + v[mi - sx];
-                h.hyy = v[mi + sy] - 2.0 * f + v[mi - sy];
-                h.hzz = v[mi + sz] - 2.0 * f + v[mi - sz];
-                h.hxy = 0.25 * ((v[mi + sx + sy] - v[mi + sx - sy]) - (v[mi - sx + sy] - v[mi - sx - sy]));
-                h.hxz = 0.25 * ((v[mi + sx + sz] - v[mi + sx - sz]) - (v[mi - sx + sz] - v[mi - sx - sz]));
-                h.hyz = 0.25 * ((v[mi + sy + sz] - v[mi + sy - sz]) - (v[mi - sy + sz] - v[mi - sy - sz]));
-                // third differences: along one axis (5 nodes; 0 where the axis
-                // has only 4), and the central differences of the second ones
-                auto d3 = [&](int64_t st, int a) {
-                    return g.n[a] >= 5 ? 0.5 * ((v[mi + 2 * st] - v[mi - 2 * st]) -
-                                                2.0 * (v[mi + st] - v[mi - st]))
-                                       : 0.0;
-                };
-                auto d2d1 = [&](int64_t s2, int64_t s1) {   // d/d(s1) of d2/d(s2)^2
-                    return 0.5 * ((v[mi + s2 + s1] - 2.0 * v[mi + s1] + v[mi - s2 + s1]) -
-                                  (v[mi + s2 - s1] - 2.0 * v[mi - s1] + v[mi - s2 - s1]));
-                };
-                if constexpr (tile_cubic<NI>()) {
-                    h.txxx = d3(sx, 0);
-                    h.tyyy = d3(sy, 1);
-                    h.tzzz = d3(sz, 2);
-                    h.txxy = d2d1(sx, sy);
-                    h.txxz = d2d1(sx, sz);
-                    h.txyy = d2d1(sy, sx);
-                    h.tyyz = d2d1(sy, sz);
-                    h.txzz = d2d1(sz, sx);
-                    h.tyzz = d2d1(sz, sy);
-                    h.txyz = 0.125 * ((v[mi + sx + sy + sz] - v[mi + sx + sy - sz]) -
-                                      (v[mi + sx - sy + sz] - v[mi + sx - sy - sz]) -
-                                      (v[mi - sx + sy + sz] - v[mi - sx + sy - sz]) +
-                                      (v[mi - sx - sy + sz] - v[mi - sx - sy - sz]));
-                } else {
-                    h.txxx = h.tyyy = h.tzzz = h.txxy = h.txxz = 0.0;
-                    h.txyy = h.tyyz = h.txzz = h.tyzz = h.txyz = 0.0;
-                }
-            }
+            const int64_t mi = m[0] + g.nstride[1] * m[1] + g.nstride[2] * m[2];
+            bad = !tile_head_at<NI>(g, vals + k * g.nnodes, mi, k, h);
         } else {
             h.f[k] = h.gx[k] = h.gy[k] = h.gz[k] = 0.0;
         }
-        if (k == 0) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) h.m[a] = (int32_t)(m[a] - c0[a]);
-            h.pad = 0;
+    }
+    if (threadIdx.x == 0) best = INT32_MAX;
+    if (__syncthreads_or(bad)) {
+        // search the tile for the nearest centre with a finite model
+        for (int e = threadIdx.x; e < S::NT; e += blockDim.x) {
+            const int x = e % S::NX, y = (e / S::NX) % S::NY, z = e / (S::NX * S::NY);
+            const int64_t cx = c0[0] + x, cy = c0[1] + y, cz = c0[2] + z;
+            if (cx < lo[0] || cx > hi[0] || cy < lo[1] || cy > hi[1] || cz < lo[2] || cz > hi[2])
+                continue;
+            const int64_t ci = cx + g.nstride[1] * cy + g.nstride[2] * cz;
+            TileHead t;
+            bool ok = true;
+            for (int kk = 0; kk < NI && ok; ++kk)
+                ok = tile_head_at<NI>(g, vals + kk * g.nnodes, ci, kk, t);
+            if (ok) {
+                const int64_t dx = cx - m[0], dy = cy - m[1], dz = cz - m[2];
+                atomicMin(&best, (int)((dx * dx + dy * dy + dz * dz) * 1024 + e));
+            }
         }
+        __syncthreads();
+        const int e = best;
+        if (e != INT32_MAX) {
+            const int r = e & 1023;
+            m[0] = c0[0] + r % S::NX;
+            m[1] = c0[1] + (r / S::NX) % S::NY;
+            m[2] = c0[2] + r / (S::NX * S::NY);
+        }
+        if (k < NI) {
+            const int64_t mi = m[0] + g.nstride[1] * m[1] + g.nstride[2] * m[2];
+            if (!tile_head_at<NI>(g, vals + k * g.nnodes, mi, k, h)) tile_head_zero<NI>(h, k);
+        }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h.m[a] = (int32_t)(m[a] - c0[a]);
+        h.pad = 0;
     }
     __syncthreads();
     if (threadIdx.x == 0) heads[b] = h;
@@ -837,14 +920,18 @@ k_build_tiles3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals,
         const int x = e % S::NX, y = (e / S::NX) % S::NY, z = e / (S::NX * S::NY);
         const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z;
         // records past the tile or nodes past the grid (used only by cells
-        // outside the grid) are 0
+        // outside the grid) are 0; non-finite node values are NaN
         const bool in = e < S::NT && nx < g.n[0] && ny < g.n[1] && nz < g.n[2];
         const int64_t src = in ? nx + g.nstride[1] * ny + g.nstride[2] * nz : 0;
         const double px = x - h.m[0], py = y - h.m[1], pz = z - h.m[2];
         float d[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int k = 0; k < NI; ++k)
-            d[k] = in ? (float)(vals[k * g.nnodes + src] - tile_model<NI>(h, k, px, py, pz)) : 0.f;
+        for (int kk = 0; kk < NI; ++kk) {
+            const double f = vals[kk * g.nnodes + src];
+            d[kk] = !in ? 0.f
+                        : isfinite(f) ? (float)(f - tile_model<NI>(h, kk, px, py, pz))
+                                      : __builtin_nanf("");
+        }
         if constexpr (NI == 1) tiles[b * S::NTP + e] = d[0];
         else tiles[b * S::NTP + e] = make_float4(d[0], d[1], d[2], d[3]);
     }
@@ -1221,7 +1308,9 @@ size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int
                        sizeof(double) * (size_t)(arbs3::sep3_nodes(bx, by, bz) +
                                                  arbs3::sep3_t1(bx, by, bz) +
                                                  arbs3::sep3_t2(bx, by, bz));
-    return lds <= 160 * 1024 ? lds : 0;            // gfx950: 160 KiB of LDS per workgroup
+    // gfx950: 160 KiB of LDS per workgroup, with room for the kernel's static
+    // LDS (one byte per cell)
+    return lds + 256 <= 160 * 1024 ? lds : 0;
 }
 
 hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
