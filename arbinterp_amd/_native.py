@@ -71,6 +71,7 @@ SIGNATURES = [
     ("arb_field_set_boundary", c_i32, [c_vp, c_i32, c_vp]),
     ("arb_build_coeffs", c_i32, [c_vp, c_vp]),
     ("arb_query", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
+    ("arb_query_noalloc", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     ("arb_query_point", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     ("arb_table_copy", c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     ("arb_table_load", c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp]),

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -368,6 +369,40 @@ static int interp_count(int mode, int* comps, int* norm) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free query (arb_query_noalloc): one fixed array
+// per GPU, resident from the library's load until the process ends.  Host
+// code never uses the name `scratch` as an address (on the host it is only a
+// stand-in for the device symbol): the device address of each GPU's copy is
+// looked up once per device with hipGetSymbolAddress and kept in
+// g_scratch_dev.  Every call on one GPU shares that GPU's array, so calls
+// that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[ARB_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of the current device's scratch (looked up on the first
+// call on that device).
+static int scratch_address(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return fail(ARB_EINVAL, "device %d is beyond the %d devices the scratch table holds", dev,
+                    kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(ARB_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return ARB_OK;
+}
+
 extern "C" {
 
 const char* arb_last_error(void) { return g_err.c_str(); }
@@ -409,6 +444,15 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     if (!f) return fail(ARB_ENOMEM, "host allocation failed");
     struct Guard { arb_field* f; ~Guard() { if (f) arb_field_destroy(f); } } guard{f};
     HIPCHK(hipGetDevice(&f->device));
+    // set-up of the allocation-free query on this device: the first lookup of
+    // the scratch array's address makes the runtime load the code object that
+    // holds it (its storage is committed then), so it happens here, in a call
+    // that allocates anyway, and never inside arb_query_noalloc
+    {
+        unsigned char* sp = nullptr;
+        int rc = scratch_address(f->device, &sp);
+        if (rc) return rc;
+    }
     f->mode = mode;
     f->precision = precision;
     f->ninterp = interp_count(mode, &f->has_comps, &f->has_norm);
@@ -652,18 +696,72 @@ static int ensure_tiles(arb_field* f, hipStream_t s) {
     return ARB_OK;
 }
 
-int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
-              void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
-              void* stream) {
-    if (!f) return fail(ARB_EINVAL, "arb_query: NULL handle");
+// ---------------------------------------------------------------------------
+// Per-call workspace of the bucketed query path: the radix sort's key and
+// row-index buffers (two each), its tile histograms and their scans, and the
+// bucket bounds (`first` positions, then `offs`).  Carved from one base
+// address, each piece 256-byte aligned; with base == nullptr only the size
+// is computed.
+// ---------------------------------------------------------------------------
+struct SortWs {
+    uint32_t* k[2] = {nullptr, nullptr};
+    int32_t* v[2] = {nullptr, nullptr};
+    int32_t* hist = nullptr;
+    int32_t* hscan = nullptr;
+    int32_t* first = nullptr;
+    int32_t* offs = nullptr;
+};
+
+static int64_t carve_sort_ws(unsigned char* base, int64_t n, int64_t nbt, SortWs* w) {
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) -> void* {
+        void* p = base ? (void*)(base + off) : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const int64_t hl = 256 * arb_rs_ntiles(n);
+    for (int k = 0; k < 2; ++k) {
+        w->k[k] = static_cast<uint32_t*>(take(sizeof(uint32_t) * n));
+        w->v[k] = static_cast<int32_t*>(take(sizeof(int32_t) * n));
+    }
+    w->hist = static_cast<int32_t*>(take(sizeof(int32_t) * hl));
+    w->hscan = static_cast<int32_t*>(take(sizeof(int32_t) * (hl + 256)));
+    w->first = static_cast<int32_t*>(take(sizeof(int32_t) * arb_rs_first_elems(nbt)));
+    w->offs = static_cast<int32_t*>(take(sizeof(int32_t) * (nbt + 1)));
+    return off;
+}
+
+// arb_query and arb_query_noalloc.  With noalloc the call allocates nothing:
+// the table or node tiles must already be allocated (they are rebuilt in
+// place when the node values changed), the bucketed path takes its sort
+// workspace from `scratch` and writes the output arrays directly (no packed
+// record workspace), and flags that need other workspaces are refused.
+static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
+                      void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
+                      void* stream, bool noalloc) {
+    const char* fn = noalloc ? "arb_query_noalloc" : "arb_query";
+    if (!f) return fail(ARB_EINVAL, "%s: NULL handle", fn);
     if (n < 0 || (n > 0 && (!q_dev || ldq < f->g.ndim)))
-        return fail(ARB_EINVAL, "arb_query: bad positions (n=%lld ldq=%lld)", (long long)n,
+        return fail(ARB_EINVAL, "%s: bad positions (n=%lld ldq=%lld)", fn, (long long)n,
                     (long long)ldq);
     const bool packed = (flags & ARB_Q_PACKED) != 0;
     if (n > 0 && (f->has_comps || packed) && !comps_dev)
-        return fail(ARB_EINVAL, "arb_query: comps is NULL");
+        return fail(ARB_EINVAL, "%s: comps is NULL", fn);
     if (n > 0 && !packed && f->has_norm && (!norms_dev || !grads_dev))
-        return fail(ARB_EINVAL, "arb_query: norms/grads is NULL");
+        return fail(ARB_EINVAL, "%s: norms/grads is NULL", fn);
+    if (noalloc) {
+        if (flags & (ARB_Q_TIMING | ARB_Q_GROUP_ATOMIC))
+            return fail(ARB_EINVAL, "%s: ARB_Q_TIMING and ARB_Q_GROUP_ATOMIC need workspaces "
+                        "of their own and are not available here", fn);
+        if (n > ARB_NOALLOC_MAX_ROWS)
+            return fail(ARB_EINVAL, "%s: %lld rows exceed the limit of %d per call", fn,
+                        (long long)n, ARB_NOALLOC_MAX_ROWS);
+        int dev = -1;
+        HIPCHK(hipGetDevice(&dev));
+        if (dev != f->device)
+            return fail(ARB_EINVAL, "%s: the field lives on device %d, device %d is current", fn,
+                        f->device, dev);
+    }
     hipStream_t s = (hipStream_t)stream;
     bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
     // automatic choice: the bucketed path from about one query per four cells
@@ -689,6 +787,14 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                        !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC));
     const bool nodeform = nodes && sorted;
     const ArbBuckets& BB = fused ? f->BF : f->B;
+    if (noalloc) {
+        if (nodes && !(f->tiles && f->tiles_bytes == arb_tiles_bytes(f->B, f->ninterp, nullptr)))
+            return fail(ARB_EINVAL, "%s: the field's node tiles are not allocated; prepare the "
+                        "field with one arb_query call with the same flags (n may be 0)", fn);
+        if (!fused && !nodes && !f->table)
+            return fail(ARB_EINVAL, "%s: the field's coefficient table is not allocated; "
+                        "prepare the field with arb_build_coeffs", fn);
+    }
     int rc = (fused || nodes) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (nodes) {
@@ -697,7 +803,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
     }
 
     if (n == 0) return ARB_OK;
-    if (flags & ARB_Q_CHECK) HIPCHK(hipMemsetAsync(f->flag + 1, 0, sizeof(int32_t), s));
+    if (flags & ARB_Q_CHECK) HIPCHK(arb_launch_fill_i32(f->flag + 1, 1, 0, s));
     ArbOut o{};
     o.comps = comps_dev;
     o.norms = norms_dev;
@@ -728,8 +834,29 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
             HIPCHK(hipEventElapsedTime(&f->last_ms[3], f->ev[0], f->ev[4]));
         }
     } else {
-        rc = ensure_workspace(f, n, BB.nbt);
-        if (rc) return rc;
+        SortWs w;
+        if (noalloc) {
+            unsigned char* base = nullptr;
+            rc = scratch_address(f->device, &base);
+            if (rc) return rc;
+            const int64_t need = carve_sort_ws(nullptr, n, BB.nbt, &w);
+            if (need > (int64_t)ARB_NOALLOC_SCRATCH_BYTES)
+                return fail(ARB_EINVAL, "%s: %lld rows over %lld buckets need %lld bytes of "
+                            "scratch; the limit is %lld", fn, (long long)n, (long long)BB.nbt,
+                            (long long)need, (long long)ARB_NOALLOC_SCRATCH_BYTES);
+            carve_sort_ws(base, n, BB.nbt, &w);
+        } else {
+            rc = ensure_workspace(f, n, BB.nbt);
+            if (rc) return rc;
+            for (int k = 0; k < 2; ++k) {
+                w.k[k] = f->rs_k[k];
+                w.v[k] = f->rs_v[k];
+            }
+            w.hist = f->rs_hist;
+            w.hscan = f->rs_hscan;
+            w.first = f->ws_count;
+            w.offs = f->ws_offs;
+        }
         // separate output arrays of a 3D field: the bucketed kernels write
         // each row as one packed record into the workspace, and k_unpack
         // splits the records into the arrays in row order (random
@@ -740,16 +867,17 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         // and grads, comps alone, and 4D (whose evaluation hides the writes:
         // 2e6 rows, eval time unchanged, +33 us of split) gain nothing.
         const bool wide = f->precision == ARB_PREC_FP64 || out64;
-        const bool split = !packed && f->g.ndim == 3 && f->has_norm && (f->has_comps || wide);
+        const bool split =
+            !noalloc && !packed && f->g.ndim == 3 && f->has_norm && (f->has_comps || wide);
         if (split) {
-            const int32_t w = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
+            const int32_t pw = arb_packed_width(f->has_comps, f->has_norm, f->g.ndim);
             const int64_t esz = wide ? (int64_t)sizeof(double) : (int64_t)sizeof(float);
-            rc = ensure_records(f, n * w * esz);
+            rc = ensure_records(f, n * pw * esz);
             if (rc) return rc;
             o.comps = f->ws_rec;
             o.norms = nullptr;
             o.grads = nullptr;
-            o.packed = w;
+            o.packed = pw;
         }
         const int32_t* perm = f->ws_perm;
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
@@ -783,34 +911,34 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
                                                  esz * (size_t)f->g.nc * f->ninterp *
                                                      ((size_t)1 << (2 * f->g.ndim)));
             HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, sentrow, f->has_comps,
-                                   f->has_norm, q_dev, n, ldq, o, f->rs_k[0], cellbits,
-                                   f->rs_hist, s));
+                                   f->has_norm, q_dev, n, ldq, o, w.k[0], cellbits, w.hist, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
             // the sort also leaves each bucket's first sorted position in
-            // ws_offs (ws_count as scratch)
-            HIPCHK(arb_launch_radix_sort(f->rs_k[0], f->rs_v[0], f->rs_k[1], f->rs_v[1], n, bits,
-                                         f->rs_hist, f->rs_hscan, 1, 1, f->ws_offs, BB.nbt,
-                                         cellbits, f->ws_count, &in_first, s));
+            // w.offs (w.first as scratch)
+            HIPCHK(arb_launch_radix_sort(w.k[0], w.v[0], w.k[1], w.v[1], n, bits, w.hist,
+                                         w.hscan, 1, 1, w.offs, BB.nbt, cellbits, w.first,
+                                         &in_first, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
-            perm = f->rs_v[in_first ? 0 : 1];
+            perm = w.v[in_first ? 0 : 1];
         }
+        const int32_t* offs = w.offs;
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));
         if (flags & ARB_Q_SORTED_GLOBAL)
             HIPCHK(arb_launch_eval_perm(f->g, f->B, f->precision, out64, f->table, f->ninterp,
-                                        f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
+                                        f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
                                         perm, s));
         else if (nodeform)
             HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, q_dev, n, ldq, o,
-                                         f->ws_offs, perm, f->eval_threads, s));
+                                         offs, perm, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
-                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, f->ws_offs,
+                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
                                          perm, f->eval_threads, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
-                                          f->ws_offs, perm, f->eval_threads, s));
+                                          offs, perm, f->eval_threads, s));
         if (split)
             HIPCHK(arb_launch_unpack(f->g.ndim, f->precision == ARB_PREC_FP64 || out64,
                                      f->has_comps, f->has_norm, f->ws_rec, comps_dev, norms_dev,
@@ -829,6 +957,20 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_d
         if (h) return fail(ARB_EINDEX, "a query reduced to a cell index past the sentinel");
     }
     return ARB_OK;
+}
+
+int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
+              void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
+              void* stream) {
+    return query_impl(f, q_dev, n, ldq, comps_dev, norms_dev, grads_dev, cells_dev, flags, stream,
+                      false);
+}
+
+int arb_query_noalloc(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void* comps_dev,
+                      void* norms_dev, void* grads_dev, int64_t* cells_dev, int32_t flags,
+                      void* stream) {
+    return query_impl(f, q_dev, n, ldq, comps_dev, norms_dev, grads_dev, cells_dev, flags, stream,
+                      true);
 }
 
 // Layout of the single-point record, in doubles: point, components, norm,

@@ -178,6 +178,34 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
               void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
               int32_t flags, void* stream);
 
+/* ---- allocation-free query ------------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, flags and results as arb_query (the same
+ * kernels: outputs are bitwise those of arb_query), but no call allocates:
+ *   - the bucketed path's per-call workspace (radix-sort keys and row
+ *     indices, digit histograms, bucket bounds) is carved from one fixed
+ *     device array of ARB_NOALLOC_SCRATCH_BYTES per GPU, held from the
+ *     library's load until the process ends; separate output arrays are
+ *     written directly (no packed-record workspace);
+ *   - the field must already hold its coefficient table (table paths:
+ *     arb_build_coeffs) or node tiles (mixed 3D: one arb_query with the same
+ *     flags, n may be 0); they are rebuilt in place after
+ *     arb_field_set_values.  Otherwise ARB_EINVAL;
+ *   - n <= ARB_NOALLOC_MAX_ROWS, and the workspace of n rows over the field's
+ *     buckets must fit the array (about 16 bytes per row plus 8 per bucket),
+ *     else ARB_EINVAL with the sizes in the message;
+ *   - ARB_Q_TIMING and ARB_Q_GROUP_ATOMIC are refused (ARB_EINVAL);
+ *   - the field's device must be current.
+ * Every call on one GPU shares that GPU's array and leaves its contents
+ * behind (each call writes what it reads first), so calls that overlap on
+ * one GPU -- on different streams, threads or handles -- race: order them. */
+#define ARB_NOALLOC_SCRATCH_BYTES (48u << 20)
+#define ARB_NOALLOC_MAX_ROWS (1 << 21)
+int arb_query_noalloc(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
+                      void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
+                      int32_t flags, void* stream);
+
 /* One point, host memory in and out: pt (d FP64); comps (3), norm (1), grad
  * (d), all FP64, for the outputs the mode has (else nullable); cell (int64,
  * nullable) = the flat cell index.  The point and the results pass through a
