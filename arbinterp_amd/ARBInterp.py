@@ -59,6 +59,33 @@ def _stream_handle(torch):
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _check_out(torch, t, name, shape, dtype, align=8):
+    """A caller-supplied output buffer must be a contiguous CUDA tensor of the
+    given shape and dtype (the kernels write whole rows through its pointer);
+    `align` is the byte alignment of its data (16 for packed records)."""
+    if t is None:
+        raise ValueError("out: %s buffer is missing" % name)
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError("out: %s must be a CUDA tensor" % name)
+    if t.dtype != dtype:
+        raise ValueError("out: %s must be %s, got %s" % (name, dtype, t.dtype))
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError("out: %s must have shape %s, got %s" % (name, tuple(shape),
+                                                                 tuple(t.shape)))
+    if not t.is_contiguous():
+        raise ValueError("out: %s must be contiguous" % name)
+    if t.data_ptr() % align:
+        raise ValueError("out: %s must be %d-byte aligned" % (name, align))
+
+
+def _check_positions(torch, q, d, what):
+    if not isinstance(q, torch.Tensor) or q.dtype != torch.float64 or not q.is_cuda \
+            or q.dim() != 2 or q.shape[1] < d or (q.shape[0] > 1 and q.stride(1) != 1) \
+            or q.stride(0) < d:
+        raise ValueError("%s needs a CUDA float64 tensor (n x >= %d) with contiguous rows"
+                         % (what, d))
+
+
 class _Interp:
     """Shared implementation of the 3D and 4D classes."""
 
@@ -99,6 +126,10 @@ class _Interp:
         self._kind = kind
         self._variant = {"scalar": 2, "vector": 1, "norm": 2, "both": 3}.get(kind, 1)
         self._field_src = field
+        # a float32 field queried with float32 positions reduces in float32
+        # (numpy promotion in the reference, ARBInterp.py:358-373)
+        self._field_f32 = (field.dtype == torch.float32) if is_tensor else \
+            (np.asarray(field).dtype == np.float32)
         if is_tensor:
             rows = field.detach().to(device="cuda", dtype=torch.float64).contiguous()
         else:
@@ -275,13 +306,22 @@ class _Interp:
         `out` = a tuple returned by an earlier call of the same size reuses
         those buffers (no allocation)."""
         torch = _torch()
-        if q.dtype != torch.float64 or not q.is_cuda or q.stride(1) != 1:
-            raise ValueError("query_device needs a CUDA float64 tensor with contiguous rows")
+        _check_positions(torch, q, self.NDIM, "query_device")
         n = q.shape[0]
         dt = torch.float64 if (self.precision == "fp64" or out_f64) else torch.float32
         if out is not None:
+            if len(out) < (4 if cells else 3):
+                raise ValueError("out must hold (comps, norms, grads%s)"
+                                 % (", cells" if cells else ""))
             comps, norms, grads = out[:3]
-            cel = out[3] if (cells and len(out) > 3) else None
+            cel = out[3] if cells else None
+            if self._info.has_comps:
+                _check_out(torch, comps, "comps", (n, 3), dt)
+            if self._info.has_norm:
+                _check_out(torch, norms, "norms", (n,), dt)
+                _check_out(torch, grads, "grads", (n, self.NDIM), dt)
+            if cells:
+                _check_out(torch, cel, "cells", (n,), torch.int64)
         else:
             comps, norms, grads = self._alloc_out(torch, n, dt)
             cel = torch.empty((n,), dtype=torch.int64, device="cuda") if cells else None
@@ -299,11 +339,12 @@ class _Interp:
         record of `packed_width` elements, [comps(3)] [value, grad(d)], zero
         padded.  Returns the (n, packed_width) tensor; `unpack` gives views."""
         torch = _torch()
-        if q.dtype != torch.float64 or not q.is_cuda or q.stride(1) != 1:
-            raise ValueError("query_packed needs a CUDA float64 tensor with contiguous rows")
+        _check_positions(torch, q, self.NDIM, "query_packed")
         n = q.shape[0]
         W = int(self._info.packed_width)
         dt = torch.float64 if (self.precision == "fp64" or out_f64) else torch.float32
+        if out is not None:
+            _check_out(torch, out, "records", (n, W), dt, align=16)
         rec = out if out is not None else torch.empty((n, W), dtype=dt, device="cuda")
         if not flags & (N.ARB_Q_DIRECT | N.ARB_Q_SORTED | N.ARB_Q_SORTED_GLOBAL):
             flags |= self.default_flags
@@ -347,6 +388,8 @@ class _Interp:
         cells = torch.empty((n,), dtype=torch.int64, device="cuda")
         flags = (N.ARB_Q_CHECK | (N.ARB_Q_OUT_F64 if self.precision == "mixed" else 0)
                  | self.default_flags)
+        if self._field_f32 and query.dtype == np.float32:
+            flags |= N.ARB_Q_REDUCE_F32
         rc = self._run(torch, q_dev, n, d, comps, norms, grads, cells, flags)
         self.queryInds = cells.cpu().numpy()
         if rc == N.ARB_EINDEX:
@@ -376,6 +419,8 @@ class _Interp:
             raise IndexError("index -9223372036854775808 is out of bounds for axis 0 with size %d"
                              % (self.nc + 1))
         torch = _torch()
+        if self._field_f32 and np.asarray(query).dtype == np.float32:
+            return self._point_f32(torch, pt)
         if getattr(self, "_pt_out", None) is None:
             # comps(3), norm, grad(4) FP64 and the cell, reused by every call
             self._pt_out = np.zeros(8, dtype=np.float64)
@@ -401,6 +446,32 @@ class _Interp:
         if self._variant == 2:
             return nv, gv
         return out[0:3].copy(), nv, gv
+
+    def _point_f32(self, torch, pt):
+        """A float32 point of a float32 field: the float32 reduction of the
+        batch path on one row (the reference's sQuery reduces in float32 too;
+        it also rounds the basis products to float32, which this path does
+        not, a difference of order 1e-7 relative)."""
+        d = self.NDIM
+        q_dev = torch.from_numpy(pt[None, :]).to("cuda")
+        comps, norms, grads = self._alloc_out(torch, 1, torch.float64)
+        cells = torch.empty((1,), dtype=torch.int64, device="cuda")
+        flags = N.ARB_Q_CHECK | N.ARB_Q_REDUCE_F32 | (N.ARB_Q_OUT_F64 if self.precision == "mixed"
+                                                      else 0)
+        rc = self._run(torch, q_dev, 1, d, comps, norms, grads, cells, flags)
+        self.queryInd = int(cells.cpu().numpy()[0])
+        if rc == N.ARB_EINDEX:
+            raise IndexError("index %d is out of bounds for axis 0 with size %d"
+                             % (self.queryInd, self.nc + 1))
+        N.check(rc, "query")
+        c = comps.cpu().numpy()[0] if comps is not None else None
+        if self._variant == 1:
+            return c
+        nv = np.float64(norms.cpu().numpy()[0])
+        gv = grads.cpu().numpy()[0]
+        if self._variant == 2:
+            return nv, gv
+        return c, nv, gv
 
     def Query(self, query):
         """Query1/2/3 (ARBInterp.py:177-211 / :880-914): 2-D input with more

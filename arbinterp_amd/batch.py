@@ -13,7 +13,7 @@ import ctypes
 import torch
 
 from . import _native as N
-from .ARBInterp import _stream_handle
+from .ARBInterp import _check_out, _check_positions, _stream_handle
 
 
 def _ptr(t):
@@ -45,8 +45,13 @@ class FieldBatch:
     def query(self, q, counts, out_f64=False, check=False, out=None):
         """q: CUDA float64 tensor (sum(counts) x >= d), field k's rows after
         field k-1's.  Returns (comps, norms, grads) tensors for all rows."""
+        if len(counts) != len(self.interps) or any(int(x) < 0 for x in counts):
+            raise ValueError("counts must hold one non-negative count per field")
         n = int(sum(counts))
         d = self.ndim
+        _check_positions(torch, q, d, "FieldBatch.query")
+        if q.shape[0] < n:
+            raise ValueError("q has %d rows, the counts need %d" % (q.shape[0], n))
         dt = torch.float64 if (self.precision == "fp64" or out_f64) else torch.float32
         if out is None:
             comps = torch.empty((n, 3), dtype=dt, device="cuda") if self.has_comps else None
@@ -54,6 +59,11 @@ class FieldBatch:
             grads = torch.empty((n, d), dtype=dt, device="cuda") if self.has_norm else None
         else:
             comps, norms, grads = out
+            if self.has_comps:
+                _check_out(torch, comps, "comps", (n, 3), dt)
+            if self.has_norm:
+                _check_out(torch, norms, "norms", (n,), dt)
+                _check_out(torch, grads, "grads", (n, d), dt)
         c = (ctypes.c_int64 * len(counts))(*[int(x) for x in counts])
         flags = (N.ARB_Q_OUT_F64 if out_f64 else 0) | (N.ARB_Q_CHECK if check else 0)
         N.check(N.lib().arb_batch_query(self._h, _ptr(q), c, q.stride(0), _ptr(comps),

@@ -227,6 +227,8 @@ struct arb_field {
     // writes in place (host address, and the same memory as the GPU sees it)
     double* pin = nullptr;
     double* pin_dev = nullptr;
+    // batches holding this field (destroying the field invalidates them)
+    std::vector<arb_batch*> batches;
 };
 
 // Choose the cell block of one bucket: the largest of a fixed ladder whose
@@ -763,6 +765,11 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                         f->device, dev);
     }
     hipStream_t s = (hipStream_t)stream;
+    // float32 field and positions: float32 reduction on the direct table kernel
+    const bool red32 = (flags & ARB_Q_REDUCE_F32) != 0;
+    if (red32) flags = (flags | ARB_Q_DIRECT | ARB_Q_TABLE) & ~(ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL);
+    ArbGeom gq = f->g;
+    gq.red32 = red32 ? 1 : 0;
     bool sorted = (flags & (ARB_Q_SORTED | ARB_Q_SORTED_GLOBAL)) != 0;
     // automatic choice: the bucketed path from about one query per four cells
     // with node tiles (it stages ~128 bytes per cell), from one per two cells
@@ -822,7 +829,11 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
     for (float& m : f->last_ms) m = 0.f;
     if (!sorted) {
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
-        if (nodes)
+        if (red32)
+            HIPCHK(arb_launch_query_direct_red32(gq, f->precision, out64, f->table, f->ninterp,
+                                                 f->has_comps, f->has_norm, q_dev, n, ldq, o,
+                                                 s));
+        else if (nodes)
             HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, out64, f->tiles, f->sent,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o, s));
         else
@@ -1169,6 +1180,8 @@ int arb_push_rk4(arb_field* f, double* x_dev, double* v_dev, int64_t n, double d
 // ---------------------------------------------------------------------------
 struct arb_batch {
     std::vector<arb_field*> fields;
+    std::vector<const void*> tables;   // each field's table as the descriptors hold it
+    int32_t dead = 0;                  // a field of the batch was destroyed
     int32_t ndim = 3, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
     int32_t chunk = 256, threads = 256, blocks_per_cu = 4, ncu = 256;
     std::vector<int64_t> last_counts;
@@ -1202,6 +1215,7 @@ int arb_batch_create(arb_field* const* fields, int32_t nfields, void* stream, ar
         int rc = arb_build_coeffs(f, stream);
         if (rc) return rc;
         b->fields.push_back(f);
+        b->tables.push_back(f->table);
     }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -1213,6 +1227,9 @@ int arb_batch_create(arb_field* const* fields, int32_t nfields, void* stream, ar
     HIPCHK(hipMalloc(&b->flag, sizeof(int32_t)));
     HIPCHK(hipMemsetAsync(b->flag, 0, sizeof(int32_t), (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (arb_field* f : b->fields)
+        if (std::find(f->batches.begin(), f->batches.end(), b) == f->batches.end())
+            f->batches.push_back(b);
     guard.b = nullptr;
     *out = b;
     return ARB_OK;
@@ -1237,8 +1254,21 @@ int arb_batch_query(arb_batch* b, double* q_dev, const int64_t* counts, int64_t 
                     void* comps_dev, void* norms_dev, void* grads_dev, int32_t flags,
                     void* stream) {
     if (!b || !counts) return fail(ARB_EINVAL, "arb_batch_query: NULL argument");
+    if (b->dead) return fail(ARB_EINVAL, "arb_batch_query: a field of the batch was destroyed");
     const int nf = (int)b->fields.size();
     hipStream_t s = (hipStream_t)stream;
+    // tables made stale by arb_field_set_values are rebuilt first; a field
+    // whose table moved gets its descriptor written again
+    bool moved = false;
+    for (int k = 0; k < nf; ++k) {
+        int rc = arb_build_coeffs(b->fields[k], stream);
+        if (rc) return rc;
+        if (b->fields[k]->table != b->tables[k]) {
+            b->tables[k] = b->fields[k]->table;
+            moved = true;
+        }
+    }
+    if (moved) b->last_counts.clear();
     std::vector<int64_t> cnt(counts, counts + nf);
     int64_t total = 0;
     for (int64_t c : cnt) {
@@ -1283,6 +1313,9 @@ int arb_batch_query(arb_batch* b, double* q_dev, const int64_t* counts, int64_t 
 
 void arb_batch_destroy(arb_batch* b) {
     if (!b) return;
+    for (arb_field* f : b->fields)
+        if (f) f->batches.erase(std::remove(f->batches.begin(), f->batches.end(), b),
+                                f->batches.end());
     if (b->desc_dev) (void)hipFree(b->desc_dev);
     if (b->counter) (void)hipFree(b->counter);
     if (b->flag) (void)hipFree(b->flag);
@@ -1361,6 +1394,11 @@ int arb_table_load(arb_field* f, int64_t cell0, int64_t ncells, const void* src_
 
 void arb_field_destroy(arb_field* f) {
     if (!f) return;
+    for (arb_batch* b : f->batches) {
+        b->dead = 1;
+        for (auto& bf : b->fields)
+            if (bf == f) bf = nullptr;
+    }
     if (f->vals) (void)hipFree(f->vals);
     if (f->tiles) (void)hipFree(f->tiles);
     if (f->table) (void)hipFree(f->table);

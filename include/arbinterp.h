@@ -101,6 +101,13 @@ extern "C" {
                                agree with the table path to FP32 rounding,
                                not bitwise)                                   */
 
+#define ARB_Q_REDUCE_F32 4096 /* the field and the positions were float32 in the
+                               caller (the values here are their exact FP64
+                               widening): reduce cells in float32 and form the
+                               power basis with float32 powers, as numpy does
+                               for those dtypes (ARBInterp.py:358-373,
+                               :493-503).  Uses the direct table kernel      */
+
 /* ---- boundary extension (arb_field_set_boundary) ----------------------- */
 #define ARB_BC_NONE      0  /* the reference's box: [2nd node, 2nd-last node]   */
 #define ARB_BC_NEUMANN   1  /* ghost nodes by even reflection: zero normal slope
@@ -246,7 +253,10 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
 /* ---- many fields, one launch ----------------------------------------------
  * No reference counterpart (the reference queries one field per call).  All
  * fields of a batch share ndim, mode and precision; their tables are built at
- * creation.  The batch keeps pointers to the fields, which must outlive it. */
+ * creation, and rebuilt by a batch query after arb_field_set_values.  The
+ * batch keeps pointers to the fields: destroying one of them makes every
+ * later arb_batch_query of the batch return ARB_EINVAL (destroy the batch
+ * with arb_batch_destroy, in any order). */
 typedef struct arb_batch arb_batch;
 
 int arb_batch_create(arb_field* const* fields, int32_t nfields, void* stream, arb_batch** out);

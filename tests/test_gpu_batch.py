@@ -112,3 +112,74 @@ def test_rk4_mixed_vs_fp64(mods):
     print("mixed vs fp64 positions after 200 steps: max |dx| %.2e" % dev)
     assert np.array_equal(np.isnan(out["fp64"]), np.isnan(out["mixed"]))
     assert dev < 1e-5
+
+
+def test_batch_follows_new_values_and_dead_fields(mods):
+    """New node values (arb_field_set_values) reach the next batch query:
+    doubling every node doubles every output bit for bit.  Destroying a
+    field of the batch makes later batch queries raise instead of reading
+    freed memory."""
+    import ctypes
+    torch, G, B = mods
+    from arbinterp_amd import _native as NN
+    fields = W.multi_field(seed=6, nfields=3, edge_lo=10, edge_hi=20, q_lo=500, q_hi=2000)
+    interps = [G.tricubic(r, "quiet", precision="fp64") for r, _ in fields]
+    counts = [len(q) for _, q in fields]
+    qd = torch.from_numpy(np.concatenate([q for _, q in fields])).cuda()
+    bt = B.FieldBatch(interps)
+    _, n0, g0 = bt.query(qd, counts)
+    n0, g0 = n0.clone(), g0.clone()
+    t = interps[1]
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    v = torch.empty((t._info.nnodes,), dtype=torch.float64, device="cuda")
+    NN.check(NN.lib().arb_field_values(t._h, ctypes.c_void_p(v.data_ptr()), st))
+    v2 = 2.0 * v
+    NN.check(NN.lib().arb_field_set_values(t._h, ctypes.c_void_p(v2.data_ptr()), st))
+    _, n1, g1 = bt.query(qd, counts)
+    a, b = counts[0], counts[0] + counts[1]
+    assert _eq(n1[:a], n0[:a]) and _eq(n1[b:], n0[b:])
+    assert _eq(n1[a:b], 2.0 * n0[a:b]) and _eq(g1[a:b], 2.0 * g0[a:b])
+    interps[2].close()
+    with pytest.raises(ValueError, match="destroyed"):
+        bt.query(qd, counts)
+    bt.close()
+    for k in (0, 1):
+        interps[k].close()
+
+
+def test_out_buffers_are_checked(mods):
+    """Caller-supplied output buffers of the wrong shape, dtype, device or
+    layout are refused before any kernel runs."""
+    torch, G, B = mods
+    rows, q = W.dipole3d(seed=4, n=(14, 13, 12), nq=1000)
+    t = G.tricubic(rows, "quiet", mode="both", precision="mixed")
+    qd = torch.from_numpy(q).cuda()
+    f32 = dict(dtype=torch.float32, device="cuda")
+    good = (torch.empty((1000, 3), **f32), torch.empty((1000,), **f32),
+            torch.empty((1000, 3), **f32))
+    t.query_device(qd, out=good)
+    bad_sets = [
+        (torch.empty((999, 3), **f32), good[1], good[2]),                  # too few rows
+        (good[0], good[1].double(), good[2]),                              # dtype
+        (good[0], good[1], torch.empty((1000, 3), dtype=torch.float32)),   # host memory
+        (good[0], good[1], torch.empty((3, 1000), **f32).t()),             # not contiguous
+        (good[0], None, good[2]),                                          # missing
+    ]
+    for bad in bad_sets:
+        with pytest.raises(ValueError):
+            t.query_device(qd, out=bad)
+    W8 = int(t.field_info().packed_width)
+    with pytest.raises(ValueError):
+        t.query_packed(qd, out=torch.empty((1000, W8), **f32), out_f64=True)
+    base = torch.empty((1000 * W8 + 1,), dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="aligned"):
+        t.query_packed(qd, out=base[1:].view(1000, W8), out_f64=True)
+    with pytest.raises(ValueError):
+        t.query_device(qd.float())
+    bt = B.FieldBatch([t])
+    with pytest.raises(ValueError):
+        bt.query(qd, [1001])
+    with pytest.raises(ValueError):
+        bt.query(qd, [1000], out=(good[0], good[1], torch.empty((1000, 4), **f32)))
+    bt.close()
+    t.close()
