@@ -44,6 +44,7 @@ ARB_Q_PACKED = 256
 ARB_Q_SORT_CELL = 512
 ARB_Q_FUSED = 1024
 ARB_Q_TABLE = 2048
+ARB_Q_REDUCE_F32 = 4096
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64

@@ -20,6 +20,7 @@ struct ArbGeom {
     int64_t nc;               // number of cells
     int64_t nnodes;
     int32_t ndim;
+    int32_t red32;            // reduce in float32 (float32 field and queries)
 };
 
 // Outputs of one query launch.
@@ -64,7 +65,28 @@ struct ArbBuckets {
 // is identical.  Rows outside the box (or with a NaN coordinate) map to the
 // sentinel cell nc with NaN local coordinates.
 // Returns the unclamped flat index; *oob is set for rows strictly outside.
+//
+// With g.red32 (a float32 field queried with float32 positions) the reduction
+// runs in float32, as numpy does for those dtypes (ARBInterp.py:358-373 with
+// float32 IntMin and h, :547-556): iu = fl32(fl32(q - IntMin) / h32), h32 the
+// float32 spacing, ix = floor(iu), local = iu - ix in float32; the flat index
+// is formed from the integer ix exactly as before.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void arb_reduce_axis(double x, const ArbGeom& g, int a, bool out,
+                                                double* iu, double* fl, double* loc) {
+    if (g.red32) {
+        const float uf = out ? NAN : ((float)x - (float)g.lo[a]) / (float)g.h[a];
+        const float ff = floorf(uf);
+        *iu = (double)uf;
+        *fl = (double)ff;
+        *loc = (double)(uf - ff);
+    } else {
+        *iu = out ? NAN : (x - g.lo[a]) / g.h[a];
+        *fl = floor(*iu);
+        *loc = *iu - *fl;
+    }
+}
+
 template <int D>
 __device__ __forceinline__ int64_t arb_reduce(const double* __restrict__ q, const ArbGeom& g,
                                               double* __restrict__ loc, bool* oob) {
@@ -80,9 +102,8 @@ __device__ __forceinline__ int64_t arb_reduce(const double* __restrict__ q, cons
     bool bad = out;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        double iu = out ? NAN : (x[a] - g.lo[a]) / g.h[a];
-        double fl = floor(iu);
-        loc[a] = iu - fl;
+        double iu, fl;
+        arb_reduce_axis(x[a], g, a, out, &iu, &fl, &loc[a]);
         bad |= isnan(iu);
         int64_t ia = bad ? 0 : (int64_t)fl;
         flat += ia * mul;
@@ -109,9 +130,8 @@ __device__ __forceinline__ int64_t arb_reduce_axes(const double* __restrict__ q,
     bool bad = out;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-        double iu = out ? NAN : (x[a] - g.lo[a]) / g.h[a];
-        double fl = floor(iu);
-        loc[a] = iu - fl;
+        double iu, fl;
+        arb_reduce_axis(x[a], g, a, out, &iu, &fl, &loc[a]);
         bad |= isnan(iu);
         ia[a] = bad ? 0 : (int64_t)fl;
         flat += ia[a] * mul;
@@ -325,6 +345,12 @@ hipError_t arb_launch_batch_query(int32_t ndim, int32_t prec, int32_t out_f64, c
 hipError_t arb_launch_rk4(const ArbGeom& g, int32_t prec, const void* table, int32_t ninterp,
                           int32_t kn, double* X, double* V, int64_t n, double dt, int32_t nsteps,
                           hipStream_t s);
+// Direct kernel for float32 fields queried with float32 positions (g.red32):
+// float32 reduction, and the power basis with numpy's float32 powers.
+hipError_t arb_launch_query_direct_red32(const ArbGeom& g, int32_t prec, int32_t out_f64,
+                                         const void* table, int32_t ninterp, int32_t has_comps,
+                                         int32_t has_norm, const double* q, int64_t n,
+                                         int64_t ldq, const ArbOut& o, hipStream_t s);
 hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f64,
                                    const void* table, int32_t ninterp, int32_t has_comps,
                                    int32_t has_norm, const double* q, int64_t n, int64_t ldq,

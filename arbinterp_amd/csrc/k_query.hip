@@ -124,6 +124,103 @@ k_query_direct(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t
 }
 
 // ---------------------------------------------------------------------------
+// Float32 fields queried with float32 positions.  numpy then reduces in
+// float32 (arb_reduce with g.red32) and forms the power basis from the
+// float32 local coordinate with float32 powers before the FP64 sums
+// (ARBInterp.py:493-503: queryCoords**2, **3 and 3*queryCoords**2 are
+// float32 arrays): X = (1, u, fl32(u^2), fl32(u^3)), dX = (0, 1, fl32(2u),
+// fl32(3 fl32(u^2))).  The cube is the float32 nearest to u^3 (numpy's
+// vectorised float32 power is itself within an ulp of it, not always equal).
+// Values are then sum_ijk alpha_ijk X_i Y_j Z_k in FP64, contracted x first.
+// ---------------------------------------------------------------------------
+__device__ float cube_f32_nearest(float u) {
+    const double u2 = (double)u * (double)u;    // exact: 48 significant bits
+    const double c = u2 * (double)u;            // u^3 rounded to FP64
+    const double e = fma(u2, (double)u, -c);    // u^3 - c exactly
+    float r = (float)c;
+    const double dr = (double)r;
+    if (e != 0.0 && c != dr) {
+        // c lies exactly between r and its neighbour: the remainder decides
+        const float nb = c > dr ? nextafterf(r, INFINITY) : nextafterf(r, -INFINITY);
+        if (c - dr == (double)nb - c) r = ((e > 0.0) == (nb > r)) ? nb : r;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void basis_f32(double loc, double* X, double* dX) {
+    const float u = (float)loc;                 // loc holds a float32 value
+    const float u2 = u * u;
+    X[0] = 1.0;
+    X[1] = (double)u;
+    X[2] = (double)u2;
+    X[3] = (double)cube_f32_nearest(u);
+    dX[0] = 0.0;
+    dX[1] = 1.0;
+    dX[2] = (double)(2.0f * u);
+    dX[3] = (double)(3.0f * u2);
+}
+
+// sum over the K coefficients a of a_i * prod_axis b[axis][e_axis]
+template <int D, typename TC>
+__device__ __forceinline__ double contract_basis(const TC* a, const double* const* b) {
+    double out = 0.0;
+#pragma unroll 1
+    for (int l = 0; l < (D == 4 ? 4 : 1); ++l) {
+        double z = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            double y = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const TC* c = a + 64 * l + 16 * k + 4 * j;
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) x = fma((double)c[i], b[0][i], x);
+                y = fma(x, b[1][j], y);
+            }
+            z = fma(y, b[2][k], z);
+        }
+        out = D == 4 ? fma(z, b[3][l], out) : z;
+    }
+    return out;
+}
+
+template <int D, typename TC, typename TO>
+__global__ void __launch_bounds__(256)
+k_query_direct_red32(ArbGeom g, const TC* __restrict__ table, int32_t ninterp,
+                     int32_t has_comps, int32_t has_norm, const double* __restrict__ q,
+                     int64_t n, int64_t ldq, ArbOut o) {
+    constexpr int K = (D == 3) ? 64 : 256;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[D];
+    const int64_t flat = reduce_row<D>(q, ldq, g, o, i, loc);
+    const TC* a = table + flat * (int64_t)ninterp * K;
+    double X[D][4], dX[D][4];
+#pragma unroll
+    for (int d = 0; d < D; ++d) basis_f32(loc[d], X[d], dX[d]);
+    const double* b[4] = {X[0], X[1], X[2], D == 4 ? X[D - 1] : X[0]};
+    TO rec[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) rec[r] = (TO)0;
+    int k = 0;
+    if (has_comps) {
+        for (int c = 0; c < 3; ++c) rec[c] = (TO)contract_basis<D, TC>(a + c * K, b);
+        k = 3;
+    }
+    if (has_norm) {
+        const int r0 = has_comps ? 3 : 0;
+        rec[r0] = (TO)contract_basis<D, TC>(a + k * K, b);
+        for (int d = 0; d < D; ++d) {
+            const double* bd[4] = {b[0], b[1], b[2], b[3]};
+            bd[d] = dX[d];
+            rec[r0 + 1 + d] = (TO)(contract_basis<D, TC>(a + k * K, bd) / g.h[d]);
+        }
+    }
+    store_rec<D, TO>(rec, has_comps, has_norm, o, i);
+}
+
+// ---------------------------------------------------------------------------
 // Cell-bucketed path.  Queries are grouped by the block of cells they fall in
 // (k_keys + the radix sort of k_sort.hip by default; with ARB_Q_GROUP_ATOMIC
 // a counting sort: atomic rank per bucket, one scan, one scatter of the query
@@ -1197,7 +1294,37 @@ hipError_t launch_direct(const ArbGeom& g, int32_t prec, int32_t out_f64, const 
     return hipGetLastError();
 }
 
+template <int D>
+hipError_t launch_direct_red32(const ArbGeom& g, int32_t prec, int32_t out_f64,
+                               const void* table, int32_t ninterp, int32_t has_comps,
+                               int32_t has_norm, const double* q, int64_t n, int64_t ldq,
+                               const ArbOut& o, hipStream_t s) {
+    dim3 grid(grid1(n, 256)), block(256);
+    if (prec == ARB_PREC_FP64)
+        hipLaunchKernelGGL((k_query_direct_red32<D, double, double>), grid, block, 0, s, g,
+                           (const double*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    else if (out_f64)
+        hipLaunchKernelGGL((k_query_direct_red32<D, float, double>), grid, block, 0, s, g,
+                           (const float*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    else
+        hipLaunchKernelGGL((k_query_direct_red32<D, float, float>), grid, block, 0, s, g,
+                           (const float*)table, ninterp, has_comps, has_norm, q, n, ldq, o);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t arb_launch_query_direct_red32(const ArbGeom& g, int32_t prec, int32_t out_f64,
+                                         const void* table, int32_t ninterp, int32_t has_comps,
+                                         int32_t has_norm, const double* q, int64_t n,
+                                         int64_t ldq, const ArbOut& o, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (g.ndim == 3)
+        return launch_direct_red32<3>(g, prec, out_f64, table, ninterp, has_comps, has_norm, q,
+                                      n, ldq, o, s);
+    return launch_direct_red32<4>(g, prec, out_f64, table, ninterp, has_comps, has_norm, q, n,
+                                  ldq, o, s);
+}
 
 hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f64,
                                    const void* table, int32_t ninterp, int32_t has_comps,

@@ -131,14 +131,66 @@ def test_edge_battery_golden(dev, golden):
         compare_outputs(res1, qa1, g, pre + "s_", o, p[None, :], tol=TOL64, label=pre + mode)
 
 
-@pytest.mark.parametrize("tag", ["f64f32", "f32f64"])
+# A float32 field queried with float32 positions: the reference reduces
+# cells in float32 and raises the float32 local coordinates to their powers
+# in float32 (numpy promotion); so does ARB_Q_REDUCE_F32, with the float32
+# cube rounded to nearest.  numpy's own vectorised float32 power differs from
+# the nearest float32 in about one case in five (an ulp of u^3), so values
+# agree to 1e-8 of the condition-floored scale, not to FP64 rounding
+# (measured 6e-10 value, 2e-8 gradient; 5.5e-8 and 2.8e-6 with an FP64
+# reduction).  Cell indices and NaN rows are identical.
+TOL_F32F32 = 1e-7
+
+
+@pytest.mark.parametrize("tag", ["f64f32", "f32f64", "f32f32"])
 def test_dtype_promotion_golden(dev, golden, tag):
     g = golden("dtype_promotion")
     t = dev.tricubic(g[tag + "_field"].copy(), "quiet")
     o = O.tricubic(g[tag + "_field"].copy(), "quiet")
     res, qa = run_query(t, g[tag + "_query"])
-    compare_outputs(res, qa, g, tag + "_", o, g[tag + "_query"], tol=TOL64)
+    compare_outputs(res, qa, g, tag + "_", o, g[tag + "_query"],
+                    tol=TOL_F32F32 if tag == "f32f32" else TOL64)
     assert np.array_equal(t.queryInds, g[tag + "_queryInds"])
+    if tag == "f32f32":
+        # single float32 points take the float32 reduction too (an
+        # out-of-box point makes Query2 raise TypeError in both)
+        q32 = g[tag + "_query"]
+        for r in range(0, len(q32), 37):
+            try:
+                pv = t.Query(q32[r].copy())
+            except TypeError:
+                pv = None
+            try:
+                ov = o.Query(q32[r].copy())
+            except TypeError:
+                ov = None
+            assert (pv is None) == (ov is None), r
+            if pv is not None:
+                assert t.queryInd == o.queryInd == g[tag + "_queryInds"][r], r
+                assert np.allclose(pv[0], ov[0], rtol=1e-6, atol=1e-9), r
+                assert np.allclose(pv[1], ov[1], rtol=1e-5, atol=1e-7), r
+
+
+def test_reduce_f32_at_cell_edges(dev):
+    """Points one float32 ulp below a cell edge: the float32 reduction may
+    round iu up to the edge and give the next cell, as numpy does; the cell
+    index must follow the float32 arithmetic exactly."""
+    import torch
+    from arbinterp_amd import _native as NN
+    rows, _ = W.scalar3d_gaussians(seed=7, n=(12, 11, 10), nq=10)
+    f32 = rows.astype(np.float32)
+    t = dev.tricubic(f32.copy(), "quiet")
+    o = O.tricubic(f32.copy(), "quiet")
+    rng = np.random.default_rng(2)
+    cells = rng.integers(1, 6, size=(400, 3))
+    edge = np.stack([o.p.int_min[a] + cells[:, a] * o.p.h[a] for a in range(3)], axis=1)
+    q = np.nextafter(edge.astype(np.float32), np.float32(-np.inf))
+    q[::2] = edge[::2].astype(np.float32)
+    qd = torch.from_numpy(q.astype(np.float64)).cuda()
+    _, _, _, cl = t.query_device(qd, cells=True, flags=NN.ARB_Q_REDUCE_F32)
+    torch.cuda.synchronize()
+    o.Query(q.copy())
+    assert np.array_equal(cl.cpu().numpy(), o.queryInds)
 
 
 def test_mixed_precision_error_bound(dev):
