@@ -68,3 +68,15 @@ def test_packed_record_layout_documented():
     assert (width(1, 1, 3), width(0, 1, 3), width(1, 1, 4), width(0, 1, 4), width(1, 0, 3)) == \
         (8, 4, 8, 8, 4)
     assert C.sizeof(N.FieldInfo) % 8 == 0
+
+
+def test_reference_import_paths_bind_gpu_classes():
+    """`from ARBInterp import ARBInterp` (the reference's tests/test_ARBInterp.py:1)
+    and `from ARBTools.ARBInterp import tricubic, quadcubic` (its examples)
+    give the GPU classes (importing needs no GPU; constructing one does)."""
+    from ARBInterp import ARBInterp
+    from ARBTools.ARBInterp import quadcubic, tricubic
+    import arbinterp_amd.ARBInterp as G
+    assert ARBInterp.tricubic is G.tricubic is tricubic
+    assert ARBInterp.quadcubic is G.quadcubic is quadcubic
+    assert ARBInterp.__version__ == "1.8"
