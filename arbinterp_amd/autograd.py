@@ -5,7 +5,9 @@ positions for the interpolator's scalar interpolant (scalar mode, or |B| for
 'norm'/'both'), with gradients flowing to
 
   * positions: gv * grad U + Hess U . gg (the analytic Hessian of the
-    interpolant, evaluated by the backward kernel);
+    interpolant, evaluated by the backward kernel; for mixed-precision 3D
+    fields from the node tiles, whose FP64 model carries the cancelling part
+    of the Hessian);
   * values (the stored node values, canonical x-fastest order, shape
     (ninterp, nnodes) or (nnodes,) for one column): the adjoint of the linear
     map values -> (U, grad U), computed deterministically on the GPU
@@ -44,11 +46,10 @@ class InterpFunction(torch.autograd.Function):
         N.check(N.lib().arb_field_set_values(interp._h, _ptr(vals), _stream_handle(torch)),
                 "set_values")
         interp._info.table_built = 0
-        # the backward reads the coefficient table, so the forward reads it
-        # too (ARB_Q_TABLE; mixed 3D queries otherwise use node tiles): one
-        # build per set of values, and the gradients are exactly those of the
-        # values returned
-        _, U, G = interp.query_device(q, out_f64=True, flags=N.ARB_Q_TABLE)
+        # forward and backward read the same coefficient source: node tiles
+        # for mixed 3D fields (an FP64 model per block plus FP32 residuals;
+        # no table is built), the table otherwise; one build per set of values
+        _, U, G = interp.query_device(q, out_f64=True)
         ctx.save_for_backward(q)
         ctx.interp = interp
         ctx.pos_dtype = positions.dtype

@@ -1072,8 +1072,15 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     if (n > INT32_MAX - 1 || f->g.nc > INT32_MAX - 1)
         return fail(ARB_EINVAL, "arb_query_backward: too many queries or cells for int32 indices");
     hipStream_t s = (hipStream_t)stream;
-    int rc = arb_build_coeffs(f, stream);
+    // mixed 3D fields take U's derivatives from the node tiles (FP64 model
+    // plus FP32 residuals), as their forward queries do; others from the table
+    const bool nodes = node_path(f, 0);
+    int rc = nodes ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
+    if (nodes) {
+        rc = ensure_tiles(f, s);
+        if (rc) return rc;
+    }
     const int64_t nc = f->g.nc;
     if (dval_dev) {
         if (n > f->bw_n) {
@@ -1099,7 +1106,11 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
         HIPCHK(arb_launch_fill_i32(f->bw_count, nc + 1, 0, s));
         HIPCHK(arb_launch_fill_i32(f->bw_cursor, nc + 1, 0, s));
     }
-    if (n > 0)
+    if (n > 0 && nodes)
+        HIPCHK(arb_launch_bwd_tiles(f->g, f->B, f->ninterp, f->tiles, q_dev, n, ldq, gv_dev,
+                                    gg_dev, dq_dev, dval_dev ? f->bw_key : nullptr, f->bw_count,
+                                    dval_dev ? f->bw_rec : nullptr, s));
+    else if (n > 0)
         HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
                                      q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
                                      dval_dev ? f->bw_key : nullptr, f->bw_count,

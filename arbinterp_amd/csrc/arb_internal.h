@@ -305,6 +305,12 @@ hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t
                                   int32_t out_f64, const void* tiles, const void* sent,
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, hipStream_t s);
+// Backward of the last interpolant from the node tiles (mixed 3D; ninterp 4
+// or 1): d/dq, and the node adjoint's per-query cell key and record.
+hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                const void* tiles, const double* q, int64_t n, int64_t ldq,
+                                const double* gv, const double* gg, double* dq, int32_t* key,
+                                int32_t* count, double* rec, hipStream_t s);
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                  int32_t out_f64, const void* tiles, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,

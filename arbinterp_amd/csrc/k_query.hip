@@ -1204,6 +1204,146 @@ k_query_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __rest
                                                         lc, loc, i);
 }
 
+// Second derivatives of the Hermite weights (w_o'' for hermite_weights).
+__device__ __forceinline__ void hermite_weights2(float u, float* ddw) {
+    ddw[0] = fmaf(-3.0f, u, 2.0f);
+    ddw[1] = fmaf(9.0f, u, -5.0f);
+    ddw[2] = fmaf(-9.0f, u, 4.0f);
+    ddw[3] = fmaf(3.0f, u, -1.0f);
+}
+
+// Backward of the last interpolant's (value, gradient) outputs from the node
+// tiles (mixed precision, 3D).  Per query: d/dq = gv grad U + Hess U . gg,
+// with grad U and Hess U formed the way the forward forms U from the tiles:
+// the FP64 model's derivatives at the query (plus the interpolant of its
+// x^3, y^3, z^3 terms, phi''(u) = 12u - 6) and the FP32 contractions of the
+// residual records with the Hermite weights w, w' and w''.  The model
+// carries the large, cancelling part of the Hessian in FP64, so only the
+// residuals' rounding reaches it.  Also writes the node adjoint's per-query
+// cell key and record (as k_bwd_points does).
+template <int NI, int BX, int BY, int BZ>
+__global__ void __launch_bounds__(256)
+k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
+             const TileHead* __restrict__ heads, const double* __restrict__ q, int64_t n,
+             int64_t ldq, const double* __restrict__ gv, const double* __restrict__ gg,
+             double* __restrict__ dq, int32_t* __restrict__ key, int32_t* __restrict__ count,
+             double* __restrict__ rec) {
+    using S = TileShape<NI, BX, BY, BZ>;
+    using R = typename TileRec<NI>::type;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[3];
+    int64_t ia[3];
+    bool oob;
+    const int64_t flat = arb_reduce_axes<3>(q + i * ldq, g, loc, ia, &oob);
+    const bool ok = flat < g.nc;
+    if (key) {
+        key[i] = ok ? (int32_t)flat : -1;
+        if (ok) atomicAdd(&count[flat], 1);
+    }
+    if (rec && ok) {
+        double4* r4 = reinterpret_cast<double4*>(rec + i * 8);
+        r4[0] = make_double4(loc[0], loc[1], loc[2], gv[i]);
+        r4[1] = make_double4(gg[i * 3] / g.h[0], gg[i * 3 + 1] / g.h[1], gg[i * 3 + 2] / g.h[2], 0.0);
+    }
+    if (!dq) return;
+    if (!ok) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) dq[i * 3 + x] = NAN;
+        return;
+    }
+    int lc;
+    const int64_t b = bucket_of<3>(flat, ia, g, B, &lc);
+    const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
+    float wx[4], dx[4], ddx[4], wy[4], dy[4], ddy[4], wz[4], dz[4], ddz[4];
+    hermite_weights((float)loc[0], wx, dx);
+    hermite_weights((float)loc[1], wy, dy);
+    hermite_weights((float)loc[2], wz, dz);
+    hermite_weights2((float)loc[0], ddx);
+    hermite_weights2((float)loc[1], ddy);
+    hermite_weights2((float)loc[2], ddz);
+    const R* base = tiles + b * S::NTP + (lz * S::NY + ly) * S::NX + lx;
+    // residual part: value, gradient and Hessian (node units), FP32
+    float gr[3] = {0.f, 0.f, 0.f}, hr[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};   // xx yy zz xy xz yz
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float b00 = 0.f, b10 = 0.f, b01 = 0.f, b20 = 0.f, b11 = 0.f, b02 = 0.f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const R* r = base + (k * S::NY + j) * S::NX;
+            float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                const float v = last_of(r[x]);
+                a0 = fmaf(wx[x], v, a0);
+                a1 = fmaf(dx[x], v, a1);
+                a2 = fmaf(ddx[x], v, a2);
+            }
+            b00 = fmaf(wy[j], a0, b00);
+            b10 = fmaf(wy[j], a1, b10);
+            b01 = fmaf(dy[j], a0, b01);
+            b20 = fmaf(wy[j], a2, b20);
+            b11 = fmaf(dy[j], a1, b11);
+            b02 = fmaf(ddy[j], a0, b02);
+        }
+        gr[0] = fmaf(wz[k], b10, gr[0]);
+        gr[1] = fmaf(wz[k], b01, gr[1]);
+        gr[2] = fmaf(dz[k], b00, gr[2]);
+        hr[0] = fmaf(wz[k], b20, hr[0]);
+        hr[1] = fmaf(wz[k], b02, hr[1]);
+        hr[2] = fmaf(ddz[k], b00, hr[2]);
+        hr[3] = fmaf(wz[k], b11, hr[3]);
+        hr[4] = fmaf(dz[k], b10, hr[4]);
+        hr[5] = fmaf(dz[k], b01, hr[5]);
+    }
+    // model part (FP64) at the query, in tile coordinates
+    const TileHead& h = heads[b];
+    constexpr int L = NI - 1;
+    const double px = (double)(lx + 1 - h.m[0]) + loc[0];
+    const double py = (double)(ly + 1 - h.m[1]) + loc[1];
+    const double pz = (double)(lz + 1 - h.m[2]) + loc[2];
+    double gm[3];
+    tile_model_grad<NI>(h, L, px, py, pz, gm);
+    double H[3][3];
+    H[0][0] = h.hxx;
+    H[1][1] = h.hyy;
+    H[2][2] = h.hzz;
+    H[0][1] = h.hxy;
+    H[0][2] = h.hxz;
+    H[1][2] = h.hyz;
+    if constexpr (tile_cubic<NI>()) {
+        gm[0] += h.txxx * (1.0 / 6.0) * fma(6.0 * loc[0], loc[0] - 1.0, 1.0);
+        gm[1] += h.tyyy * (1.0 / 6.0) * fma(6.0 * loc[1], loc[1] - 1.0, 1.0);
+        gm[2] += h.tzzz * (1.0 / 6.0) * fma(6.0 * loc[2], loc[2] - 1.0, 1.0);
+        H[0][0] += h.txxx * px + h.txxy * py + h.txxz * pz +
+                   h.txxx * (1.0 / 6.0) * fma(12.0, loc[0], -6.0);
+        H[1][1] += h.tyyy * py + h.txyy * px + h.tyyz * pz +
+                   h.tyyy * (1.0 / 6.0) * fma(12.0, loc[1], -6.0);
+        H[2][2] += h.tzzz * pz + h.txzz * px + h.tyzz * py +
+                   h.tzzz * (1.0 / 6.0) * fma(12.0, loc[2], -6.0);
+        H[0][1] += h.txxy * px + h.txyy * py + h.txyz * pz;
+        H[0][2] += h.txxz * px + h.txzz * pz + h.txyz * py;
+        H[1][2] += h.tyyz * py + h.tyzz * pz + h.txyz * px;
+    }
+    H[0][0] += (double)hr[0];
+    H[1][1] += (double)hr[1];
+    H[2][2] += (double)hr[2];
+    H[0][1] += (double)hr[3];
+    H[0][2] += (double)hr[4];
+    H[1][2] += (double)hr[5];
+    H[1][0] = H[0][1];
+    H[2][0] = H[0][2];
+    H[2][1] = H[1][2];
+    const double v = gv[i];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+        double acc = v * ((gm[x] + (double)gr[x]) / g.h[x]);
+#pragma unroll
+        for (int y = 0; y < 3; ++y) acc += (H[x][y] / (g.h[x] * g.h[y])) * gg[i * 3 + y];
+        dq[i * 3 + x] = acc;
+    }
+}
+
 template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
@@ -1517,6 +1657,32 @@ hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t
     ARB_NODE_SHAPES(ARB_TILES_CASE)
 #undef ARB_TILES_CASE
 #undef L_TILES
+    return hipErrorInvalidValue;
+}
+
+hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
+                                const void* tiles, const double* q, int64_t n, int64_t ldq,
+                                const double* gv, const double* gg, double* dq, int32_t* key,
+                                int32_t* count, double* rec, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (ninterp != 4 && ninterp != 1) return hipErrorInvalidValue;   // modes with a gradient
+    int64_t hoff = 0;
+    arb_tiles_bytes(B, ninterp, &hoff);
+    const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
+    dim3 grid(grid1(n, 256)), block(256);
+#define L_BWD(NI, X, Y, Z)                                                                    \
+    hipLaunchKernelGGL((k_bwd_tiles3<NI, X, Y, Z>), grid, block, 0, s, g, B,                  \
+                       (const typename TileRec<NI>::type*)tiles, heads, q, n, ldq, gv, gg, dq, \
+                       key, count, rec)
+#define ARB_BWD_CASE(X, Y, Z)                                                                 \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
+        if (ninterp == 4) L_BWD(4, X, Y, Z);                                                  \
+        else L_BWD(1, X, Y, Z);                                                               \
+        return hipGetLastError();                                                             \
+    }
+    ARB_NODE_SHAPES(ARB_BWD_CASE)
+#undef ARB_BWD_CASE
+#undef L_BWD
     return hipErrorInvalidValue;
 }
 
