@@ -140,6 +140,10 @@ def test_edge_battery_golden(dev, golden):
 # (measured 6e-10 value, 2e-8 gradient; 5.5e-8 and 2.8e-6 with an FP64
 # reduction).  Cell indices and NaN rows are identical.
 TOL_F32F32 = 1e-7
+# mixed precision near the zeros of a fast-varying scalar field (floored
+# normwise metric): node tiles and the FP32 coefficient table
+TOL_NODES_ZC = 5e-6
+TOL_TABLE_ZC = 5e-5
 
 
 @pytest.mark.parametrize("tag", ["f64f32", "f32f64", "f32f32"])
@@ -348,10 +352,12 @@ def test_query_paths_bitwise_identical(dev, case):
         assert torch.equal(outs["table"][3], outs["direct"][3])
         # the same NaN rows as the table family, and the oracle's values on
         # every row inside the box: within 2e-7 of the condition scale (the
-        # FP32 bound), and under the floored metric within 1e-6 or the
-        # table path's own error, whichever is larger (near a zero of a scalar
-        # that varies fast across a cell neither FP32 form holds 1e-6 per
-        # value: table 2.5e-5, node tiles 3.3e-6 on this 23 x 19 x 14 grid)
+        # FP32 bound), and under the floored metric within the stated bound
+        # of each path.  A scalar field that crosses zero and varies fast
+        # across a cell (one dipole component on this coarse 23 x 19 x 14
+        # grid) is the hard case for FP32 evaluation: there the node tiles
+        # are bounded by 5e-6 (measured 3.3e-6) and the FP32 table by 5e-5
+        # (2.5e-5); the 'both' field holds 1e-6 on both paths
         for a, b in zip(outs["table"][:3], outs["lds"][:3]):
             if a is not None:
                 assert torch.equal(torch.isnan(a), torch.isnan(b)), case
@@ -375,7 +381,9 @@ def test_query_paths_bitwise_identical(dev, case):
             ms, _ = O.rel_err_normwise(b, r, s, floor=1.0)
             print("%s node tiles k=%d: %.3e (table %.3e), relative to S %.3e" % (case, k, mx, mt, ms))
             assert ms <= 2e-7, (case, k, ms)
-            assert mx <= max(TOLMIX, mt), (case, k, mx, mt)
+            zero_crossing = case == "3d_scalar_mixed"
+            assert mx <= (TOL_NODES_ZC if zero_crossing else TOLMIX), (case, k, mx)
+            assert mt <= (TOL_TABLE_ZC if zero_crossing else TOLMIX), (case, k, mt)
 
 
 def test_coefficient_save_load_roundtrip(dev, tmp_path):
