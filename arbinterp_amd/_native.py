@@ -86,6 +86,7 @@ SIGNATURES = [
     ("arb_batch_destroy", None, [c_vp]),
     ("arb_push_rk4", c_i32, [c_vp, c_vp, c_vp, c_i64, ctypes.c_double, c_i64, c_i32, c_i32, c_vp]),
     ("arb_last_timings", c_i32, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("arb_device_copy", c_i32, [c_vp, c_vp, c_i64, c_vp]),
     ("arb_set_tuning", c_i32, [c_vp, ctypes.POINTER(c_i32), c_i32]),
     ("arb_operator_matrix", c_i32, [c_i32, c_i32, c_vp]),
     ("arb_csv_shape", c_i32, [ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)]),

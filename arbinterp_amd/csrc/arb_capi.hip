@@ -195,6 +195,8 @@ struct arb_field {
     int32_t* ws_perm = nullptr; // bucket-sorted query indices
     void* ws_rec = nullptr;     // packed output records of a bucketed call (split afterwards)
     int64_t ws_rec_bytes = 0;
+    void* ws_qrec = nullptr;    // per-row records of the node-tile path (16 bytes per row)
+    int64_t ws_qrec_n = 0;
     int64_t ws_n = 0;
     int32_t* ws_count = nullptr;
     int32_t* ws_offs = nullptr;
@@ -220,9 +222,9 @@ struct arb_field {
     hipGraphExec_t push_exec = nullptr;
     double push_key_d = 0.0;
     int64_t push_key[5] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[5] = {};
+    hipEvent_t ev[6] = {};
     int32_t ev_ready = 0;
-    float last_ms[4] = {0, 0, 0, 0};
+    float last_ms[5] = {0, 0, 0, 0, 0};
     // single-point mailbox: pinned host record the point kernel reads and
     // writes in place (host address, and the same memory as the GPU sees it)
     double* pin = nullptr;
@@ -344,6 +346,19 @@ static int ensure_workspace(arb_field* f, int64_t n, int64_t nbt) {
         HIPCHK(hipMalloc(&f->ws_count, sizeof(int32_t) * arb_rs_first_elems(nbt)));
         HIPCHK(hipMalloc(&f->ws_offs, sizeof(int32_t) * (nbt + 1)));
         f->ws_nb = nbt + 1;
+    }
+    return ARB_OK;
+}
+
+// Per-row records of the node-tile bucketed path (n x 16 bytes), made on
+// first use.
+static int ensure_qrec(arb_field* f, int64_t n) {
+    if (n > f->ws_qrec_n) {
+        if (f->ws_qrec) (void)hipFree(f->ws_qrec);
+        f->ws_qrec = nullptr;
+        f->ws_qrec_n = 0;
+        HIPCHK(hipMalloc(&f->ws_qrec, (size_t)n * 16));
+        f->ws_qrec_n = n;
     }
     return ARB_OK;
 }
@@ -712,9 +727,10 @@ struct SortWs {
     int32_t* hscan = nullptr;
     int32_t* first = nullptr;
     int32_t* offs = nullptr;
+    void* qrec = nullptr;       // node-tile path: 16-byte record per row
 };
 
-static int64_t carve_sort_ws(unsigned char* base, int64_t n, int64_t nbt, SortWs* w) {
+static int64_t carve_sort_ws(unsigned char* base, int64_t n, int64_t nbt, bool qrec, SortWs* w) {
     int64_t off = 0;
     auto take = [&](int64_t bytes) -> void* {
         void* p = base ? (void*)(base + off) : nullptr;
@@ -730,6 +746,7 @@ static int64_t carve_sort_ws(unsigned char* base, int64_t n, int64_t nbt, SortWs
     w->hscan = static_cast<int32_t*>(take(sizeof(int32_t) * (hl + 256)));
     w->first = static_cast<int32_t*>(take(sizeof(int32_t) * arb_rs_first_elems(nbt)));
     w->offs = static_cast<int32_t*>(take(sizeof(int32_t) * (nbt + 1)));
+    if (qrec) w->qrec = take(16 * n);
     return off;
 }
 
@@ -850,15 +867,20 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             unsigned char* base = nullptr;
             rc = scratch_address(f->device, &base);
             if (rc) return rc;
-            const int64_t need = carve_sort_ws(nullptr, n, BB.nbt, &w);
+            const int64_t need = carve_sort_ws(nullptr, n, BB.nbt, nodeform, &w);
             if (need > (int64_t)ARB_NOALLOC_SCRATCH_BYTES)
                 return fail(ARB_EINVAL, "%s: %lld rows over %lld buckets need %lld bytes of "
                             "scratch; the limit is %lld", fn, (long long)n, (long long)BB.nbt,
                             (long long)need, (long long)ARB_NOALLOC_SCRATCH_BYTES);
-            carve_sort_ws(base, n, BB.nbt, &w);
+            carve_sort_ws(base, n, BB.nbt, nodeform, &w);
         } else {
             rc = ensure_workspace(f, n, BB.nbt);
             if (rc) return rc;
+            if (nodeform) {
+                rc = ensure_qrec(f, n);
+                if (rc) return rc;
+                w.qrec = f->ws_qrec;
+            }
             for (int k = 0; k < 2; ++k) {
                 w.k[k] = f->rs_k[k];
                 w.v[k] = f->rs_v[k];
@@ -922,7 +944,8 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                                                  esz * (size_t)f->g.nc * f->ninterp *
                                                      ((size_t)1 << (2 * f->g.ndim)));
             HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, sentrow, f->has_comps,
-                                   f->has_norm, q_dev, n, ldq, o, w.k[0], cellbits, w.hist, s));
+                                   f->has_norm, q_dev, n, ldq, o, w.k[0], cellbits, w.hist,
+                                   nodeform ? w.qrec : nullptr, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
             int in_first = 1;
             // the sort also leaves each bucket's first sorted position in
@@ -940,7 +963,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                                         f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
                                         perm, s));
         else if (nodeform)
-            HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, q_dev, n, ldq, o,
+            HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, w.qrec, n, o,
                                          offs, perm, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
@@ -950,14 +973,15 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
                                           offs, perm, f->eval_threads, s));
+        if (timing) HIPCHK(hipEventRecord(f->ev[4], s));
         if (split)
             HIPCHK(arb_launch_unpack(f->g.ndim, f->precision == ARB_PREC_FP64 || out64,
                                      f->has_comps, f->has_norm, f->ws_rec, comps_dev, norms_dev,
                                      grads_dev, n, s));
         if (timing) {
-            HIPCHK(hipEventRecord(f->ev[4], s));
-            HIPCHK(hipEventSynchronize(f->ev[4]));
-            for (int k = 0; k < 4; ++k)
+            HIPCHK(hipEventRecord(f->ev[5], s));
+            HIPCHK(hipEventSynchronize(f->ev[5]));
+            for (int k = 0; k < 5; ++k)
                 HIPCHK(hipEventElapsedTime(&f->last_ms[k], f->ev[k], f->ev[k + 1]));
         }
     }
@@ -1333,9 +1357,17 @@ void arb_batch_destroy(arb_batch* b) {
     delete b;
 }
 
-int arb_last_timings(const arb_field* f, float* ms4) {
-    if (!f || !ms4) return fail(ARB_EINVAL, "arb_last_timings: NULL argument");
-    for (int k = 0; k < 4; ++k) ms4[k] = f->last_ms[k];
+int arb_last_timings(const arb_field* f, float* ms5) {
+    if (!f || !ms5) return fail(ARB_EINVAL, "arb_last_timings: NULL argument");
+    for (int k = 0; k < 5; ++k) ms5[k] = f->last_ms[k];
+    return ARB_OK;
+}
+
+int arb_device_copy(void* dst_dev, const void* src_dev, int64_t bytes, void* stream) {
+    if (!dst_dev || !src_dev || bytes < 0 || bytes % 16 ||
+        ((uintptr_t)dst_dev | (uintptr_t)src_dev) % 16)
+        return fail(ARB_EINVAL, "arb_device_copy: pointers and size must be 16-byte multiples");
+    HIPCHK(arb_launch_copy16(dst_dev, src_dev, bytes / 16, (hipStream_t)stream));
     return ARB_OK;
 }
 
@@ -1420,6 +1452,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->ws_rank) (void)hipFree(f->ws_rank);
     if (f->ws_perm) (void)hipFree(f->ws_perm);
     if (f->ws_rec) (void)hipFree(f->ws_rec);
+    if (f->ws_qrec) (void)hipFree(f->ws_qrec);
     if (f->ws_count) (void)hipFree(f->ws_count);
     if (f->ws_offs) (void)hipFree(f->ws_offs);
     for (int k = 0; k < 2; ++k) {

@@ -266,6 +266,32 @@ __device__ __forceinline__ int64_t bucket_of(int64_t flat, const int64_t* ia, co
     return bidx;
 }
 
+// Cell-local coordinates of the node-tile paths, as every one of them (and
+// the per-row records of the bucketed path) carries them: the FP32 value
+// per axis, which the Hermite weights use, plus an 8-bit correction per
+// axis in units of 2^-32, so that the FP64 model sees the FP64 coordinate
+// to within 2^-32 (the FP32 rounding alone would move the point by up to
+// 2^-25 of a cell, too much near a zero of a fast-varying scalar).  The
+// corrections share a 32-bit word with the local cell index (8 bits: up to
+// 256 cells per bucket).
+__device__ __forceinline__ uint32_t loc_encode(const double* loc, float* hi) {
+    uint32_t packed = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        hi[a] = (float)loc[a];
+        int k = (int)rint((loc[a] - (double)hi[a]) * 4294967296.0);
+        k = k < -128 ? -128 : (k > 127 ? 127 : k);
+        packed |= (uint32_t)(k & 255) << (8 * a + 8);
+    }
+    return packed;
+}
+
+__device__ __forceinline__ void loc_decode(const float* hi, uint32_t packed, double* loc) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        loc[a] = (double)hi[a] + (double)(int8_t)((packed >> (8 * a + 8)) & 255u) * 0x1p-32;
+}
+
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -319,7 +345,8 @@ template <int D, typename T, typename TC, typename TO>
 __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBuckets& B,
                                                const TC* __restrict__ sent, int32_t has_comps,
                                                int32_t has_norm, const ArbOut& o, int64_t i,
-                                               const double* qp, int32_t cellbits) {
+                                               const double* qp, int32_t cellbits,
+                                               float4* __restrict__ qrec) {
     double loc[D];
     int64_t ia[D];
     bool oob;
@@ -335,6 +362,15 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
     }
     int lc;
     const uint32_t b = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
+    if constexpr (D == 3) {
+        // the row's record for the node-tile evaluation: its local
+        // coordinates (FP32 and corrections) and local cell in the bucket
+        if (qrec) {
+            float lf[3];
+            const uint32_t lcq = loc_encode(loc, lf) | (uint32_t)lc;
+            qrec[i] = make_float4(lf[0], lf[1], lf[2], __uint_as_float(lcq));
+        }
+    }
     return (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
 }
 
@@ -343,7 +379,7 @@ __global__ void __launch_bounds__(KEYS_THREADS)
 k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
        uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
-       int64_t ntiles) {
+       int64_t ntiles, float4* __restrict__ qrec) {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;                           // KEYS_THREADS == 256 == digits
     __syncthreads();
@@ -362,7 +398,7 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
             const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
             if (i < n) {
                 const uint32_t k = key_of_row<D, T, TC, TO>(g, B, sent, has_comps, has_norm, o,
-                                                            i, qv[u], cellbits);
+                                                            i, qv[u], cellbits, qrec);
                 key[i] = k;
                 atomicAdd(&cnt[k & 255u], 1);
             }
@@ -1061,19 +1097,20 @@ __device__ __forceinline__ float last_of(float x) { return x; }
 template <typename TO, int NI, int BX, int BY, int BZ, int YS, int ZS>
 __device__ __forceinline__ void tile_eval(const ArbGeom& g, const ArbBuckets& B,
                                           const typename TileRec<NI>::type* tile,
-                                          const TileHead& h, const ArbOut& o, int lc,
-                                          const double* loc, int64_t qi) {
+                                          const TileHead& h, const ArbOut& o,
+                                          const float* lf, uint32_t lcq, int64_t qi) {
     using R = typename TileRec<NI>::type;
     constexpr bool GRAD = NI != 3;
+    const int lc = (int)(lcq & 255u);
     const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
     // compiler barriers around the contraction: they keep the tile head's
     // LDS loads (32 doubles) from being hoisted out of the query loop and
     // held in registers through it (121 -> 76 VGPRs, 4 -> 6 waves per SIMD)
     asm volatile("" ::: "memory");
     float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
-    hermite_weights((float)loc[0], wx, dx);
-    hermite_weights((float)loc[1], wy, dy);
-    hermite_weights((float)loc[2], wz, dz);
+    hermite_weights(lf[0], wx, dx);
+    hermite_weights(lf[1], wy, dy);
+    hermite_weights(lf[2], wz, dz);
     const R* base = tile + lz * ZS + ly * YS + lx;
     R val{};
     float gx = 0.f, gy = 0.f, gz = 0.f;
@@ -1110,6 +1147,8 @@ __device__ __forceinline__ void tile_eval(const ArbGeom& g, const ArbBuckets& B,
     asm volatile("" ::: "memory");
     // the models at the query, in tile coordinates (cell lx spans nodes
     // lx + 1 and lx + 2)
+    double loc[3];
+    loc_decode(lf, lcq, loc);
     const double px = (double)(lx + 1 - h.m[0]) + loc[0];
     const double py = (double)(ly + 1 - h.m[1]) + loc[1];
     const double pz = (double)(lz + 1 - h.m[2]) + loc[2];
@@ -1155,23 +1194,6 @@ __device__ __forceinline__ void tile_eval(const ArbGeom& g, const ArbBuckets& B,
 }
 
 
-// Evaluate row qi (position qp) of bucket b from the node tile in LDS.
-template <typename TO, int NI, int BX, int BY, int BZ>
-__device__ __forceinline__ void eval_row_tile(const ArbGeom& g, const ArbBuckets& B,
-                                              const typename TileRec<NI>::type* tile,
-                                              const TileHead& h, const ArbOut& o, int64_t b,
-                                              int64_t qi, const double* qp) {
-    using S = TileShape<NI, BX, BY, BZ>;
-    double loc[3];
-    int64_t ia[3];
-    bool oob;
-    const int64_t flat = arb_reduce_axes<3>(qp, g, loc, ia, &oob);
-    int lc;
-    const int64_t bb = bucket_of<3>(flat, ia, g, B, &lc);
-    if (bb != b || flat >= g.nc) return;            // cannot happen: same reduction as k_keys
-    tile_eval<TO, NI, BX, BY, BZ, S::YS, S::ZS>(g, B, tile, h, o, lc, loc, qi);
-}
-
 // Lane-per-query node-tile kernel (small batches and single points): each
 // lane reads its cell's 64 records and its block's head from the tiles in
 // global memory.  Rows outside the box, or past the sentinel, are evaluated
@@ -1200,8 +1222,10 @@ k_query_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __rest
     }
     int lc;
     const int64_t b = bucket_of<3>(flat, ia, g, B, &lc);
+    float lf[3];
+    const uint32_t lcq = loc_encode(loc, lf) | (uint32_t)lc;
     tile_eval<TO, NI, BX, BY, BZ, S::NX, S::NX * S::NY>(g, B, tiles + b * S::NTP, heads[b], o,
-                                                        lc, loc, i);
+                                                        lf, lcq, i);
 }
 
 // Second derivatives of the Hermite weights (w_o'' for hermite_weights).
@@ -1255,6 +1279,13 @@ k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restri
     int lc;
     const int64_t b = bucket_of<3>(flat, ia, g, B, &lc);
     const int lx = lc & (BX - 1), ly = (lc >> B.lsh[1]) & (BY - 1), lz = (lc >> B.lsh[2]) & (BZ - 1);
+    // the forward's local coordinates (FP32 for the weights, decoded for
+    // the model)
+    {
+        float lf[3];
+        const uint32_t lcq = loc_encode(loc, lf);
+        loc_decode(lf, lcq, loc);
+    }
     float wx[4], dx[4], ddx[4], wy[4], dy[4], ddy[4], wz[4], dz[4], ddz[4];
     hermite_weights((float)loc[0], wx, dx);
     hermite_weights((float)loc[1], wy, dy);
@@ -1347,9 +1378,8 @@ k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restri
 template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
-              const TileHead* __restrict__ heads, const double* __restrict__ q, int64_t n,
-              int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
-              const int32_t* __restrict__ perm) {
+              const TileHead* __restrict__ heads, const float4* __restrict__ qrec, int64_t n,
+              ArbOut o, const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
     using S = TileShape<NI, BX, BY, BZ>;
     using RB = typename TileRec<NI>::bits;
     __shared__ typename TileRec<NI>::type tile[S::NZ * S::ZS];
@@ -1359,13 +1389,11 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restr
     if (start == end) return;                       // uniform over the workgroup
     int32_t s = start + threadIdx.x;
     int64_t qi = -1;
-    double qp[3];
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
     if (s < end) {
         const int64_t pi = perm[s];
         qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
-        const int64_t qq = qi >= 0 ? qi : 0;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) qp[a] = q[qq * ldq + a];
+        r = qrec[qi >= 0 ? qi : 0];
     }
     // stage the tile (read once per call: nontemporal) and its head
     const RB* src = reinterpret_cast<const RB*>(tiles + b * S::NTP);
@@ -1378,13 +1406,19 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restr
         reinterpret_cast<double*>(&h)[threadIdx.x] =
             reinterpret_cast<const double*>(heads + b)[threadIdx.x];
     __syncthreads();
-    if (qi >= 0) eval_row_tile<TO, NI, BX, BY, BZ>(g, B, tile, h, o, b, qi, qp);
+    // each row's record, written by k_keys: FP32 local coordinates and the
+    // local cell inside the bucket (the same reduction as every path's)
+    if (qi >= 0) {
+        const float lf[3] = {r.x, r.y, r.z};
+        tile_eval<TO, NI, BX, BY, BZ, S::YS, S::ZS>(g, B, tile, h, o, lf, __float_as_uint(r.w), qi);
+    }
     for (s += blockDim.x; s < end; s += blockDim.x) {
         const int64_t pi = perm[s];
         if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
-#pragma unroll
-        for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
-        eval_row_tile<TO, NI, BX, BY, BZ>(g, B, tile, h, o, b, pi, qp);
+        const float4 rr = qrec[pi];
+        const float lf[3] = {rr.x, rr.y, rr.z};
+        tile_eval<TO, NI, BX, BY, BZ, S::YS, S::ZS>(g, B, tile, h, o, lf, __float_as_uint(rr.w),
+                                                    pi);
     }
 }
 
@@ -1476,6 +1510,21 @@ hipError_t arb_launch_query_direct(const ArbGeom& g, int32_t prec, int32_t out_f
                                 o, s);
     return launch_direct<4>(g, prec, out_f64, table, ninterp, has_comps, has_norm, q, n, ldq, o,
                             s);
+}
+
+namespace {
+__global__ void __launch_bounds__(256)
+k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+}  // namespace
+
+hipError_t arb_launch_copy16(void* dst, const void* src, int64_t n16, hipStream_t s) {
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy16, dim3(grid1(n16, 256)), dim3(256), 0, s, (const uint4*)src,
+                       (uint4*)dst, n16);
+    return hipGetLastError();
 }
 
 hipError_t arb_launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
@@ -1687,17 +1736,17 @@ hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t n
 }
 
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
-                                 int32_t out_f64, const void* tiles, const double* q, int64_t n,
-                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                 const int32_t* perm, int32_t block_threads, hipStream_t s) {
+                                 int32_t out_f64, const void* tiles, const void* qrec, int64_t n,
+                                 const ArbOut& o, const int32_t* offs, const int32_t* perm,
+                                 int32_t block_threads, hipStream_t s) {
     int64_t hoff = 0;
     arb_tiles_bytes(B, ninterp, &hoff);
     const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
     dim3 grid((unsigned)B.nbt), block(block_threads);
 #define L_NODES(TO, NI, X, Y, Z)                                                              \
     hipLaunchKernelGGL((k_eval_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,             \
-                       (const typename TileRec<NI>::type*)tiles, heads, q, n, ldq, o, offs,   \
-                       perm)
+                       (const typename TileRec<NI>::type*)tiles, heads, (const float4*)qrec,  \
+                       n, o, offs, perm)
 #define L_NODES_NI(TO, X, Y, Z)                                                               \
     if (ninterp == 4) L_NODES(TO, 4, X, Y, Z);                                                \
     else if (ninterp == 3) L_NODES(TO, 3, X, Y, Z);                                           \
@@ -1764,13 +1813,14 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* sent, int32_t has_comps, int32_t has_norm,
                            const double* q, int64_t n, int64_t ldq, const ArbOut& o,
-                           uint32_t* key, int32_t cellbits, int32_t* hist, hipStream_t s) {
+                           uint32_t* key, int32_t cellbits, int32_t* hist, void* qrec,
+                           hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
                        s, g, B, (const TC*)sent, has_comps, has_norm, q, n, ldq, o, key,         \
-                       cellbits, hist, ntiles)
+                       cellbits, hist, ntiles, (float4*)qrec)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

@@ -3,10 +3,12 @@
 Workload: 128^3 vector field of 6 point dipoles (h = 1.0, 0.8, 1.25 mm, N x 7
 rows), mode='both' (components, |B| and grad|B|), 10^7 query positions
 uniform in the interpolation interior, already resident in HBM.  One step =
-one batch query of all 10^7 positions.  The headline path is mixed precision
-(FP64 cell reduction and per-block node models, FP32 node-tile differences
-and evaluation); the plain FP64 path (FP64 coefficient table) is timed
-beside it.
+one batch query of all 10^7 positions, with the outputs in the reference's
+layout (comps, |B| and grad|B| arrays).  The headline path is mixed
+precision (FP64 cell reduction and per-block node models, FP32 node-tile
+differences and evaluation, FP32 outputs); the plain FP64 path (FP64
+coefficient table, FP64 outputs) is timed beside it, and so are the other
+query paths and the packed-record layouts.
 
 Prints one JSON line (rank 0).  `--gpus N` with N > 1 is not supported: this
 build runs on one GPU.
@@ -29,8 +31,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "value+grad queries/s, mixed-prec vs FP64 plain on 1 GPU at ≤1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
-# evaluation kernel of the headline step (mixed, 3D, FP64 output records)
-HEAD_KERNEL = "k_eval_tiles3<double, 4, 4, 4, 2>"
+# the bound of the metric's relative error: normwise per query and output
+# kind, with the oracle's conditioning floor (oracle.rel_err_normwise)
+ERR_BOUND = 1e-6
+# evaluation kernel of the headline step (mixed, 3D 'both', FP32 output
+# records split into the reference's arrays afterwards)
+HEAD_KERNEL = "k_eval_tiles3<float, 4, 4, 4, 2>"
 # bytes of one node-tile head (TileHead: 32 doubles and 4 int32)
 TILE_HEAD_BYTES = 272
 
@@ -56,8 +62,9 @@ def parse():
                     choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield", "point"],
                     help="dipole3d: the headline JSON line; the others print their own line")
     ap.add_argument("--profile-only", action="store_true",
-                    help="only the mixed-precision default path: build, warm up, --steps queries "
-                         "(for counter runs; no oracle, no CPU baseline, no JSON line)")
+                    help="only the mixed-precision headline path (default flags, output arrays): "
+                         "--steps queries (for counter runs; no oracle, no CPU baseline, no JSON "
+                         "line)")
     return ap.parse_args()
 
 
@@ -103,10 +110,9 @@ def main():
         rows, q = W.dipole3d(seed=2, n=n, nq=args.nq)
         qd = torch.from_numpy(q).cuda()
         t = tricubic(rows, "quiet", mode="both", precision="mixed")
-        t.allCoeffs()
-        rec = t.query_packed(qd, flags=args.flags, out_f64=True)
+        out = t.query_device(qd, flags=args.flags)
         for _ in range(args.steps):
-            t.query_packed(qd, out=rec, flags=args.flags, out_f64=True)
+            t.query_device(qd, out=out, flags=args.flags)
         torch.cuda.synchronize()
         log("profile-only: %d steps done" % args.steps)
         return
@@ -129,24 +135,24 @@ def main():
                 "sorted_global": NN.ARB_Q_SORTED_GLOBAL}
     if args.variants:
         variants = {k: v for k, v in variants.items() if k in args.variants.split(",")}
+    if "auto" not in variants:
+        variants = dict(auto=0, **variants)
     res = {}
     for prec in ("mixed", "fp64"):
         t = tricubic(rows, "quiet", mode="both", precision=prec)
-        torch.cuda.synchronize()
-        tb = time.time()
-        t.allCoeffs()
-        build_ms = (time.time() - tb) * 1e3
+        build_ms = one_off_build_ms(torch, t, qd)
+        # output layouts: "arrays" the reference's three arrays (comps N x 3,
+        # |B| N, grad N x 3; FP32 for mixed, FP64 for fp64: the drop-in's
+        # default, and the headline), "p64" one 64-byte FP64 record per row,
+        # "p32" one 32-byte FP32 record (mixed only)
         out = t.query_device(qd, cells=True)
-        # output layouts: "p64" one 64-byte FP64 record per row (the headline;
-        # full 64-byte writes), "p32" one 32-byte FP32 record (mixed only),
-        # "arrays" separate comps / norm / grad arrays
         recs = {"p64": t.query_packed(qd, out_f64=True)}
         if prec == "mixed":
             recs["p32"] = t.query_packed(qd)
         torch.cuda.synchronize()
         per = {}
         for vname, fl in variants.items():
-            for layout in list(recs) + ["arrays"]:
+            for layout in ["arrays"] + list(recs):
                 f = fl | args.flags
                 if layout == "arrays":
                     fn = lambda f=f: t.query_device(qd, out=out[:3], flags=f)
@@ -155,20 +161,20 @@ def main():
                     fn = lambda f=f, r=recs[layout], r64=r64: t.query_packed(qd, out=r, flags=f,
                                                                             out_f64=r64)
                 ms = time_steps(torch, fn, args.steps, args.warmup)
-                st = np.zeros(4)
+                st = np.zeros(5)
                 for _ in range(args.timing_steps):
                     fn(f=f | NN.ARB_Q_TIMING)
                     st += np.array(t.timings())
                 st /= max(1, args.timing_steps)
-                key = vname if layout == "p64" else vname + "_" + layout
+                key = vname if layout == "arrays" else vname + "_" + layout
                 per[key] = {"ms_per_step": ms, "stage_ms": [round(x, 5) for x in st.tolist()]}
                 log("%s %-20s: %.3f ms/step  %.3e q/s  stages %s" % (prec, key, ms,
                                                                      args.nq / ms * 1e3, per[key]["stage_ms"]))
         # the headline path's own outputs, for the accuracy check (the timing
         # loops above leave the last variant's rows in the buffers)
-        rec = t.query_packed(qd, out=recs["p64"], out_f64=True)
+        out = t.query_device(qd, cells=True, out=out)
         torch.cuda.synchronize()
-        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, rec=rec)
+        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per)
     nc = res["mixed"]["obj"].nc
     cells = res["mixed"]["out"][3]
     valid = cells[cells < nc]
@@ -186,7 +192,7 @@ def main():
     UB = int(torch.unique(bk).numel())
     tile_bytes = (blk[0] + 3) * (blk[1] + 3) * (blk[2] + 3) * 16 + TILE_HEAD_BYTES
 
-    # accuracy of the mixed path against the FP64 oracle on a sample
+    # accuracy of both paths against the FP64 oracle on a sample of rows
     from oracle import arbinterp_oracle as O
     rng = np.random.default_rng(7)
     idx = np.sort(rng.choice(args.nq, min(args.err_sample, args.nq), replace=False))
@@ -195,7 +201,7 @@ def main():
     scales = o.condition_scale(q[idx])
     err = {}
     for prec in ("mixed", "fp64"):
-        c, nv, g = (x.double().cpu().numpy() for x in res[prec]["obj"].unpack(res[prec]["rec"]))
+        c, nv, g = (x.double().cpu().numpy() for x in res[prec]["out"][:3])
         got = (c[idx], nv[idx, None], g[idx])
         e = {}
         for name, a, b, s in zip(("comps", "norm", "grad"), got, ref, scales):
@@ -205,23 +211,28 @@ def main():
                        "elementwise_rms": erms}
         err[prec] = e
     log("errors vs oracle:", json.dumps(err))
+    worst = max(v["normwise_max"] for v in err["mixed"].values())
+    within = bool(worst <= ERR_BOUND)
 
-    # The step is timed on the library's default path ("auto").  The roofline
-    # figure is for the evaluation kernel of that path over its HIP-event
-    # duration.  Algorithmic bytes: mixed (node tiles) N*(24 + 7*8) + UB*tile
-    # bytes; FP64 (coefficient table) N*(24 + 7*8) + U*4*64*8.
+    # The step is timed on the library's default path ("auto") with the
+    # reference's output layout.  roofline.achieved: the algorithmic bytes of
+    # the evaluation (N*(24 + 7*s_o) + U*4*64*s_c: FP64 positions in, 7 outputs,
+    # each touched cell's 4 x 64 coefficients once; FP32 s_o = s_c = 4 for the
+    # mixed path) over the evaluation kernel's own HIP-event duration.
     N = args.nq
-    head = "auto" if "auto" in res["mixed"]["per"] else next(iter(res["mixed"]["per"]))
-    ms = res["mixed"]["per"][head]["ms_per_step"]
-    ms64 = res["fp64"]["per"][head]["ms_per_step"]
-    # algorithmic output bytes are the 7 values per row, 8 bytes each in the
-    # FP64 record (the record's pad word is overhead of the layout, not counted)
-    alg_bytes = N * (24 + 7 * 8) + UB * tile_bytes
+    ms = res["mixed"]["per"]["auto"]["ms_per_step"]
+    ms64 = res["fp64"]["per"]["auto"]["ms_per_step"]
+    st = res["mixed"]["per"]["auto"]["stage_ms"]
+    st64 = res["fp64"]["per"]["auto"]["stage_ms"]
+    alg_bytes = N * (24 + 7 * 4) + U * 4 * 64 * 4
     alg64 = N * (24 + 7 * 8) + U * 4 * 64 * 8
-    ev_ms = res["mixed"]["per"][head]["stage_ms"][3] or ms
-    ev64 = res["fp64"]["per"][head]["stage_ms"][3] or ms64
+    node_bytes = N * (24 + 7 * 4) + UB * tile_bytes
+    ev_ms = st[3] or ms
+    ev64 = st64[3] or ms64
     achieved = alg_bytes / (ev_ms * 1e-3) / 1e9
     ach64 = alg64 / (ev64 * 1e-3) / 1e9
+    copy_gbs = copy_bandwidth(torch, NN)
+    log("copy kernel: %.1f GB/s" % copy_gbs)
 
     cpu = None
     if args.cpu_chunks > 0:
@@ -240,6 +251,8 @@ def main():
                          "(%.1f s cold fill, single thread)" % (args.cpu_chunks, cold)}
         log("cpu baseline: %.3e q/s" % cpu["value"])
 
+    per_m = res["mixed"]["per"]
+    per_d = res["fp64"]["per"]
     qps = N / (ms * 1e-3)
     line = {
         "metric": METRIC,
@@ -259,10 +272,9 @@ def main():
                         "%d interior FP64 positions resident in HBM" % (n + (N,)),
             "precision": "mixed (FP64 cell reduction and tile models, FP32 node-tile "
                          "differences and evaluation)",
-            "outputs": "comps, |B|, grad|B| per row as one 64-byte record of FP64 values "
-                       "(query_packed(out_f64=True); FP32 arithmetic, exact widening); the "
-                       "32-byte FP32 record and the separate-array layouts are timed under "
-                       "paths.*_p32 and paths.*_arrays",
+            "outputs": "the reference's layout: comps (N,3), |B| (N), grad|B| (N,3) as separate "
+                       "FP32 arrays (query_device); the packed 64-byte FP64 and 32-byte FP32 "
+                       "records are timed under paths.*_p64 / *_p32",
             "queries": N, "cells": nc, "distinct_cells_touched": U,
             "bucket_block": blk, "distinct_buckets_touched": UB,
             "node_tile_bytes": tile_bytes,
@@ -271,18 +283,88 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic(HEAD_KERNEL),
                      "kernel": HEAD_KERNEL, "kernel_ms": ev_ms,
-                     "algorithmic_bytes": alg_bytes},
+                     "algorithmic_bytes": alg_bytes,
+                     "algorithmic_bytes_per_query": alg_bytes / N,
+                     "copy_kernel_GBs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
+                     "node_tile_bytes": node_bytes,
+                     "node_tile_achieved": node_bytes / (ev_ms * 1e-3) / 1e9,
+                     "split_ms": st[4],
+                     "eval_plus_split_frac": alg_bytes / ((ev_ms + st[4]) * 1e-3) / 1e9
+                     / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
+        "accuracy": {"metric": "normwise per query and output kind: |o-r|_2 / max(|r|_2, "
+                               "1e-4 |S|_2), S the oracle's condition scale; elementwise "
+                               "under the same floor reported beside it",
+                     "bound": ERR_BOUND, "mixed_normwise_max": worst,
+                     "within_bound": within, "sample_rows": int(len(idx))},
         "fp64_plain": {"value": N / (ms64 * 1e-3), "ms_per_step": ms64,
-                       "eval_kernel_ms": ev64, "roofline_achieved_GBs": ach64,
-                       "frac": ach64 / HBM_PEAK_GBS},
+                       "eval_kernel_ms": ev64, "split_ms": st64[4],
+                       "roofline_achieved_GBs": ach64, "frac": ach64 / HBM_PEAK_GBS,
+                       "algorithmic_bytes": alg64},
         "speedup_vs_fp64": ms64 / ms,
+        "packed_records": {
+            "mixed_p64_ms": per_m["auto_p64"]["ms_per_step"],
+            "mixed_p32_ms": per_m["auto_p32"]["ms_per_step"],
+            "fp64_p64_ms": per_d["auto_p64"]["ms_per_step"],
+            "speedup_p64_vs_fp64_p64": per_d["auto_p64"]["ms_per_step"] / per_m["auto_p64"]["ms_per_step"],
+        },
         "paths": {p: res[p]["per"] for p in ("mixed", "fp64")},
-        "table_build_ms": {"mixed": res["mixed"]["build_ms"], "fp64": res["fp64"]["build_ms"]},
+        "one_off_build_ms": {"mixed_node_tiles": res["mixed"]["build_ms"],
+                             "fp64_table": res["fp64"]["build_ms"]},
         "rel_err_vs_fp64_oracle": err,
-        "algorithmic_bytes_per_step": alg_bytes,
     }
     print(json.dumps(line), flush=True)
+    if not within:
+        log("FAILED: mixed normwise error %.3e exceeds the bound %.0e" % (worst, ERR_BOUND))
+        sys.exit(3)
+
+
+def one_off_build_ms(torch, t, qd):
+    """Duration (HIP events) of the field's one-off build for its default
+    query path: the node tiles of a mixed 3D field, the coefficient table
+    otherwise.  Timed on a rebuild (same node values set again), so no
+    allocation falls inside."""
+    import ctypes
+    from arbinterp_amd import _native as NN
+    L = NN.lib()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    vals = torch.empty((t.field_info().ninterp, t.field_info().nnodes), dtype=torch.float64,
+                       device="cuda")
+    NN.check(L.arb_field_values(t._h, ctypes.c_void_p(vals.data_ptr()), st))
+    tiles = t.precision == "mixed"
+
+    def build():
+        if tiles:
+            t.query_device(qd[:0])
+        else:
+            NN.check(L.arb_build_coeffs(t._h, st))
+    build()
+    NN.check(L.arb_field_set_values(t._h, ctypes.c_void_p(vals.data_ptr()), st))
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    build()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def copy_bandwidth(torch, NN, mib=512, reps=10):
+    """GB/s of the library's plain 16-byte-vector device copy (read + write
+    bytes), on the current stream."""
+    import ctypes
+    nb = mib << 20
+    src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    L = NN.lib()
+
+    def fn():
+        NN.check(L.arb_device_copy(ctypes.c_void_p(dst.data_ptr()),
+                                   ctypes.c_void_p(src.data_ptr()), nb, st))
+    ms = time_steps(torch, fn, reps, 3)
+    return 2 * nb / (ms * 1e-3) / 1e9
 
 
 def profiled_traffic(kernel):

@@ -200,14 +200,15 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
  *     flags, n may be 0); they are rebuilt in place after
  *     arb_field_set_values.  Otherwise ARB_EINVAL;
  *   - n <= ARB_NOALLOC_MAX_ROWS, and the workspace of n rows over the field's
- *     buckets must fit the array (about 16 bytes per row plus 8 per bucket),
- *     else ARB_EINVAL with the sizes in the message;
+ *     buckets must fit the array (about 16 bytes per row, 32 on the mixed 3D
+ *     node-tile path, plus 8 per bucket), else ARB_EINVAL with the sizes in
+ *     the message;
  *   - ARB_Q_TIMING and ARB_Q_GROUP_ATOMIC are refused (ARB_EINVAL);
  *   - the field's device must be current.
  * Every call on one GPU shares that GPU's array and leaves its contents
  * behind (each call writes what it reads first), so calls that overlap on
  * one GPU -- on different streams, threads or handles -- race: order them. */
-#define ARB_NOALLOC_SCRATCH_BYTES (48u << 20)
+#define ARB_NOALLOC_SCRATCH_BYTES (80u << 20)
 #define ARB_NOALLOC_MAX_ROWS (1 << 21)
 int arb_query_noalloc(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
                       void* comps_dev, void* norms_dev, void* grads_dev, int64_t* cells_dev,
@@ -290,10 +291,19 @@ int arb_push_rk4(arb_field* f, double* x_dev, double* v_dev, int64_t n, double d
                  int64_t nsteps, int32_t steps_per_launch, int32_t use_graph, void* stream);
 
 /* Stage durations (ms, HIP events on the call's stream) of the last arb_query
- * made with ARB_Q_TIMING: [0] reduction + bucketing (whole kernel on the
- * direct path), [1] bucket scan, [2] scatter, [3] evaluation.  No reference
- * counterpart (the reference has no timers); used by the benchmark. */
-int arb_last_timings(const arb_field* f, float* ms4);
+ * made with ARB_Q_TIMING, five entries: [0] reduction + bucket keys (the
+ * whole kernel on the direct path, reported in [3]), [1] radix sort (bucket
+ * scan with ARB_Q_GROUP_ATOMIC), [2] scatter (ARB_Q_GROUP_ATOMIC), [3]
+ * evaluation, [4] split of packed records into separate output arrays (0
+ * when not used).  No reference counterpart (the reference has no timers);
+ * used by the benchmark. */
+int arb_last_timings(const arb_field* f, float* ms5);
+
+/* A plain device-to-device copy of `bytes` (a multiple of 16; both pointers
+ * 16-byte aligned) with one 16-byte load and store per thread: the
+ * bandwidth a streaming kernel reaches on this GPU, measured by the
+ * benchmark beside the evaluation kernel.  No reference counterpart. */
+int arb_device_copy(void* dst_dev, const void* src_dev, int64_t bytes, void* stream);
 
 /* Tuning of the bucketed query path (used by the autotuner): cells per block
  * along each axis (ndim entries, NULL = library default) and threads per

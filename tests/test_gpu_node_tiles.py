@@ -125,20 +125,27 @@ def _quadratic_field(n=(26, 22, 19), h=(0.11, 0.09, 0.13), origin=(0.2, -0.4, 0.
 
 def test_node_tiles_reproduce_quadratics(dev):
     """Components: B = (q, 0, l) with q quadratic and l linear.  The linear
-    component is reproduced to FP64 rounding (the components' model is
-    linear), the zero component is exactly zero, and q keeps its quadratic
-    part in FP32 differences (a few 2^-24 of that part).  |B|: with l = 0,
-    |B| = q (q > 0 on the box), and value and gradient are reproduced to
-    FP64 rounding (|B|'s model is quadratic)."""
+    component is reproduced exactly up to the node tiles' local coordinate
+    (the components' model is linear and carried in FP64; the node tiles
+    carry the cell-local coordinate as FP32 plus a correction in units of
+    2^-32, a shift of at most 2^-32 cell per axis), the zero component is
+    exactly zero, and q keeps its quadratic part in FP32 differences (a few
+    2^-24 of that part).  |B|: with l = 0, |B| = q (q > 0 on the box), and
+    value and gradient are reproduced to FP64 rounding plus that shift
+    (|B|'s model is quadratic)."""
     rows, axes, quad, grad, lin = _quadratic_field()
     t = dev.tricubic(rows.copy(), "quiet", mode="both", precision="mixed")
     lo, hi = t._lo, t._hi
+    h = np.array([t.hx, t.hy, t.hz])
     rng = np.random.default_rng(4)
     q = lo + (hi - lo) * rng.uniform(0.0, 1.0 - 1e-9, size=(120_000, 3))
     c, nv, g, cells = _nodes_query(t, q)
     x, y, z = q[:, 0], q[:, 1], q[:, 2]
+    shift = 2.0 ** -31 * h                       # bound on the move of the point per axis
     ref_l = lin(x, y, z)
-    assert np.max(np.abs(c[:, 2] - ref_l)) <= 1e-13 * np.max(np.abs(ref_l))
+    slope_l = np.array([0.7, 0.9, 0.35])
+    assert np.max(np.abs(c[:, 2] - ref_l)) <= (1e-13 * np.max(np.abs(ref_l))
+                                                + 1.01 * np.dot(slope_l, shift))
     assert np.all(c[:, 1] == 0.0)
     ref_q = quad(x, y, z)
     assert np.max(np.abs(c[:, 0] - ref_q)) <= 1e-6 * np.max(np.abs(ref_q))
@@ -148,9 +155,13 @@ def test_node_tiles_reproduce_quadratics(dev):
     rows2[:, 5] = 0.0
     t = dev.tricubic(rows2, "quiet", mode="both", precision="mixed")
     c, nv, g, cells = _nodes_query(t, q)
-    assert np.max(np.abs(nv - ref_q)) <= 1e-13 * np.max(np.abs(ref_q))
     ref_g = grad(x, y, z)
-    assert np.max(np.abs(g - ref_g)) <= 1e-12 * np.max(np.abs(ref_g))
+    gmax = np.abs(ref_g).max(axis=0)             # per-axis bound of |dq/dx_a|
+    hess = np.array([[0.3, 0.12, -0.08], [0.12, 0.2, 0.05], [-0.08, 0.05, 0.4]])
+    assert np.max(np.abs(nv - ref_q)) <= (1e-13 * np.max(np.abs(ref_q))
+                                          + 1.01 * np.dot(gmax, shift))
+    gerr = np.abs(g - ref_g).max(axis=0)
+    assert np.all(gerr <= 1e-12 * np.max(np.abs(ref_g)) + 1.01 * np.abs(hess) @ shift), gerr
     t.close()
 
 
