@@ -492,7 +492,73 @@ k_unpack(const TO* __restrict__ rec, int32_t has_comps, int32_t has_norm,
     }
 }
 
-// Scatter: query index into bucket order.
+// The same split for 3D records when the three arrays are 16-byte aligned:
+// a workgroup reads its 256 rows' records (one per thread) into LDS, then
+// writes each array's part for those rows (768 comps, 256 norms, 768 grads
+// values, contiguous) as consecutive 16-byte vectors, so every store
+// instruction of a wave covers whole cache lines.
+constexpr int UNPACK_ROWS = 256;
+
+template <typename TO, int W>
+__global__ void __launch_bounds__(UNPACK_ROWS)
+k_unpack_lds(const TO* __restrict__ rec, int32_t has_comps, TO* __restrict__ comps,
+             TO* __restrict__ norms, TO* __restrict__ grads, int64_t n) {
+    static_assert(W == 4 || W == 8, "records are 4 or 8 elements");
+    using V = typename std::conditional<sizeof(TO) == 4, float4, double2>::type;
+    constexpr int PER = 16 / sizeof(TO);
+    __shared__ __align__(16) TO sc[UNPACK_ROWS * 3];
+    __shared__ __align__(16) TO sn[UNPACK_ROWS];
+    __shared__ __align__(16) TO sg[UNPACK_ROWS * 3];
+    const int64_t r0 = (int64_t)blockIdx.x * UNPACK_ROWS;
+    const int64_t i = r0 + threadIdx.x;
+    if (i < n) {
+        TO r[W];
+        const V* src = reinterpret_cast<const V*>(rec + i * W);
+#pragma unroll
+        for (int v = 0; v < W / PER; ++v) {
+            const V x = src[v];
+            r[v * PER] = x.x;
+            r[v * PER + 1] = x.y;
+            if constexpr (PER == 4) {
+                r[v * PER + 2] = x.z;
+                r[v * PER + 3] = x.w;
+            }
+        }
+        // constant indices in each branch keep r in registers
+        if (has_comps) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sc[threadIdx.x * 3 + c] = r[c < W ? c : 0];
+            sn[threadIdx.x] = r[3 < W ? 3 : 0];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sg[threadIdx.x * 3 + d] = r[4 + d < W ? 4 + d : 0];
+        } else {
+            sn[threadIdx.x] = r[0];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) sg[threadIdx.x * 3 + d] = r[1 + d];
+        }
+    }
+    __syncthreads();
+    const int64_t rows = (n - r0) < UNPACK_ROWS ? (n - r0) : UNPACK_ROWS;
+    if (rows == UNPACK_ROWS) {
+        // r0 is a multiple of 256, so each part starts 16-byte aligned
+        auto put = [&](TO* dst, const TO* lds, int m) {
+            V* d = reinterpret_cast<V*>(dst);
+            const V* l = reinterpret_cast<const V*>(lds);
+            for (int v = threadIdx.x; v < m / PER; v += UNPACK_ROWS) d[v] = l[v];
+        };
+        if (has_comps) put(comps + r0 * 3, sc, UNPACK_ROWS * 3);
+        put(norms + r0, sn, UNPACK_ROWS);
+        put(grads + r0 * 3, sg, UNPACK_ROWS * 3);
+    } else {
+        for (int e = threadIdx.x; e < rows * 3; e += UNPACK_ROWS) {
+            if (has_comps) comps[r0 * 3 + e] = sc[e];
+            grads[r0 * 3 + e] = sg[e];
+        }
+        for (int e = threadIdx.x; e < rows; e += UNPACK_ROWS) norms[r0 + e] = sn[e];
+    }
+}
+
+// Scatter: query index into bucket order.// Scatter: query index into bucket order.
 __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
                           const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1554,6 +1620,22 @@ hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, i
                              hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int32_t w = arb_packed_width(has_comps, has_norm, ndim);
+    const bool aligned = ((uintptr_t)comps | (uintptr_t)norms | (uintptr_t)grads) % 16 == 0;
+    if (ndim == 3 && has_norm && aligned) {
+        const dim3 grid(grid1(n, UNPACK_ROWS));
+#define L_UNPACK4(TO, WW)                                                                      \
+    hipLaunchKernelGGL((k_unpack_lds<TO, WW>), grid, dim3(UNPACK_ROWS), 0, s, (const TO*)rec,     \
+                       has_comps, (TO*)comps, (TO*)norms, (TO*)grads, n)
+        if (out_f64) {
+            if (w == 4) L_UNPACK4(double, 4);
+            else L_UNPACK4(double, 8);
+        } else {
+            if (w == 4) L_UNPACK4(float, 4);
+            else L_UNPACK4(float, 8);
+        }
+#undef L_UNPACK4
+        return hipGetLastError();
+    }
 #define L_UNPACK(TO, DD, WW)                                                                   \
     hipLaunchKernelGGL((k_unpack<TO, DD, WW>), dim3(grid1(n, 256)), dim3(256), 0, s,             \
                        (const TO*)rec, has_comps, has_norm, (TO*)comps, (TO*)norms,             \

@@ -81,6 +81,81 @@ k_rs_digit_scan(const int32_t* __restrict__ hist, int64_t ntiles, int32_t* __res
     if (threadIdx.x == 0) tot[blockIdx.x] = carry;
 }
 
+// Exclusive scan over the 256 digits of one value per thread (threads 0..255,
+// the first four waves) and the inclusive total, with wave shuffles and one
+// barrier; every thread of the workgroup must call it.
+__device__ __forceinline__ int32_t digits_exscan(int32_t v, int32_t* wsum, int32_t* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int32_t x = threadIdx.x < 256 ? v : 0;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (w < 4 && lane == 63) wsum[w] = x;
+    __syncthreads();
+    int32_t before = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) before += (k < w) ? wsum[k] : 0;
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    return before + x - (threadIdx.x < 256 ? v : 0);
+}
+
+// First pass of the LSD sort (the lowest digit; the values are the element
+// indices).  An LSD sort needs only its later passes to be stable, so each
+// pair's place inside its digit comes from an LDS atomic instead of the
+// stable wave-ballot ranking of k_rs_downsweep.  The tile's digit counts are
+// the first pass's histogram, made with the keys.
+__global__ void __launch_bounds__(RS_THREADS)
+k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
+           int32_t* __restrict__ vout, int64_t n, const int32_t* __restrict__ hist,
+           const int32_t* __restrict__ hscan, const int32_t* __restrict__ tot,
+           int64_t ntiles) {
+    __shared__ int32_t cnt[256];                // next free place per digit in the tile
+    __shared__ int32_t gbase[256];              // global start of the tile's part of each run
+    __shared__ int32_t toff[256];               // tile-local start of each digit's run
+    __shared__ int32_t wsum[2][4];
+    __shared__ uint32_t skey[RS_TILE];
+    __shared__ int32_t sval[RS_TILE];
+    const bool dt = threadIdx.x < 256;
+    const int dg0 = dt ? threadIdx.x : 0;
+    const int32_t tc = dt ? hist[(int64_t)dg0 * ntiles + blockIdx.x] : 0;
+    const int32_t gt = dt ? tot[dg0] : 0;
+    const int32_t hs = dt ? hscan[(int64_t)dg0 * ntiles + blockIdx.x] : 0;
+    int32_t ttot, gtot;
+    const int32_t tex = digits_exscan(tc, wsum[0], &ttot);
+    const int32_t gex = digits_exscan(gt, wsum[1], &gtot);
+    if (dt) {
+        toff[dg0] = tex;
+        cnt[dg0] = tex;
+        gbase[dg0] = gex + hs;
+    }
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = base + (int64_t)j * RS_THREADS + threadIdx.x;
+        if (e < n) {
+            const uint32_t k = __builtin_nontemporal_load(&kin[e]);
+            const int lp = atomicAdd(&cnt[k & 255u], 1);
+            // lp < RS_TILE: the tile holds tc[d] pairs of digit d, its run
+            // starts at toff[d] and the runs tile [0, RS_TILE)
+            skey[lp] = k;
+            sval[lp] = (int32_t)e;
+        }
+    }
+    __syncthreads();
+    const int cntt = (int)((n - base) < RS_TILE ? (n - base) : RS_TILE);
+    for (int i = threadIdx.x; i < cntt; i += RS_THREADS) {
+        const uint32_t k = skey[i];
+        const uint32_t dg = k & 255u;
+        const int64_t pos = (int64_t)gbase[dg] + (i - toff[dg]);
+        if (pos < 0 || pos >= n) continue;    // unreachable: a digit's run stays inside [0, n)
+        vout[pos] = sval[i];
+        kout[pos] = k;
+    }
+}
+
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int64_t n, int shift,
@@ -367,8 +442,13 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
         hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
                            tot);
         const int32_t* vin = (shift == 0 && index_vals) ? nullptr : va;
-        hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, vin,
-                           kb, vb, n, shift, hscan, tot, ntiles, last ? first : nullptr, gshift);
+        if (shift == 0 && index_vals && !last)
+            hipLaunchKernelGGL(k_rs_pass1, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, kb,
+                               vb, n, hist, hscan, tot, ntiles);
+        else
+            hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka,
+                               vin, kb, vb, n, shift, hscan, tot, ntiles, last ? first : nullptr,
+                               gshift);
         uint32_t* tk = ka; ka = kb; kb = tk;
         int32_t* tv = va; va = vb; vb = tv;
     }
