@@ -11,7 +11,9 @@ fallback follow the reference.
 
 Extensions (keyword arguments the reference does not have; its defaults are
 kept): `precision='fp64'` (default, plain FP64) or `'mixed'` (FP32 table and
-evaluation), `quiet=True` as a keyword, `boundary=None` (default, the
+evaluation), `table_limit=None` (bytes: a larger coefficient table is not
+held and queries form coefficients on the fly, as they do when the table
+does not fit in device memory), `quiet=True` as a keyword, `boundary=None` (default, the
 reference's box) or `'neumann'` / `'dirichlet'` (one ghost node layer per
 face, so the whole sampled box is interpolated; the attributes then describe
 the extended grid), and `query_device(q)` for positions already in GPU memory
@@ -97,6 +99,7 @@ class _Interp:
         self.eps = 10 * np.finfo(float).eps            # ARBInterp.py:11
         precision = kwargs.pop("precision", "fp64")
         boundary = kwargs.pop("boundary", None)
+        table_limit = kwargs.pop("table_limit", None)
         quiet = ("quiet" in args) or bool(kwargs.pop("quiet", False))
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -146,6 +149,8 @@ class _Interp:
             N.check(L.arb_field_set_boundary(h, _BC_CODE[boundary], _stream_handle(torch)),
                     "boundary")
         self.boundary = boundary
+        if table_limit is not None:
+            N.check(L.arb_set_table_limit(h, int(table_limit)), "table_limit")
         info = N.FieldInfo()
         N.check(L.arb_field_info_get(h, ctypes.byref(info)))
         self._info = info
@@ -202,11 +207,27 @@ class _Interp:
         return m
 
     def allCoeffs(self):
-        """Build the whole coefficient table on the GPU (ARBInterp.py:523-525)."""
+        """Build the whole coefficient table on the GPU (ARBInterp.py:523-525).
+        A table that does not fit in device memory (or in `table_limit`) is
+        not built: the field then forms each bucket's coefficients on the fly
+        in every query, with the same results, as the reference fills only
+        the cells its queries touch (ARBInterp.py:376-377)."""
         torch = _torch()
-        N.check(N.lib().arb_build_coeffs(self._h, _stream_handle(torch)), "allCoeffs")
+        rc = N.lib().arb_build_coeffs(self._h, _stream_handle(torch))
+        if rc == N.ARB_ENOMEM and self.field_info().on_the_fly:
+            return
+        N.check(rc, "allCoeffs")
         torch.cuda.current_stream().synchronize()
         self._info.table_built = 1
+
+    def set_table_limit(self, max_bytes):
+        """Largest coefficient table this field may hold (bytes; 0 = no limit
+        but device memory).  Above it, queries form coefficients on the fly
+        (bitwise the table's results); the calls that read the table itself
+        (coefficient_table, the backward, batches, the particle push) raise
+        MemoryError."""
+        N.check(N.lib().arb_set_table_limit(self._h, int(max_bytes)), "table_limit")
+        self._info = self.field_info()
 
     def timings(self):
         """Stage durations (ms) of the last query made with ARB_Q_TIMING:

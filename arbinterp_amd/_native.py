@@ -60,7 +60,7 @@ class FieldInfo(ctypes.Structure):
         ("int_min", ctypes.c_double * 4), ("int_max", ctypes.c_double * 4),
         ("nc", c_i64), ("nnodes", c_i64), ("table_bytes", c_i64),
         ("packed_width", c_i32), ("boundary", c_i32), ("block", c_i32 * 4),
-        ("tiles_bytes", c_i64),
+        ("tiles_bytes", c_i64), ("on_the_fly", c_i32), ("reserved", c_i32),
     ]
 
 
@@ -71,6 +71,7 @@ SIGNATURES = [
     ("arb_field_info_get", c_i32, [c_vp, ctypes.POINTER(FieldInfo)]),
     ("arb_field_set_boundary", c_i32, [c_vp, c_i32, c_vp]),
     ("arb_build_coeffs", c_i32, [c_vp, c_vp]),
+    ("arb_set_table_limit", c_i32, [c_vp, c_i64]),
     ("arb_query", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     ("arb_query_noalloc", c_i32, [c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     ("arb_query_point", c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -169,6 +169,8 @@ struct arb_field {
     ArbGeom g{};
     int32_t mode = 0, precision = 0, ninterp = 0, has_comps = 0, has_norm = 0;
     int32_t table_built = 0;
+    int64_t table_limit = 0;    // largest table the caller allows (0: no limit)
+    int32_t otf = 0;            // the table does not fit: coefficients formed on the fly
     int32_t sentinel_set = 0;   // sentinel row `sent` written
     int32_t table_loaded = 0;   // rows came from arb_table_load
     int32_t bc = ARB_BC_NONE;   // boundary extension (arb_field_set_boundary)
@@ -599,6 +601,7 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out) {
     out->boundary = f->bc;
     for (int a = 0; a < f->g.ndim; ++a) out->block[a] = f->B.nbt ? f->B.b[a] : 0;
     out->tiles_bytes = f->tiles_bytes;
+    out->on_the_fly = f->otf;
     return ARB_OK;
 }
 
@@ -663,8 +666,30 @@ static int ensure_sentinel(arb_field* f, hipStream_t s) {
     return ARB_OK;
 }
 
+// The table's storage, allocated by the first build.  A table larger than
+// the caller's limit (arb_set_table_limit), or one the device cannot hold,
+// switches the field to forming coefficients on the fly (otf): queries then
+// take the fused bucketed kernels, which give the table's bits, and the
+// calls that need the table itself return ARB_ENOMEM.
 static int ensure_table_alloc(arb_field* f) {
-    if (!f->table) HIPCHK(hipMalloc(&f->table, (size_t)f->table_bytes));
+    if (f->table) return ARB_OK;
+    if (!f->otf && f->table_limit > 0 && f->table_bytes > f->table_limit) f->otf = 1;
+    if (!f->otf) {
+        const hipError_t e = hipMalloc(&f->table, (size_t)f->table_bytes);
+        if (e == hipErrorOutOfMemory) {
+            (void)hipGetLastError();
+            f->table = nullptr;
+            f->otf = 1;
+        } else if (e != hipSuccess) {
+            f->table = nullptr;
+            return fail(ARB_EHIP, "hipMalloc of the coefficient table failed: %s",
+                        hipGetErrorString(e));
+        }
+    }
+    if (f->otf)
+        return fail(ARB_ENOMEM, "the coefficient table (%lld bytes) does not fit%s; queries form "
+                    "coefficients on the fly", (long long)f->table_bytes,
+                    f->table_limit > 0 ? " the table size limit" : " in device memory");
     return ARB_OK;
 }
 
@@ -807,19 +832,43 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
     }
     // node tiles (mixed 3D): the direct path, and the bucketed path unless
     // another grouping or the fused kernel is asked for; no table
-    const bool nodes = !fused && node_path(f, flags) &&
-                       !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC));
+    bool nodes = !fused && node_path(f, flags) &&
+                 !(flags & (ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC));
+    // a table path whose table does not fit: coefficients on the fly, in the
+    // fused bucketed kernels (3D and 4D), whatever path was asked for
+    bool fused4 = false;
+    if (!fused && !nodes && !f->table && !noalloc) {
+        const int rc0 = ensure_table_alloc(f);
+        if (rc0 && !f->otf) return rc0;
+    }
+    if (!fused && !nodes && f->otf) {
+        if (red32)
+            return fail(ARB_ENOMEM, "%s: the float32 reduction reads the coefficient table, which "
+                        "does not fit", fn);
+        if (n > INT32_MAX - 1)
+            return fail(ARB_EINVAL, "%s: too many rows for the on-the-fly path", fn);
+        flags &= ~(ARB_Q_DIRECT | ARB_Q_SORTED_GLOBAL | ARB_Q_GROUP_ATOMIC);
+        sorted = true;
+        if (f->BF.nbt == 0) choose_buckets(f, f->BF, false);
+        if (f->g.ndim == 3)
+            fused = arb_fused_lds(f->g, f->BF, f->ninterp, f->eval_threads) > 0;
+        else
+            fused4 = arb_fused4_lds(f->g, f->BF) > 0;
+        if (!fused && !fused4)
+            return fail(ARB_ENOMEM, "%s: the coefficient table does not fit and the bucket of "
+                        "this field is too large to form its coefficients on the fly", fn);
+    }
     const bool nodeform = nodes && sorted;
-    const ArbBuckets& BB = fused ? f->BF : f->B;
+    const ArbBuckets& BB = (fused || fused4) ? f->BF : f->B;
     if (noalloc) {
         if (nodes && !(f->tiles && f->tiles_bytes == arb_tiles_bytes(f->B, f->ninterp, nullptr)))
             return fail(ARB_EINVAL, "%s: the field's node tiles are not allocated; prepare the "
                         "field with one arb_query call with the same flags (n may be 0)", fn);
-        if (!fused && !nodes && !f->table)
+        if (!fused && !fused4 && !nodes && !f->table)
             return fail(ARB_EINVAL, "%s: the field's coefficient table is not allocated; "
                         "prepare the field with arb_build_coeffs", fn);
     }
-    int rc = (fused || nodes) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
+    int rc = (fused || fused4 || nodes) ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
     if (rc) return rc;
     if (nodes) {
         rc = ensure_tiles(f, s);
@@ -939,7 +988,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             // (a loaded table brings its own), the sentinel row otherwise
             const size_t esz = f->precision == ARB_PREC_FP64 ? sizeof(double) : sizeof(float);
             const void* sentrow =
-                (fused || nodes) ? f->sent
+                (fused || fused4 || nodes) ? f->sent
                                  : (const void*)((const char*)f->table +
                                                  esz * (size_t)f->g.nc * f->ninterp *
                                                      ((size_t)1 << (2 * f->g.ndim)));
@@ -969,6 +1018,10 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
                                          perm, f->eval_threads, s));
+        else if (fused4)
+            HIPCHK(arb_launch_eval_fused4(f->g, BB, f->precision, out64, f->vals, f->ninterp,
+                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
+                                          perm, f->eval_threads, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
@@ -1022,7 +1075,10 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     hipStream_t s = (hipStream_t)stream;
     const bool nodes = node_path(f, 0);
     int rc = nodes ? ensure_sentinel(f, s) : arb_build_coeffs(f, stream);
-    if (rc) return rc;
+    // no table (it does not fit): the point goes through the bucketed
+    // on-the-fly path as a one-row batch, with the same results
+    const bool otf_point = rc == ARB_ENOMEM && f->otf;
+    if (rc && !otf_point) return rc;
     if (nodes) {
         rc = ensure_tiles(f, s);
         if (rc) return rc;
@@ -1047,7 +1103,15 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     o.cells = reinterpret_cast<int64_t*>(f->pin_dev + PIN_CELL);
     o.ldq = d;
     o.flag = reinterpret_cast<int32_t*>(f->pin_dev + PIN_FLAG);
-    if (nodes)
+    bool past = false;           // the point reduced past the sentinel cell
+    if (otf_point) {
+        // with ARB_Q_CHECK the call synchronises and reports a row past the
+        // sentinel after the cell has been written
+        rc = query_impl(f, f->pin_dev + PIN_Q, 1, d, o.comps, o.norms, o.grads, o.cells,
+                        ARB_Q_SORTED | ARB_Q_OUT_F64 | ARB_Q_CHECK, stream, false);
+        past = rc == ARB_EINDEX;
+        if (rc && !past) return rc;
+    } else if (nodes)
         HIPCHK(arb_launch_query_tiles(f->g, f->B, f->ninterp, 1, f->tiles, f->sent, f->has_comps,
                                       f->has_norm, f->pin_dev + PIN_Q, 1, d, o, s));
     else
@@ -1062,7 +1126,7 @@ int arb_query_point(arb_field* f, const double* pt, double* comps, double* norm,
     }
     const int64_t flat = *reinterpret_cast<volatile int64_t*>(f->pin + PIN_CELL);
     if (cell) *cell = flat;
-    if (*hflag) return fail(ARB_EINDEX, "the point reduced to a cell index past the sentinel");
+    if (*hflag || past) return fail(ARB_EINDEX, "the point reduced to a cell index past the sentinel");
     return ARB_OK;
 }
 
@@ -1355,6 +1419,23 @@ void arb_batch_destroy(arb_batch* b) {
     if (b->counter) (void)hipFree(b->counter);
     if (b->flag) (void)hipFree(b->flag);
     delete b;
+}
+
+int arb_set_table_limit(arb_field* f, int64_t max_bytes) {
+    if (!f || max_bytes < 0) return fail(ARB_EINVAL, "arb_set_table_limit: bad argument");
+    f->table_limit = max_bytes;
+    if (max_bytes > 0 && f->table && f->table_bytes > max_bytes) {
+        if (f->table_loaded)
+            return fail(ARB_EINVAL, "the loaded coefficient table (%lld bytes) exceeds the limit",
+                        (long long)f->table_bytes);
+        HIPCHK(hipDeviceSynchronize());     // no queued work may still read the table
+        (void)hipFree(f->table);
+        f->table = nullptr;
+        f->table_built = 0;
+    }
+    // the table is tried again under the new limit at its next build
+    f->otf = (max_bytes > 0 && f->table_bytes > max_bytes) ? 1 : 0;
+    return ARB_OK;
 }
 
 int arb_last_timings(const arb_field* f, float* ms5) {

@@ -300,6 +300,15 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
+// 4D bucketed query without the table (coefficients formed per bucket with
+// the 4D table build's arithmetic); arb_fused4_lds is 0 where it does not
+// apply.
+size_t arb_fused4_lds(const ArbGeom& g, const ArbBuckets& B);
+hipError_t arb_launch_eval_fused4(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                  int32_t out_f64, const double* vals, int32_t ninterp,
+                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
 bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t has_comps,
                   int32_t has_norm);
 int64_t arb_tiles_bytes(const ArbBuckets& B, int32_t ninterp, int64_t* head_offset);

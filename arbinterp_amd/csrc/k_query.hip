@@ -558,7 +558,7 @@ k_unpack_lds(const TO* __restrict__ rec, int32_t has_comps, TO* __restrict__ com
     }
 }
 
-// Scatter: query index into bucket order.// Scatter: query index into bucket order.
+// Scatter: query index into bucket order.
 __global__ void k_scatter(const int32_t* __restrict__ bkt, const int32_t* __restrict__ rank,
                           const int32_t* __restrict__ offs, int32_t* __restrict__ perm, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -813,6 +813,184 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 #pragma unroll
         for (int a = 0; a < 3; ++a) qp[a] = q[pi * ldq + a];
         eval_row_lds<3, T, T, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4D without the coefficient table (the table does not fit, or a table size
+// limit is set): one workgroup per bucket forms its cells' coefficients in
+// LDS with the arithmetic of the 4D table build (k_coef.hip: per power of t
+// the t-combined 3D node tile through the separable passes, then the
+// row-shift correction P . c with c_k = G(k-1) - G(k), row 85 zero, and NaN
+// for every coefficient of a cell whose stencil holds a non-finite value),
+// so the results are bitwise those of the table, and evaluates the bucket's
+// queries as the staged kernel does.  Dynamic LDS: the coefficient block
+// (ncell * cellstride * 16 bytes), then doubles: the block's 4D node tile,
+// one t-combined 3D tile, the pass buffers, the corner differences G and
+// the per-cell c.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double hd4(int e, int k) {
+    // the derivative columns of the 1D Hermite map, [[0,0],[1,0],[-2,-1],[1,1]]
+    // (row = power 0..3, column = end 0/1)
+    return e == 0 ? 0.0 : e == 1 ? (k ? 0.0 : 1.0) : e == 2 ? (k ? -1.0 : -2.0) : 1.0;
+}
+
+template <int BX, int BY, int BZ, int BT>
+struct Fused4Shape {
+    static constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3, NTT = BT + 3;
+    static constexpr int NT4 = NX * NY * NZ * NTT;                 // node tile
+    static constexpr int NT3 = NX * NY * NZ;                       // t-combined tile
+    static constexpr int GX = BX + 1, GY = BY + 1, GZ = BZ + 1, GT = BT + 1;
+    static constexpr int NG = GX * GY * GZ * GT;                   // corner differences
+    static constexpr int NC = BX * BY * BZ * BT;
+    static constexpr int DOUBLES = NT4 + NT3 + (NY * NZ * BX * 4) + (NZ * BY * BX * 16) + NG + NC * 16;
+};
+
+template <typename T, typename TO, int BX, int BY, int BZ, int BT>
+__global__ void __launch_bounds__(256)
+k_eval_fused4(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
+              int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
+              int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
+              const int32_t* __restrict__ perm) {
+    using S = Fused4Shape<BX, BY, BZ, BT>;
+    extern __shared__ uint4 smem[];
+    __shared__ unsigned char cbad[S::NC];           // cells with a non-finite stencil
+    const int64_t b = blockIdx.x;
+    const int32_t start = offs[b], end = offs[b + 1];
+    if (start == end) return;                       // uniform over the workgroup
+    int64_t c0[4];
+    {
+        int64_t rem = b;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            c0[a] = (rem % B.nb[a]) << B.bsh[a];
+            rem /= B.nb[a];
+        }
+    }
+    double* nodes = reinterpret_cast<double*>(smem + B.ncell * B.cellstride);
+    double* t3 = nodes + S::NT4;
+    double* t1 = t3 + S::NT3;
+    double* t2 = t1 + S::NY * S::NZ * BX * 4;
+    double* gcorn = t2 + S::NZ * BY * BX * 16;
+    double* cc = gcorn + S::NG;
+    T* cf = reinterpret_cast<T*>(smem);
+    const int cstr = B.cellstride * (16 / (int)sizeof(T));   // elements per LDS cell
+    // this thread's first query, loaded before the coefficient work
+    int32_t s = start + threadIdx.x;
+    int64_t qi = -1;
+    double qp[4];
+    if (s < end) {
+        const int64_t pi = perm[s];
+        qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
+        const int64_t qq = qi >= 0 ? qi : 0;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) qp[a] = q[qq * ldq + a];
+    }
+    for (int k = 0; k < ninterp; ++k) {
+        const double* F = vals + (int64_t)k * g.nnodes;
+        __syncthreads();                            // the previous interpolant is done
+        // node tile, x fastest then y, z, t; nodes past the grid read 0 (only
+        // cells outside the grid use them)
+        int nonfinite = 0;
+        for (int e = threadIdx.x; e < S::NT4; e += blockDim.x) {
+            const int x = e % S::NX, y = (e / S::NX) % S::NY, z = (e / (S::NX * S::NY)) % S::NZ;
+            const int t = e / (S::NX * S::NY * S::NZ);
+            const int64_t nx = c0[0] + x, ny = c0[1] + y, nz = c0[2] + z, nt = c0[3] + t;
+            double v = 0.0;
+            if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2] && nt < g.n[3])
+                v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz + g.nstride[3] * nt];
+            nodes[e] = v;
+            nonfinite |= !isfinite(v);
+        }
+        const bool anybad = __syncthreads_or(nonfinite) != 0;
+        auto N4 = [&](int x, int y, int z, int t) {
+            return nodes[((t * S::NZ + z) * S::NY + y) * S::NX + x];
+        };
+        if (anybad) {
+            for (int c = threadIdx.x; c < S::NC; c += blockDim.x) {
+                const int cx = c % BX, cy = (c / BX) % BY, cz = (c / (BX * BY)) % BZ;
+                const int ct = c / (BX * BY * BZ);
+                bool bad = false;
+                for (int t = 0; t < 4; ++t)
+                    for (int z = 0; z < 4; ++z)
+                        for (int y = 0; y < 4; ++y)
+                            for (int x = 0; x < 4; ++x)
+                                bad |= !isfinite(N4(cx + x, cy + y, cz + z, ct + t));
+                cbad[c] = bad ? 1 : 0;
+            }
+        }
+        // xyzt central differences at the cells' corner nodes (tile node + 1),
+        // in the table build's order of terms
+        for (int i = threadIdx.x; i < S::NG; i += blockDim.x) {
+            const int gx = i % S::GX, gy = (i / S::GX) % S::GY, gz = (i / (S::GX * S::GY)) % S::GZ;
+            const int gt = i / (S::GX * S::GY * S::GZ);
+            double acc = 0.0;
+#pragma unroll
+            for (int m = 0; m < 16; ++m) {
+                const int ox = gx + 1 + ((m & 1) ? 1 : -1), oy = gy + 1 + ((m & 2) ? 1 : -1);
+                const int oz = gz + 1 + ((m & 4) ? 1 : -1), ot = gt + 1 + ((m & 8) ? 1 : -1);
+                const double v = N4(ox, oy, oz, ot);
+                acc += (__builtin_popcount(~m & 15) & 1) ? -v : v;
+            }
+            gcorn[i] = acc * 0.0625;
+        }
+        __syncthreads();
+        // c_k = G(corner k-1) - G(corner k) per cell
+        for (int i = threadIdx.x; i < S::NC * 16; i += blockDim.x) {
+            const int kk = i & 15, c = i >> 4;
+            const int cx = c % BX, cy = (c / BX) % BY, cz = (c / (BX * BY)) % BZ;
+            const int ct = c / (BX * BY * BZ);
+            auto G = [&](int cn) {
+                const int gx = cx + (cn & 1), gy = cy + ((cn >> 1) & 1);
+                const int gz = cz + ((cn >> 2) & 1), gt = ct + ((cn >> 3) & 1);
+                return gcorn[((gt * S::GZ + gz) * S::GY + gy) * S::GX + gx];
+            };
+            cc[c * 16 + kk] = (kk ? G(kk - 1) : 0.0) - G(kk);
+        }
+        for (int ct = 0; ct < BT; ++ct) {
+            for (int et = 0; et < 4; ++et) {
+                // t-combined 3D node tile for power et of cell row ct
+                for (int i = threadIdx.x; i < S::NT3; i += blockDim.x) {
+                    const int x = i % S::NX, y = (i / S::NX) % S::NY, z = i / (S::NX * S::NY);
+                    double c[4];
+                    arbs3::cub4(N4(x, y, z, ct), N4(x, y, z, ct + 1), N4(x, y, z, ct + 2),
+                                N4(x, y, z, ct + 3), c);
+                    t3[i] = c[et];
+                }
+                __syncthreads();
+                arbs3::sep3_passes<BX, BY, BZ, false>(
+                    t3, t1, t2, [&](int cx, int cy, int cz, int i3, double a) {
+                        const int ex = i3 & 3, ey = (i3 >> 2) & 3, ez = i3 >> 4;
+                        const int i = i3 + 64 * et;
+                        const int c = cx + BX * (cy + BY * (cz + BZ * ct));
+                        if (ex && ey && ez && et) {
+                            if (i == 85) {
+                                a = 0.0;            // the reference's row is zero
+                            } else {
+                                const double* cv = cc + c * 16;
+                                double corr = 0.0;
+#pragma unroll
+                                for (int m = 0; m < 16; ++m)
+                                    corr = fma(hd4(ex, m & 1) * hd4(ey, (m >> 1) & 1) *
+                                                   hd4(ez, (m >> 2) & 1) * hd4(et, (m >> 3) & 1),
+                                               cv[m], corr);
+                                a += corr;
+                            }
+                        }
+                        if (anybad && cbad[c]) a = NAN;
+                        cf[c * cstr + k * 256 + i] = (T)a;
+                    });
+            }
+        }
+    }
+    __syncthreads();
+    if (qi >= 0) eval_row_lds<4, T, T, TO>(g, B, smem, has_comps, has_norm, o, b, qi, qp);
+    for (s += blockDim.x; s < end; s += blockDim.x) {
+        const int64_t pi = perm[s];
+        if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
+#pragma unroll
+        for (int a = 0; a < 4; ++a) qp[a] = q[pi * ldq + a];
+        eval_row_lds<4, T, T, TO>(g, B, smem, has_comps, has_norm, o, b, pi, qp);
     }
 }
 
@@ -1735,7 +1913,55 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
     return hipErrorInvalidValue;
 }
 
-// Cell blocks the node-tile kernels are instantiated for.
+// Cell blocks of the 4D fused kernel (the 4D bucket ladder).
+#define ARB_FUSED4_SHAPES(X)                                                                  \
+    X(4, 4, 2, 2) X(4, 2, 2, 2) X(2, 2, 2, 2) X(2, 2, 2, 1) X(2, 2, 1, 1) X(2, 1, 1, 1)      \
+    X(1, 1, 1, 1)
+
+// LDS bytes of k_eval_fused4 for the bucket geometry B (0 when it does not
+// apply: 4D, an instantiated block shape, the local cell index laid out as
+// the kernel writes it, within 160 KiB with its static LDS).
+size_t arb_fused4_lds(const ArbGeom& g, const ArbBuckets& B) {
+    if (g.ndim != 4) return 0;
+    const int bx = B.b[0], by = B.b[1], bz = B.b[2], bt = B.b[3];
+    size_t dbl = 0;
+#define ARB_F4_IS(X, Y, Z, T)                                                                 \
+    if (bx == X && by == Y && bz == Z && bt == T) dbl = Fused4Shape<X, Y, Z, T>::DOUBLES;
+    ARB_FUSED4_SHAPES(ARB_F4_IS)
+#undef ARB_F4_IS
+    if (dbl == 0) return 0;
+    if (B.lsh[0] != 0 || (1 << B.lsh[1]) != bx || (1 << B.lsh[2]) != bx * by ||
+        (1 << B.lsh[3]) != bx * by * bz)
+        return 0;
+    const size_t lds = (size_t)B.ncell * B.cellstride * 16 + sizeof(double) * dbl;
+    return lds + 256 <= 160 * 1024 ? lds : 0;
+}
+
+hipError_t arb_launch_eval_fused4(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
+                                  int32_t out_f64, const double* vals, int32_t ninterp,
+                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
+                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
+                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
+    const size_t lds = arb_fused4_lds(g, B);
+    if (lds == 0) return hipErrorInvalidValue;
+    dim3 grid((unsigned)B.nbt), block(block_threads);
+#define L_FUSED4(T, TO, X, Y, Z, TT)                                                          \
+    hipLaunchKernelGGL((k_eval_fused4<T, TO, X, Y, Z, TT>), grid, block, lds, s, g, B, vals,  \
+                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
+#define ARB_FUSED4_CASE(X, Y, Z, TT)                                                          \
+    if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z && B.b[3] == TT) {                          \
+        if (prec == ARB_PREC_FP64) L_FUSED4(double, double, X, Y, Z, TT);                     \
+        else if (out_f64) L_FUSED4(float, double, X, Y, Z, TT);                               \
+        else L_FUSED4(float, float, X, Y, Z, TT);                                             \
+        return hipGetLastError();                                                             \
+    }
+    ARB_FUSED4_SHAPES(ARB_FUSED4_CASE)
+#undef ARB_FUSED4_CASE
+#undef L_FUSED4
+    return hipErrorInvalidValue;
+}
+
+// Cell blocks the node-tile kernels are instantiated for.// Cell blocks the node-tile kernels are instantiated for.
 #define ARB_NODE_SHAPES(X)                                                                    \
     X(8, 8, 4) X(8, 4, 4) X(4, 4, 4) X(8, 4, 2) X(4, 4, 2) X(4, 2, 2) X(2, 2, 2) X(2, 2, 1)    \
     X(2, 1, 1) X(1, 1, 1)

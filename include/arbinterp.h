@@ -140,6 +140,10 @@ typedef struct arb_field_info {
                               path (0 before its first use or arb_set_tuning) */
     int64_t tiles_bytes;   /* bytes of the node tiles (mixed 3D bucketed
                               path; 0 until its first query)                  */
+    int32_t on_the_fly;    /* 1 when the coefficient table does not fit (device
+                              memory or arb_set_table_limit): queries form
+                              coefficients on the fly                         */
+    int32_t reserved;
 } arb_field_info;
 
 /* Analyse a field and upload it (tricubic/quadcubic constructor minus the
@@ -172,8 +176,23 @@ int arb_field_info_get(const arb_field* f, arb_field_info* out);
 int arb_field_set_boundary(arb_field* f, int32_t bc, void* stream);
 
 /* Fill the whole coefficient table (allCoeffs, ARBInterp.py:523-525,
- * :1260-1262, with calcCoefficients :570-603 / :1322-1355).  Idempotent. */
+ * :1260-1262, with calcCoefficients :570-603 / :1322-1355).  Idempotent.
+ * When the table does not fit in device memory, or exceeds the limit of
+ * arb_set_table_limit, it is not built: ARB_ENOMEM, and the field forms
+ * coefficients on the fly from then on (arb_field_info.on_the_fly). */
 int arb_build_coeffs(arb_field* f, void* stream);
+
+/* Largest coefficient table the field may hold, in bytes (0: no limit other
+ * than device memory; a held table above a new limit is released).  Fields
+ * without a table answer every query on the fly, as the reference fills only
+ * the cells that queries touch (ARBInterp.py:376-377, :489-490): 3D and 4D
+ * table paths use fused bucketed kernels that form each bucket's
+ * coefficients in LDS with the table build's arithmetic, so the results are
+ * bitwise those of the table (mixed 3D fields need no table for their node
+ * tiles); single points run as one-row batches.  arb_build_coeffs,
+ * arb_table_copy, arb_table_load, arb_query_backward, arb_batch_create,
+ * arb_push_rk4 and ARB_Q_REDUCE_F32 need the table and return ARB_ENOMEM. */
+int arb_set_table_limit(arb_field* f, int64_t max_bytes);
 
 /* Batch query (Query/rQuery, ARBInterp.py:177-211,344-521 / :880-914,1064-1258).
  * q_dev: n x ldq FP64 positions (first d columns used).  Outputs (device,
