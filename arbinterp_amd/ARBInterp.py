@@ -140,8 +140,10 @@ class _Interp:
         L = N.lib()
         h = ctypes.c_void_p()
         prec = N.ARB_PREC_FP64 if precision == "fp64" else N.ARB_PREC_MIXED
+        # a float32 array is normed in float32 (numpy.linalg.norm keeps its dtype)
+        mcode = _MODE_CODE[kind] | (N.ARB_MODE_F32_NORM if self._field_f32 else 0)
         rc = L.arb_field_create(ctypes.c_void_p(rows.data_ptr()), rows.shape[0], ncols,
-                                rows.shape[1], d, _MODE_CODE[kind], prec,
+                                rows.shape[1], d, mcode, prec,
                                 _stream_handle(torch), ctypes.byref(h))
         N.check(rc, "field")
         self._h = h

@@ -24,6 +24,7 @@ ARB_MODE_VECTOR = 1
 ARB_MODE_NORM = 2
 ARB_MODE_BOTH = 3
 ARB_MODE_VECTOR_NAN = 4
+ARB_MODE_F32_NORM = 0x100
 
 ARB_BC_NONE = 0
 ARB_BC_NEUMANN = 1

@@ -454,6 +454,8 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
         return fail(ARB_EINVAL, "arb_field_create: bad rows/nrows/ld");
     if (precision != ARB_PREC_FP64 && precision != ARB_PREC_MIXED)
         return fail(ARB_EINVAL, "unknown precision %d", precision);
+    const int32_t f32_norm = mode & ARB_MODE_F32_NORM;
+    mode &= ~ARB_MODE_F32_NORM;
     if (ncols == ndim + 1) mode = ARB_MODE_SCALAR;
     else if (mode == ARB_MODE_SCALAR || mode < 0 || mode > ARB_MODE_VECTOR_NAN)
         return fail(ARB_EINVAL, "vector field needs mode vector/norm/both (got %d)", mode);
@@ -551,8 +553,8 @@ int arb_field_create(const double* rows_dev, int64_t nrows, int32_t ncols, int64
     HIPCHK(seen.alloc(sizeof(int32_t) * g.nnodes));
     HIPCHK(hipMemsetAsync(seen.p, 0, sizeof(int32_t) * g.nnodes, s));
     HIPCHK(hipMalloc(&f->vals, sizeof(double) * g.nnodes * f->ninterp));
-    HIPCHK(arb_launch_rank_scatter(rows_dev, nrows, ld, axp, g, mode, ndim, f->vals, f->ninterp,
-                                   seen.as<int32_t>(), s));
+    HIPCHK(arb_launch_rank_scatter(rows_dev, nrows, ld, axp, g, mode | f32_norm, ndim, f->vals,
+                                   f->ninterp, seen.as<int32_t>(), s));
     HIPCHK(arb_launch_check_seen(seen.as<int32_t>(), g.nnodes, f->flag, s));
     HIPCHK(hipMemcpyAsync(&hflag, f->flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));

@@ -98,14 +98,26 @@ __global__ void k_rank_scatter(const double* __restrict__ rows, int64_t nrows, i
     }
     atomicAdd(&seen[node], 1);
     const int64_t nn = g.nnodes;
+    const bool f32_norm = (mode & ARB_MODE_F32_NORM) != 0;
+    mode &= ~ARB_MODE_F32_NORM;
     if (mode == ARB_MODE_SCALAR) {
         vals[node] = r[vcol0];
         return;
     }
     double bx = r[vcol0], by = r[vcol0 + 1], bz = r[vcol0 + 2];
     // |B| as numpy.linalg.norm(axis=1) forms it: sqrt((x*x + y*y) + z*z);
-    // compiled without FP contraction, so each product is rounded.
-    double bn = sqrt((bx * bx + by * by) + bz * bz);
+    // compiled without FP contraction, so each product is rounded.  A float32
+    // source array is normed in float32 (products, sums and the square root
+    // each rounded to float32); the float32 square root is the FP64 one
+    // rounded once more, which is exact for float32 operands (53 >= 2*24 + 2).
+    double bn;
+    if (f32_norm) {
+        const float x = (float)bx, y = (float)by, z = (float)bz;
+        const float s2 = (x * x + y * y) + z * z;
+        bn = (double)(float)sqrt((double)s2);
+    } else {
+        bn = sqrt((bx * bx + by * by) + bz * bz);
+    }
     if (mode == ARB_MODE_NORM) {
         vals[node] = bn;
         return;

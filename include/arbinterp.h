@@ -58,6 +58,10 @@ extern "C" {
                                NaN (ARBInterp.py:59-74)                          */
 #define ARB_MODE_VECTOR_NAN 4 /* no mode / unknown mode: components, sentinel
                                NaN (ARBInterp.py:75-102)                         */
+#define ARB_MODE_F32_NORM 0x100 /* OR'ed into the mode of arb_field_create: the
+                               rows hold float32 values, so |B| is formed in
+                               float32 as numpy.linalg.norm does on a float32
+                               array (ARBInterp.py:58,74,677,693)              */
 
 /* ---- precision --------------------------------------------------------- */
 #define ARB_PREC_FP64   0   /* plain: FP64 everywhere (separable coefficient
