@@ -80,9 +80,11 @@ __host__ __device__ inline int sep3_t2(int bx, int by, int bz) { return (bz + 3)
 // values, so it does not depend on the tile shape, the layout or the thread
 // order.  t1, t2: scratch of sep3_t1 / sep3_t2 doubles (16-byte aligned).
 // The caller synchronises before it reuses nodes, t1 or t2.
-template <int BX, int BY, int BZ, bool CELL_FAST, typename OUT>
-__device__ __forceinline__ void sep3_passes(const double* nodes, double* t1, double* t2,
-                                            const OUT& out) {
+// sep3_passes4 hands each z-pass thread's four results over at once:
+// out4(cx, cy, cz, exy, c) with c[ez] = alpha_{exy + 16 ez}.
+template <int BX, int BY, int BZ, bool CELL_FAST, typename OUT4>
+__device__ __forceinline__ void sep3_passes4(const double* nodes, double* t1, double* t2,
+                                             const OUT4& out4) {
     constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
     // x
     for (int t = threadIdx.x; t < BX * NY * NZ; t += blockDim.x) {
@@ -147,9 +149,18 @@ __device__ __forceinline__ void sep3_passes(const double* nodes, double* t1, dou
             constexpr int st = BY * BX * 16;            // next z
             cub4(n[0], n[st], n[2 * st], n[3 * st], c);
         }
+        out4(cx, cy, cz, exy, c);
+    }
+}
+
+template <int BX, int BY, int BZ, bool CELL_FAST, typename OUT>
+__device__ __forceinline__ void sep3_passes(const double* nodes, double* t1, double* t2,
+                                            const OUT& out) {
+    sep3_passes4<BX, BY, BZ, CELL_FAST>(nodes, t1, t2,
+                                        [&](int cx, int cy, int cz, int exy, const double* c) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) out(cx, cy, cz, exy + 16 * e, c[e]);
-    }
+    });
 }
 
 }  // namespace arbs3

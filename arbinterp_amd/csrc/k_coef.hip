@@ -6,7 +6,7 @@
 // FP32 (mixed precision), by separable passes:
 //   3D: A = M (x) M (x) M (arb_sep3.h), about 625 FMAs per cell;
 //   4D: A = M (x) M (x) M (x) M + a rank-16 correction from the reference's
-//       row shift (below).
+//       row shift (arb_sep4.h).
 //
 // Sentinel row nc, the reference's alpha*[:, nc]: zero for a scalar field
 // (ARBInterp.py:31; 4D :650) and for an explicit mode='vector' (whose NaN
@@ -14,7 +14,7 @@
 // (:57, :70, :85, :99; 4D :676, :689, :704, :718).  Rows that reduce to the
 // sentinel cell are evaluated from it.
 #include "arb_internal.h"
-#include "arb_sep3.h"
+#include "arb_sep4.h"
 
 namespace {
 
@@ -85,117 +85,99 @@ k_coef_sep3(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
     }
 }
 
-// 4D, both precisions.  A4 = M (x) M (x) M (x) M + P . dD exactly, where
-//   P = Hd (x) Hd (x) Hd (x) Hd, Hd = the derivative columns of the 1D
-//       Hermite map, [[0,0],[1,0],[-2,-1],[1,1]] (row = power, column = end);
-//   dD row k = S(k-1) - S(k), S(k) the xyzt central-difference stencil at
-//       corner k (S(-1) = 0): the reference's row shift in the d4/dxdydzdt
-//       block (ARBInterp.py:860).
-// So alpha = the separable M passes (per power of t: the t-combined 3D node
-// tile through arbs3::sep3_passes) + P . c, c_k = G(k-1) - G(k), G the xyzt
-// central difference at the cell's corner nodes (shared by neighbouring
-// cells).  P vanishes unless every power is >= 1 (81 of 256 rows), and row
-// 85 (u v w t) of A4 is identically zero: alpha_85 is set to 0 exactly.
-// Tile: 8 x 4 cells in (x, y) at one (z, t); FP64, stored as TO.
-__device__ __forceinline__ double hd(int e, int k) {
-    // Hd[e][k]: e = power 0..3, k = end 0/1
-    return e == 0 ? 0.0 : e == 1 ? (k ? 0.0 : 1.0) : e == 2 ? (k ? -1.0 : -2.0) : 1.0;
-}
+// 4D, both precisions: alpha = the separable M passes + the row-shift
+// correction (arb_sep4.h), in FP64, stored as TO.  Tile: 8 x 4 x SZ4 cells
+// in (x, y, z) at one t.  Per interpolant: the 11 x 7 x (SZ4+3) x 4 node
+// tile, the corner differences G and the per-cell c_k from it, then the node
+// tile is replaced in place by its four powers of t (cub4 along t; each
+// thread rewrites the nodes it read), and each power's 3D slice goes through
+// the separable passes, whose z-pass threads add the correction for their
+// (ex, ey) and store four coefficients.
+constexpr int SZ4 = 2;
 
 template <typename TO>
 __global__ void __launch_bounds__(256)
 k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __restrict__ table) {
-    constexpr int NX = SX + 3, NY = SY + 3;             // node tile 11 x 7 x 4 x 4
-    constexpr int GX = SX + 1, GY = SY + 1;             // corner nodes 9 x 5 x 2 x 2
-    __shared__ double n4[4][4][NY][NX];
-    __shared__ double n3[4 * NY * NX];
-    __shared__ __align__(16) double t1[NY * 4 * SX * 4];
-    __shared__ __align__(16) double t2[4 * SY * SX * 16];
-    __shared__ double gc[2][2][GY][GX];
-    __shared__ double cc[SX * SY][16];
-    __shared__ unsigned char cbad[SX * SY];             // cells with a non-finite stencil
+    constexpr int NX = SX + 3, NY = SY + 3, NZ = SZ4 + 3;
+    constexpr int GX = SX + 1, GY = SY + 1, GZ = SZ4 + 1;     // corner nodes, GT = 2
+    constexpr int NC = SX * SY * SZ4;
+    __shared__ __align__(16) double n4[4][NZ][NY][NX];
+    __shared__ __align__(16) double t1[NY * NZ * SX * 4];
+    __shared__ __align__(16) double t2[NZ * SY * SX * 16];
+    __shared__ double gc[2][GZ][GY][GX];
+    __shared__ __align__(16) double cc[NC][16];
+    __shared__ unsigned char cbad[NC];                  // cells with a non-finite stencil
+    const int64_t nzt = ((int64_t)g.npos[2] + SZ4 - 1) / SZ4;
     const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
-    const int64_t cz = blockIdx.z % g.npos[2], ct = blockIdx.z / g.npos[2];
+    const int64_t cz0 = (int64_t)(blockIdx.z % nzt) * SZ4, ct = blockIdx.z / nzt;
+    auto N4 = [&](int x, int y, int z, int t) { return n4[t][z][y][x]; };
     for (int k = 0; k < ninterp; ++k) {
         const double* F = vals + (int64_t)k * g.nnodes;
         __syncthreads();                            // the previous interpolant is done
+        // node tile; nodes past the grid read 0 (only cells outside it use them)
         int nonfinite = 0;
-        for (int i = threadIdx.x; i < 4 * 4 * NY * NX; i += blockDim.x) {
-            const int x = i % NX, y = (i / NX) % NY, z = (i / (NX * NY)) & 3, t = i / (NX * NY * 4);
-            const int64_t nx = cx0 + x, ny = cy0 + y;
+        for (int i = threadIdx.x; i < 4 * NZ * NY * NX; i += blockDim.x) {
+            const int x = i % NX, y = (i / NX) % NY, z = (i / (NX * NY)) % NZ;
+            const int t = i / (NX * NY * NZ);
+            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz0 + z;
             double v = 0.0;
-            if (nx < g.n[0] && ny < g.n[1])
-                v = F[nx + g.nstride[1] * ny + g.nstride[2] * (cz + z) + g.nstride[3] * (ct + t)];
+            if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
+                v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz + g.nstride[3] * (ct + t)];
             (&n4[0][0][0][0])[i] = v;
             nonfinite |= !isfinite(v);
         }
         const bool anybad = __syncthreads_or(nonfinite) != 0;
         if (anybad) {
             // per cell: a non-finite value among its 4^4 stencil nodes
-            for (int c = threadIdx.x; c < SX * SY; c += blockDim.x) {
-                const int cx = c % SX, cy = c / SX;
+            for (int c = threadIdx.x; c < NC; c += blockDim.x) {
+                const int cx = c % SX, cy = (c / SX) % SY, cz = c / (SX * SY);
                 bool b = false;
                 for (int t = 0; t < 4; ++t)
                     for (int z = 0; z < 4; ++z)
                         for (int y = 0; y < 4; ++y)
-                            for (int x = 0; x < 4; ++x) b |= !isfinite(n4[t][z][cy + y][cx + x]);
+                            for (int x = 0; x < 4; ++x) b |= !isfinite(N4(cx + x, cy + y, cz + z, t));
                 cbad[c] = b ? 1 : 0;
             }
-            __syncthreads();
         }
-        // xyzt central differences at the corner nodes (tile node + 1)
-        for (int i = threadIdx.x; i < 2 * 2 * GY * GX; i += blockDim.x) {
-            const int gx = i % GX, gy = (i / GX) % GY, gz = (i / (GX * GY)) & 1, gt = i / (GX * GY * 2);
-            double acc = 0.0;
+        for (int i = threadIdx.x; i < 2 * GZ * GY * GX; i += blockDim.x) {
+            const int gx = i % GX, gy = (i / GX) % GY, gz = (i / (GX * GY)) % GZ;
+            const int gt = i / (GX * GY * GZ);
+            (&gc[0][0][0][0])[i] = arbs4::corner_diff(N4, gx, gy, gz, gt);
+        }
+        __syncthreads();                            // n4 read, gc written
+        // powers of t in place, and c_k = G(corner k-1) - G(corner k) per cell
+        for (int i = threadIdx.x; i < NZ * NY * NX; i += blockDim.x) {
+            double* p = &n4[0][0][0][0] + i;
+            constexpr int st = NZ * NY * NX;
+            double c[4];
+            arbs3::cub4(p[0], p[st], p[2 * st], p[3 * st], c);
 #pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const int ox = gx + 1 + ((m & 1) ? 1 : -1), oy = gy + 1 + ((m & 2) ? 1 : -1);
-                const int oz = gz + 1 + ((m & 4) ? 1 : -1), ot = gt + 1 + ((m & 8) ? 1 : -1);
-                const double v = n4[ot][oz][oy][ox];
-                acc += (__builtin_popcount(~m & 15) & 1) ? -v : v;
-            }
-            (&gc[0][0][0][0])[i] = acc * 0.0625;
+            for (int e = 0; e < 4; ++e) p[e * st] = c[e];
         }
-        __syncthreads();
-        // c_k = G(corner k-1) - G(corner k) per cell
-        for (int i = threadIdx.x; i < SX * SY * 16; i += blockDim.x) {
-            const int kk = i & 15, cell = i >> 4, cx = cell % SX, cy = cell / SX;
-            auto G = [&](int c) { return gc[(c >> 3) & 1][(c >> 2) & 1][cy + ((c >> 1) & 1)][cx + (c & 1)]; };
+        for (int i = threadIdx.x; i < NC * 16; i += blockDim.x) {
+            const int kk = i & 15, cell = i >> 4;
+            const int cx = cell % SX, cy = (cell / SX) % SY, cz = cell / (SX * SY);
+            auto G = [&](int c) {
+                return gc[(c >> 3) & 1][cz + ((c >> 2) & 1)][cy + ((c >> 1) & 1)][cx + (c & 1)];
+            };
             cc[cell][kk] = (kk ? G(kk - 1) : 0.0) - G(kk);
         }
+        __syncthreads();
         for (int et = 0; et < 4; ++et) {
-            // t-combined 3D node tile for power et
-            for (int i = threadIdx.x; i < 4 * NY * NX; i += blockDim.x) {
-                const int x = i % NX, y = (i / NX) % NY, z = i / (NX * NY);
-                double c[4];
-                arbs3::cub4(n4[0][z][y][x], n4[1][z][y][x], n4[2][z][y][x], n4[3][z][y][x], c);
-                n3[i] = c[et];
-            }
-            __syncthreads();
-            arbs3::sep3_passes<SX, SY, 1, false>(n3, t1, t2, [&](int cx, int cy, int, int i3, double a) {
-                const int ex = i3 & 3, ey = (i3 >> 2) & 3, ez = i3 >> 4;
-                const int i = i3 + 64 * et;
-                if (ex && ey && ez && et) {
-                    if (i == 85) {
-                        a = 0.0;                    // the reference's row is zero
-                    } else {
-                        const double* c = cc[cx + SX * cy];
-                        double corr = 0.0;
+            arbs3::sep3_passes4<SX, SY, SZ4, false>(
+                &n4[et][0][0][0], t1, t2, [&](int cx, int cy, int cz, int exy, const double* c0) {
+                    const int cell = cx + SX * (cy + SY * cz);
+                    double c[4] = {c0[0], c0[1], c0[2], c0[3]};
+                    arbs4::shift_rows(cc[cell], exy & 3, exy >> 2, et, c);
+                    const int64_t x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
+                    if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
+                        const int64_t cl = x + g.npos[0] * (y + g.npos[1] * (z + g.npos[2] * ct));
+                        const bool bad = anybad && cbad[cell];
+                        TO* o = table + (cl * ninterp + k) * 256 + exy + 64 * et;
 #pragma unroll
-                        for (int m = 0; m < 16; ++m)
-                            corr = fma(hd(ex, m & 1) * hd(ey, (m >> 1) & 1) * hd(ez, (m >> 2) & 1) *
-                                           hd(et, (m >> 3) & 1),
-                                       c[m], corr);
-                        a += corr;
+                        for (int ez = 0; ez < 4; ++ez) o[16 * ez] = bad ? (TO)NAN : (TO)c[ez];
                     }
-                }
-                const int64_t x = cx0 + cx, y = cy0 + cy;
-                if (x < g.npos[0] && y < g.npos[1]) {
-                    const int64_t cell = x + g.npos[0] * (y + g.npos[1] * (cz + g.npos[2] * ct));
-                    if (anybad && cbad[cx + SX * cy]) a = NAN;
-                    table[(cell * ninterp + k) * 256 + i] = (TO)a;
-                }
-            });
+                });
         }
     }
 }
@@ -217,7 +199,8 @@ hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, i
         else
             hipLaunchKernelGGL(k_coef_sep3<float>, grid, block, 0, s, g, vals, ninterp, (float*)table);
     } else {
-        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY), (unsigned)(g.npos[2] * g.npos[3]));
+        const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY),
+                        cdiv(g.npos[2], SZ4) * (unsigned)g.npos[3]);
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL(k_coef_sep4<double>, grid, block, 0, s, g, vals, ninterp, (double*)table);
         else

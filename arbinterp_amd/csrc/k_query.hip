@@ -5,7 +5,7 @@
 // gradients.  The reference builds ~40 N x 64 temporaries per call; here each
 // query is one lane.
 #include "arb_internal.h"
-#include "arb_sep3.h"
+#include "arb_sep4.h"
 
 namespace {
 
@@ -819,9 +819,9 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 // ---------------------------------------------------------------------------
 // 4D without the coefficient table (the table does not fit, or a table size
 // limit is set): one workgroup per bucket forms its cells' coefficients in
-// LDS with the arithmetic of the 4D table build (k_coef.hip: per power of t
+// LDS with the arithmetic of the 4D table build (arb_sep4.h: per power of t
 // the t-combined 3D node tile through the separable passes, then the
-// row-shift correction P . c with c_k = G(k-1) - G(k), row 85 zero, and NaN
+// row-shift correction P . c with c_k = G(k-1) - G(k), row 85 zero; and NaN
 // for every coefficient of a cell whose stencil holds a non-finite value),
 // so the results are bitwise those of the table, and evaluates the bucket's
 // queries as the staged kernel does.  Dynamic LDS: the coefficient block
@@ -829,12 +829,6 @@ k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
 // one t-combined 3D tile, the pass buffers, the corner differences G and
 // the per-cell c.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double hd4(int e, int k) {
-    // the derivative columns of the 1D Hermite map, [[0,0],[1,0],[-2,-1],[1,1]]
-    // (row = power 0..3, column = end 0/1)
-    return e == 0 ? 0.0 : e == 1 ? (k ? 0.0 : 1.0) : e == 2 ? (k ? -1.0 : -2.0) : 1.0;
-}
-
 template <int BX, int BY, int BZ, int BT>
 struct Fused4Shape {
     static constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3, NTT = BT + 3;
@@ -924,15 +918,7 @@ k_eval_fused4(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
         for (int i = threadIdx.x; i < S::NG; i += blockDim.x) {
             const int gx = i % S::GX, gy = (i / S::GX) % S::GY, gz = (i / (S::GX * S::GY)) % S::GZ;
             const int gt = i / (S::GX * S::GY * S::GZ);
-            double acc = 0.0;
-#pragma unroll
-            for (int m = 0; m < 16; ++m) {
-                const int ox = gx + 1 + ((m & 1) ? 1 : -1), oy = gy + 1 + ((m & 2) ? 1 : -1);
-                const int oz = gz + 1 + ((m & 4) ? 1 : -1), ot = gt + 1 + ((m & 8) ? 1 : -1);
-                const double v = N4(ox, oy, oz, ot);
-                acc += (__builtin_popcount(~m & 15) & 1) ? -v : v;
-            }
-            gcorn[i] = acc * 0.0625;
+            gcorn[i] = arbs4::corner_diff(N4, gx, gy, gz, gt);
         }
         __syncthreads();
         // c_k = G(corner k-1) - G(corner k) per cell
@@ -958,27 +944,15 @@ k_eval_fused4(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
                     t3[i] = c[et];
                 }
                 __syncthreads();
-                arbs3::sep3_passes<BX, BY, BZ, false>(
-                    t3, t1, t2, [&](int cx, int cy, int cz, int i3, double a) {
-                        const int ex = i3 & 3, ey = (i3 >> 2) & 3, ez = i3 >> 4;
-                        const int i = i3 + 64 * et;
+                arbs3::sep3_passes4<BX, BY, BZ, false>(
+                    t3, t1, t2, [&](int cx, int cy, int cz, int exy, const double* c0) {
                         const int c = cx + BX * (cy + BY * (cz + BZ * ct));
-                        if (ex && ey && ez && et) {
-                            if (i == 85) {
-                                a = 0.0;            // the reference's row is zero
-                            } else {
-                                const double* cv = cc + c * 16;
-                                double corr = 0.0;
+                        double a[4] = {c0[0], c0[1], c0[2], c0[3]};
+                        arbs4::shift_rows(cc + c * 16, exy & 3, exy >> 2, et, a);
+                        const bool bad = anybad && cbad[c];
+                        T* o = cf + c * cstr + k * 256 + exy + 64 * et;
 #pragma unroll
-                                for (int m = 0; m < 16; ++m)
-                                    corr = fma(hd4(ex, m & 1) * hd4(ey, (m >> 1) & 1) *
-                                                   hd4(ez, (m >> 2) & 1) * hd4(et, (m >> 3) & 1),
-                                               cv[m], corr);
-                                a += corr;
-                            }
-                        }
-                        if (anybad && cbad[c]) a = NAN;
-                        cf[c * cstr + k * 256 + i] = (T)a;
+                        for (int ez = 0; ez < 4; ++ez) o[16 * ez] = bad ? (T)NAN : (T)a[ez];
                     });
             }
         }
