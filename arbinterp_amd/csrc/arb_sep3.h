@@ -61,7 +61,8 @@ __host__ __device__ inline int sep3_nodes(int bx, int by, int bz) {
 __host__ __device__ inline int sep3_t1(int bx, int by, int bz) { return (by + 3) * (bz + 3) * bx * 4; }
 __host__ __device__ inline int sep3_t2(int bx, int by, int bz) { return (bz + 3) * by * bx * 16; }
 
-// Block-wide (every thread must call it; it synchronises between passes):
+// Block-wide (every thread of a SEP_THREADS-thread workgroup must call it; it
+// synchronises between passes):
 // nodes[(z * NY + y) * NX + x] (NX = BX+3, NY = BY+3, NZ = BZ+3) holds the
 // node tile whose (0,0,0) is the neighbourhood origin of cell (0,0,0).  For
 // every cell (cx, cy, cz) of the tile and coefficient i = ex + 4 ey + 16 ez,
@@ -81,13 +82,16 @@ __host__ __device__ inline int sep3_t2(int bx, int by, int bz) { return (bz + 3)
 // order.  t1, t2: scratch of sep3_t1 / sep3_t2 doubles (16-byte aligned).
 // The caller synchronises before it reuses nodes, t1 or t2.
 // sep3_passes4 hands each z-pass thread's four results over at once:
-// out4(cx, cy, cz, exy, c) with c[ez] = alpha_{exy + 16 ez}.
+// out4(cx, cy, cz, exy, c) with c[ez] = alpha_{exy + 16 ez}.  The workgroup
+// has SEP_THREADS threads (a compile-time stride lets the loops unroll).
+constexpr int SEP_THREADS = 256;
+
 template <int BX, int BY, int BZ, bool CELL_FAST, typename OUT4>
 __device__ __forceinline__ void sep3_passes4(const double* nodes, double* t1, double* t2,
                                              const OUT4& out4) {
     constexpr int NX = BX + 3, NY = BY + 3, NZ = BZ + 3;
     // x
-    for (int t = threadIdx.x; t < BX * NY * NZ; t += blockDim.x) {
+    for (int t = threadIdx.x; t < BX * NY * NZ; t += SEP_THREADS) {
         const int cx = t % BX, yz = t / BX;
         const double* n = nodes + yz * NX + cx;
         double c[4];
@@ -103,7 +107,7 @@ __device__ __forceinline__ void sep3_passes4(const double* nodes, double* t1, do
     }
     __syncthreads();
     // y
-    for (int t = threadIdx.x; t < BX * BY * NZ * 4; t += blockDim.x) {
+    for (int t = threadIdx.x; t < BX * BY * NZ * 4; t += SEP_THREADS) {
         double c[4];
         if constexpr (CELL_FAST) {
             const int cx = t % BX;
@@ -134,7 +138,7 @@ __device__ __forceinline__ void sep3_passes4(const double* nodes, double* t1, do
     __syncthreads();
     // z: alpha_i, i = ex + 4 ey + 16 ez
     constexpr int NC = BX * BY * BZ;
-    for (int t = threadIdx.x; t < NC * 16; t += blockDim.x) {
+    for (int t = threadIdx.x; t < NC * 16; t += SEP_THREADS) {
         const int exy = CELL_FAST ? t / NC : (t & 15);
         const int cell = CELL_FAST ? t % NC : (t >> 4);
         const int cx = cell % BX, cy = (cell / BX) % BY, cz = cell / (BX * BY);

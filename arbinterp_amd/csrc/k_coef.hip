@@ -107,22 +107,24 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
     __shared__ double gc[2][GZ][GY][GX];
     __shared__ __align__(16) double cc[NC][16];
     __shared__ unsigned char cbad[NC];                  // cells with a non-finite stencil
-    const int64_t nzt = ((int64_t)g.npos[2] + SZ4 - 1) / SZ4;
-    const int64_t cx0 = (int64_t)blockIdx.x * SX, cy0 = (int64_t)blockIdx.y * SY;
-    const int64_t cz0 = (int64_t)(blockIdx.z % nzt) * SZ4, ct = blockIdx.z / nzt;
-    auto N4 = [&](int x, int y, int z, int t) { return n4[t][z][y][x]; };
+    // 32-bit node offsets and cell indices: the launcher admits grids of
+    // fewer than 2^31 nodes (a table of such a grid holds 2^39 bytes)
+    const int32_t n0 = (int32_t)g.n[0], n1 = (int32_t)g.n[1], n2 = (int32_t)g.n[2];
+    const int32_t s1 = (int32_t)g.nstride[1], s2 = (int32_t)g.nstride[2], s3 = (int32_t)g.nstride[3];
+    const int32_t p0 = (int32_t)g.npos[0], p1 = (int32_t)g.npos[1], p2 = (int32_t)g.npos[2];
+    const int32_t nzt = (p2 + SZ4 - 1) / SZ4;
+    const int32_t cx0 = blockIdx.x * SX, cy0 = blockIdx.y * SY;
+    const int32_t cz0 = (blockIdx.z % nzt) * SZ4, ct = blockIdx.z / nzt;
     for (int k = 0; k < ninterp; ++k) {
-        const double* F = vals + (int64_t)k * g.nnodes;
+        const double* F = vals + (int64_t)k * g.nnodes + (cx0 + s1 * cy0 + s2 * cz0 + s3 * ct);
         __syncthreads();                            // the previous interpolant is done
         // node tile; nodes past the grid read 0 (only cells outside it use them)
         int nonfinite = 0;
         for (int i = threadIdx.x; i < 4 * NZ * NY * NX; i += blockDim.x) {
             const int x = i % NX, y = (i / NX) % NY, z = (i / (NX * NY)) % NZ;
             const int t = i / (NX * NY * NZ);
-            const int64_t nx = cx0 + x, ny = cy0 + y, nz = cz0 + z;
             double v = 0.0;
-            if (nx < g.n[0] && ny < g.n[1] && nz < g.n[2])
-                v = F[nx + g.nstride[1] * ny + g.nstride[2] * nz + g.nstride[3] * (ct + t)];
+            if (cx0 + x < n0 && cy0 + y < n1 && cz0 + z < n2) v = F[x + s1 * y + s2 * z + s3 * t];
             (&n4[0][0][0][0])[i] = v;
             nonfinite |= !isfinite(v);
         }
@@ -135,14 +137,15 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
                 for (int t = 0; t < 4; ++t)
                     for (int z = 0; z < 4; ++z)
                         for (int y = 0; y < 4; ++y)
-                            for (int x = 0; x < 4; ++x) b |= !isfinite(N4(cx + x, cy + y, cz + z, t));
+                            for (int x = 0; x < 4; ++x) b |= !isfinite(n4[t][cz + z][cy + y][cx + x]);
                 cbad[c] = b ? 1 : 0;
             }
         }
         for (int i = threadIdx.x; i < 2 * GZ * GY * GX; i += blockDim.x) {
             const int gx = i % GX, gy = (i / GX) % GY, gz = (i / (GX * GY)) % GZ;
             const int gt = i / (GX * GY * GZ);
-            (&gc[0][0][0][0])[i] = arbs4::corner_diff(N4, gx, gy, gz, gt);
+            (&gc[0][0][0][0])[i] =
+                arbs4::corner_diff<NX, NX * NY, NX * NY * NZ>(&n4[gt + 1][gz + 1][gy + 1][gx + 1]);
         }
         __syncthreads();                            // n4 read, gc written
         // powers of t in place, and c_k = G(corner k-1) - G(corner k) per cell
@@ -169,11 +172,11 @@ k_coef_sep4(ArbGeom g, const double* __restrict__ vals, int32_t ninterp, TO* __r
                     const int cell = cx + SX * (cy + SY * cz);
                     double c[4] = {c0[0], c0[1], c0[2], c0[3]};
                     arbs4::shift_rows(cc[cell], exy & 3, exy >> 2, et, c);
-                    const int64_t x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
-                    if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
-                        const int64_t cl = x + g.npos[0] * (y + g.npos[1] * (z + g.npos[2] * ct));
+                    const int32_t x = cx0 + cx, y = cy0 + cy, z = cz0 + cz;
+                    if (x < p0 && y < p1 && z < p2) {
+                        const uint32_t cl = (uint32_t)(x + p0 * (y + p1 * (z + p2 * ct)));
                         const bool bad = anybad && cbad[cell];
-                        TO* o = table + (cl * ninterp + k) * 256 + exy + 64 * et;
+                        TO* o = table + ((uint64_t)cl * (uint32_t)ninterp + k) * 256 + exy + 64 * et;
 #pragma unroll
                         for (int ez = 0; ez < 4; ++ez) o[16 * ez] = bad ? (TO)NAN : (TO)c[ez];
                     }
@@ -199,6 +202,7 @@ hipError_t arb_launch_coef(const ArbGeom& g, int32_t prec, const double* vals, i
         else
             hipLaunchKernelGGL(k_coef_sep3<float>, grid, block, 0, s, g, vals, ninterp, (float*)table);
     } else {
+        if (g.nnodes > INT32_MAX) return hipErrorInvalidValue;   // 32-bit offsets
         const dim3 grid(cdiv(g.npos[0], SX), cdiv(g.npos[1], SY),
                         cdiv(g.npos[2], SZ4) * (unsigned)g.npos[3]);
         if (prec == ARB_PREC_FP64)

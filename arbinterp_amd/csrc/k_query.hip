@@ -918,7 +918,8 @@ k_eval_fused4(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t 
         for (int i = threadIdx.x; i < S::NG; i += blockDim.x) {
             const int gx = i % S::GX, gy = (i / S::GX) % S::GY, gz = (i / (S::GX * S::GY)) % S::GZ;
             const int gt = i / (S::GX * S::GY * S::GZ);
-            gcorn[i] = arbs4::corner_diff(N4, gx, gy, gz, gt);
+            gcorn[i] = arbs4::corner_diff<S::NX, S::NX * S::NY, S::NX * S::NY * S::NZ>(
+                nodes + (((gt + 1) * S::NZ + gz + 1) * S::NY + gy + 1) * S::NX + gx + 1);
         }
         __syncthreads();
         // c_k = G(corner k-1) - G(corner k) per cell
@@ -1842,8 +1843,11 @@ hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t
 
 // LDS bytes of k_eval_fused3 for the bucket geometry B (0 when the path does
 // not apply: 3D only, an instantiated block shape, within 160 KiB, node tiles
-// of all interpolants within the registers of `threads` threads).
+// of all interpolants within the registers of the workgroup).  The fused
+// kernels run arbs3::SEP_THREADS threads per workgroup whatever the tuning's
+// thread count (the separable passes stride by it).
 size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads) {
+    (void)threads;
     if (g.ndim != 3) return 0;
     const int bx = B.b[0], by = B.b[1], bz = B.b[2];
     bool shape = false;
@@ -1853,7 +1857,8 @@ size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int
     if (!shape) return 0;
     // the local cell index must be cx + bx (cy + by cz), as the kernel writes it
     if (B.lsh[0] != 0 || (1 << B.lsh[1]) != bx || (1 << B.lsh[2]) != bx * by) return 0;
-    if ((int64_t)ninterp * arbs3::sep3_nodes(bx, by, bz) > (int64_t)FUSED_REGS * threads) return 0;
+    if ((int64_t)ninterp * arbs3::sep3_nodes(bx, by, bz) > (int64_t)FUSED_REGS * arbs3::SEP_THREADS)
+        return 0;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16 +
                        sizeof(double) * (size_t)(arbs3::sep3_nodes(bx, by, bz) +
                                                  arbs3::sep3_t1(bx, by, bz) +
@@ -1868,9 +1873,10 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
-    const size_t lds = arb_fused_lds(g, B, ninterp, block_threads);
+    (void)block_threads;
+    const size_t lds = arb_fused_lds(g, B, ninterp, arbs3::SEP_THREADS);
     if (lds == 0) return hipErrorInvalidValue;
-    dim3 grid((unsigned)B.nbt), block(block_threads);
+    dim3 grid((unsigned)B.nbt), block(arbs3::SEP_THREADS);
 #define L_FUSED(T, TO, X, Y, Z)                                                               \
     hipLaunchKernelGGL((k_eval_fused3<T, TO, X, Y, Z>), grid, block, lds, s, g, B, vals,      \
                        ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
@@ -1916,9 +1922,10 @@ hipError_t arb_launch_eval_fused4(const ArbGeom& g, const ArbBuckets& B, int32_t
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, const int32_t* offs,
                                   const int32_t* perm, int32_t block_threads, hipStream_t s) {
+    (void)block_threads;
     const size_t lds = arb_fused4_lds(g, B);
     if (lds == 0) return hipErrorInvalidValue;
-    dim3 grid((unsigned)B.nbt), block(block_threads);
+    dim3 grid((unsigned)B.nbt), block(arbs3::SEP_THREADS);
 #define L_FUSED4(T, TO, X, Y, Z, TT)                                                          \
     hipLaunchKernelGGL((k_eval_fused4<T, TO, X, Y, Z, TT>), grid, block, lds, s, g, B, vals,  \
                        ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
