@@ -56,7 +56,7 @@ def test_backward_matches_oracle(mods, precision):
     # the residuals are of the order of the values, and this metric (floor
     # 1e-3 of the largest component, over 20 000 points) sees their FP32
     # rounding: measured 4.1e-5.  The finite-difference check at full size
-    # holds the north-star bar of 1e-5 (measured 2.1e-6).
+    # holds the 1e-5 bar (measured 2.1e-6).
     tol = 1e-10 if precision == "fp64" else 5e-5
     e_q = _rel(x.grad.cpu().numpy(), dq_ref)
     e_v = _rel(fld.values.grad.cpu().numpy(), dval_ref)
