@@ -5,8 +5,9 @@ workspace carved from a fixed static device array, so its outputs must be
 bitwise those of arb_query (comps, norms, grads, cell indices, NaN rows
 written into the positions).  Checked through the C ABI on two calls in a
 row, the second with fewer rows (the array keeps the first call's contents),
-for the mixed 3D node-tile path, the FP64 3D table path and a mixed 4D
-field; device memory must not change across the calls; bad sizes, refused
+for the mixed 3D node-tile path, the FP64 3D table path, a mixed 4D field
+and FP64 3D and 4D fields without a table (coefficients on the fly); device
+memory must not change across the calls; bad sizes, refused
 flags and unprepared fields give ARB_EINVAL.
 """
 import ctypes
@@ -79,6 +80,11 @@ CASES = [
      "tricubic", {}, "mixed", N.ARB_Q_TABLE, (200_000, 70_000)),
     ("mixed4d_scalar", lambda: W.moving4d(seed=10, n=(20, 18, 16, 10), nq=1)[0], "quadcubic",
      {}, "mixed", 0, (250_000, 90_000)),
+    # no table (a 1-byte table limit): coefficients formed on the fly
+    ("fp64_3d_both_on_the_fly", lambda: W.dipole3d(seed=8, n=(40, 36, 32), nq=1)[0],
+     "tricubic", {"mode": "both", "table_limit": 1}, "fp64", 0, (300_000, 120_000)),
+    ("fp64_4d_scalar_on_the_fly", lambda: W.moving4d(seed=10, n=(20, 18, 16, 10), nq=1)[0],
+     "quadcubic", {"table_limit": 1}, "fp64", 0, (250_000, 90_000)),
 ]
 
 
@@ -94,8 +100,10 @@ def test_noalloc_matches_query(torch, case):
     # prepare the field (table or node tiles) with an allocating call
     q0 = torch.zeros((1, t.NDIM), dtype=torch.float64, device="cuda")
     N.check(_call(torch, L.arb_query, t, q0[:0], _outputs(torch, t, 0, dt), fl), "prepare")
-    if prec == "fp64" or flags & N.ARB_Q_TABLE or t.NDIM == 4:
+    otf = "table_limit" in kw
+    if (prec == "fp64" or flags & N.ARB_Q_TABLE or t.NDIM == 4) and not otf:
         N.check(L.arb_build_coeffs(t._h, _stream(torch)), "build")
+    assert t.field_info().on_the_fly == (1 if otf else 0)
     qs = [torch.from_numpy(_queries(t, n, 11 + k)).cuda() for k, n in enumerate(sizes)]
     refs, q_refs = [], []
     for q in qs:
