@@ -169,3 +169,29 @@ def test_on_the_fly_refuses_table_readers(G):
         B.push_rk4(t, x, torch.zeros_like(x), dt=1e-3, nsteps=2)
     torch.cuda.synchronize()
     t.close()
+
+
+@pytest.mark.parametrize("d,bc", [(3, "neumann"), (3, "dirichlet"), (4, "neumann")])
+def test_on_the_fly_boundary_fields(G, d, bc):
+    """Boundary-extended fields (one ghost layer per face) without a table:
+    the fused kernels read the extended node grid, so the whole sampled box
+    gives the table's bits."""
+    n = (12, 11, 10) if d == 3 else (8, 7, 7, 6)
+    rows, axes = _field(d, n, 1, seed=20 + d)
+    cls = G.tricubic if d == 3 else G.quadcubic
+    tag = "%dD %s" % (d, bc)
+    print(tag, flush=True)
+    t_tab = cls(rows.copy(), "quiet", boundary=bc)
+    t_otf = cls(rows.copy(), "quiet", boundary=bc, table_limit=1)
+    # positions over the whole sampled box (the rim the boundary makes valid)
+    rng = np.random.default_rng(4)
+    lo = np.array([a[0] for a in axes])
+    hi = np.array([a[-1] for a in axes])
+    q = rng.uniform(lo, hi, size=(12_000, d))
+    for name, fl in (("bucketed", N.ARB_Q_SORTED), ("direct", N.ARB_Q_DIRECT)):
+        _bitwise(_run(t_otf, q, fl), _run(t_tab, q, fl), tag + " " + name)
+    got = _run(t_otf, q, N.ARB_Q_SORTED)
+    assert np.isfinite(got[1]).all(), tag                 # no NaN rim
+    assert t_otf.field_info().on_the_fly == 1
+    t_tab.close()
+    t_otf.close()
