@@ -88,6 +88,16 @@ def _check_positions(torch, q, d, what):
                          % (what, d))
 
 
+def _pinned_copy(torch, x):
+    """An asynchronous copy of device tensor x into new pinned host memory
+    (None for None); the caller synchronises the stream before reading it."""
+    if x is None:
+        return None
+    h = torch.empty(x.shape, dtype=x.dtype, pin_memory=True)
+    h.copy_(x, non_blocking=True)
+    return h
+
+
 class _Interp:
     """Shared implementation of the 3D and 4D classes."""
 
@@ -397,16 +407,18 @@ class _Interp:
         cell (the reference's alphamask lookup does)."""
         d = self.NDIM
         torch = _torch()
-        # whole caller rows with a coordinate strictly outside the box
-        oob = np.zeros(len(query), dtype=bool)
-        for a in range(d):
-            col = query[:, a]
-            oob |= (col < self._lo[a]) | (col > self._hi[a])
-        if oob.any():
-            query[np.where(oob)[0]] = np.nan
         n = len(query)
         qh = np.ascontiguousarray(query[:, :d], dtype=np.float64)
         q_dev = torch.from_numpy(qh).to("cuda", non_blocking=False)
+        # whole caller rows with a coordinate strictly outside the box (the
+        # same comparisons as numpy's on the caller's values, made on the
+        # device copy; float32 values widen exactly); the kernels evaluate
+        # such rows from the sentinel cell whether or not they hold NaN
+        lo = torch.tensor([float(self._lo[a]) for a in range(d)], dtype=torch.float64,
+                          device="cuda")
+        hi = torch.tensor([float(self._hi[a]) for a in range(d)], dtype=torch.float64,
+                          device="cuda")
+        oob = ((q_dev < lo) | (q_dev > hi)).any(dim=1)
         out_dt = torch.float64
         comps, norms, grads = self._alloc_out(torch, n, out_dt)
         cells = torch.empty((n,), dtype=torch.int64, device="cuda")
@@ -415,14 +427,21 @@ class _Interp:
         if self._field_f32 and query.dtype == np.float32:
             flags |= N.ARB_Q_REDUCE_F32
         rc = self._run(torch, q_dev, n, d, comps, norms, grads, cells, flags)
-        self.queryInds = cells.cpu().numpy()
+        # results come back through pinned host buffers (one synchronisation)
+        host = [_pinned_copy(torch, x) for x in (comps, norms, grads, cells)]
+        nbad = int(oob.sum())
+        bad = torch.nonzero(oob).squeeze(1).cpu().numpy() if nbad else None
+        torch.cuda.current_stream().synchronize()
+        if bad is not None:
+            query[bad] = np.nan
+        self.queryInds = host[3].numpy()
         if rc == N.ARB_EINDEX:
             raise IndexError("index %d is out of bounds for axis 0 with size %d"
                              % (int(self.queryInds.max()), self.nc + 1))
         N.check(rc, "query")
-        out_c = comps.cpu().numpy() if comps is not None else None
-        out_n = norms.cpu().numpy().reshape(n, 1) if norms is not None else None
-        out_g = grads.cpu().numpy() if grads is not None else None
+        out_c = host[0].numpy() if comps is not None else None
+        out_n = host[1].numpy().reshape(n, 1) if norms is not None else None
+        out_g = host[2].numpy() if grads is not None else None
         if self._variant == 1:
             return out_c
         if self._variant == 2:

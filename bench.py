@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--cpu-chunks", type=int, default=6,
                     help="2^16-query chunks timed for the CPU baseline (0: skip)")
     ap.add_argument("--err-sample", type=int, default=200_000)
+    ap.add_argument("--host-calls", type=int, default=2,
+                    help="timed calls of the drop-in's numpy Query on the whole batch (PCIe "
+                         "included; 0: skip)")
     ap.add_argument("--flags", type=int, default=0, help="extra ARB_Q_* flags")
     ap.add_argument("--variants", default="", help="comma list of query paths to time "
                     "(auto,direct,sorted_lds,sorted_table,sorted_fused,sorted_cell,sorted_atomic,sorted_global); default all")
@@ -174,7 +177,20 @@ def main():
         # loops above leave the last variant's rows in the buffers)
         out = t.query_device(qd, cells=True, out=out)
         torch.cuda.synchronize()
-        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per)
+        host = None
+        if prec == "mixed" and args.host_calls > 0:
+            # the drop-in's numpy Query on the same batch: positions and
+            # outputs cross PCIe (a reported rate, not the value)
+            qh = q.copy()
+            t.Query(qh)
+            th = time.time()
+            for _ in range(args.host_calls):
+                t.Query(qh)
+            hms = (time.time() - th) * 1e3 / args.host_calls
+            host = {"ms_per_call": hms, "queries_per_s": args.nq / hms * 1e3,
+                    "calls": args.host_calls}
+            log("host numpy Query: %.2f ms per call" % hms)
+        res[prec] = dict(obj=t, build_ms=build_ms, out=out, per=per, host=host)
     nc = res["mixed"]["obj"].nc
     cells = res["mixed"]["out"][3]
     valid = cells[cells < nc]
@@ -226,7 +242,7 @@ def main():
     st64 = res["fp64"]["per"]["auto"]["stage_ms"]
     alg_bytes = N * (24 + 7 * 4) + U * 4 * 64 * 4
     alg64 = N * (24 + 7 * 8) + U * 4 * 64 * 8
-    node_bytes = N * (24 + 7 * 4) + UB * tile_bytes
+    node_bytes = N * (16 + 4 + 32) + UB * tile_bytes   # row record, row index, output record
     ev_ms = st[3] or ms
     ev64 = st64[3] or ms64
     achieved = alg_bytes / (ev_ms * 1e-3) / 1e9
@@ -309,6 +325,7 @@ def main():
             "speedup_p64_vs_fp64_p64": per_d["auto_p64"]["ms_per_step"] / per_m["auto_p64"]["ms_per_step"],
         },
         "paths": {p: res[p]["per"] for p in ("mixed", "fp64")},
+        "host_numpy_path": res["mixed"]["host"],
         "one_off_build_ms": {"mixed_node_tiles": res["mixed"]["build_ms"],
                              "fp64_table": res["fp64"]["build_ms"]},
         "rel_err_vs_fp64_oracle": err,
