@@ -276,13 +276,14 @@ constexpr int ARB_RS_ITEMS = 16;
 // the keys' low 8 bits (hist[digit * ntiles + tile]).  The value of key i is
 // the row index i, implicit (the radix sort's index_vals).  Rows outside the
 // box are evaluated here from the sentinel row `sent` (ninterp x K values).
-// With qrec (3D node-tile path) every row inside the box also gets its
-// 16-byte record: FP32 local coordinates and the local cell in its bucket.
+// With qrec (3D node-tile path) every row i inside the box also gets its
+// 16-byte record, at qrec + i * qstride (in 16-byte units): FP32 local
+// coordinates and the local cell in its bucket.
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* sent, int32_t has_comps, int32_t has_norm,
                            const double* q, int64_t n, int64_t ldq, const ArbOut& o,
                            uint32_t* key, int32_t cellbits, int32_t* hist, void* qrec,
-                           hipStream_t s);
+                           int32_t qstride, hipStream_t s);
 int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
@@ -325,9 +326,12 @@ hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t n
                                 const double* gv, const double* gg, double* dq, int32_t* key,
                                 int32_t* count, double* rec, hipStream_t s);
 // Node-tile evaluation of the bucketed path; qrec: the per-row records k_keys
-// wrote (n x 16 bytes: FP32 local coordinates, local cell in the bucket).
+// wrote (16 bytes each at a stride of qstride x 16 bytes: FP32 local
+// coordinates, local cell in the bucket).  The records may live in the
+// output records themselves (row i's output overwrites its own record).
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
-                                 int32_t out_f64, const void* tiles, const void* qrec, int64_t n,
+                                 int32_t out_f64, const void* tiles, const void* qrec,
+                                 int32_t qstride, int64_t n,
                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
                                  int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,

@@ -346,7 +346,7 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
                                                const TC* __restrict__ sent, int32_t has_comps,
                                                int32_t has_norm, const ArbOut& o, int64_t i,
                                                const double* qp, int32_t cellbits,
-                                               float4* __restrict__ qrec) {
+                                               float4* __restrict__ qrec, int32_t qstride) {
     double loc[D];
     int64_t ia[D];
     bool oob;
@@ -368,7 +368,10 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
         if (qrec) {
             float lf[3];
             const uint32_t lcq = loc_encode(loc, lf) | (uint32_t)lc;
-            qrec[i] = make_float4(lf[0], lf[1], lf[2], __uint_as_float(lcq));
+            qrec[i * qstride] = make_float4(lf[0], lf[1], lf[2], __uint_as_float(lcq));
+            // a 32-byte slot is written whole: whole-line stores, where a
+            // half-written line costs a partial write below the cache
+            if (qstride == 2) qrec[i * 2 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
     return (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
@@ -379,7 +382,7 @@ __global__ void __launch_bounds__(KEYS_THREADS)
 k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
        uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
-       int64_t ntiles, float4* __restrict__ qrec) {
+       int64_t ntiles, float4* __restrict__ qrec, int32_t qstride) {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;                           // KEYS_THREADS == 256 == digits
     __syncthreads();
@@ -398,7 +401,7 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
             const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
             if (i < n) {
                 const uint32_t k = key_of_row<D, T, TC, TO>(g, B, sent, has_comps, has_norm, o,
-                                                            i, qv[u], cellbits, qrec);
+                                                            i, qv[u], cellbits, qrec, qstride);
                 key[i] = k;
                 atomicAdd(&cnt[k & 255u], 1);
             }
@@ -1597,7 +1600,7 @@ k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restri
 template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
-              const TileHead* __restrict__ heads, const float4* __restrict__ qrec, int64_t n,
+              const TileHead* __restrict__ heads, const float4* qrec, int32_t qstride, int64_t n,
               ArbOut o, const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
     using S = TileShape<NI, BX, BY, BZ>;
     using RB = typename TileRec<NI>::bits;
@@ -1612,7 +1615,7 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restr
     if (s < end) {
         const int64_t pi = perm[s];
         qi = (pi >= 0 && pi < n) ? pi : -1;         // always in range: perm holds row indices
-        r = qrec[qi >= 0 ? qi : 0];
+        r = qrec[(qi >= 0 ? qi : 0) * qstride];
     }
     // stage the tile (read once per call: nontemporal) and its head
     const RB* src = reinterpret_cast<const RB*>(tiles + b * S::NTP);
@@ -1634,7 +1637,7 @@ k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restr
     for (s += blockDim.x; s < end; s += blockDim.x) {
         const int64_t pi = perm[s];
         if (pi < 0 || pi >= n) continue;            // unreachable: perm holds row indices
-        const float4 rr = qrec[pi];
+        const float4 rr = qrec[pi * qstride];
         const float lf[3] = {rr.x, rr.y, rr.z};
         tile_eval<TO, NI, BX, BY, BZ, S::YS, S::ZS>(g, B, tile, h, o, lf, __float_as_uint(rr.w),
                                                     pi);
@@ -2025,7 +2028,8 @@ hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t n
 }
 
 hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
-                                 int32_t out_f64, const void* tiles, const void* qrec, int64_t n,
+                                 int32_t out_f64, const void* tiles, const void* qrec,
+                                 int32_t qstride, int64_t n,
                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
                                  int32_t block_threads, hipStream_t s) {
     int64_t hoff = 0;
@@ -2035,7 +2039,7 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
 #define L_NODES(TO, NI, X, Y, Z)                                                              \
     hipLaunchKernelGGL((k_eval_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,             \
                        (const typename TileRec<NI>::type*)tiles, heads, (const float4*)qrec,  \
-                       n, o, offs, perm)
+                       qstride, n, o, offs, perm)
 #define L_NODES_NI(TO, X, Y, Z)                                                               \
     if (ninterp == 4) L_NODES(TO, 4, X, Y, Z);                                                \
     else if (ninterp == 3) L_NODES(TO, 3, X, Y, Z);                                           \
@@ -2103,13 +2107,13 @@ hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, 
                            const void* sent, int32_t has_comps, int32_t has_norm,
                            const double* q, int64_t n, int64_t ldq, const ArbOut& o,
                            uint32_t* key, int32_t cellbits, int32_t* hist, void* qrec,
-                           hipStream_t s) {
+                           int32_t qstride, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
                        s, g, B, (const TC*)sent, has_comps, has_norm, q, n, ldq, o, key,         \
-                       cellbits, hist, ntiles, (float4*)qrec)
+                       cellbits, hist, ntiles, (float4*)qrec, qstride)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS
