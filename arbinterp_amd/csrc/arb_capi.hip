@@ -1030,8 +1030,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                                         perm, s));
         else if (nodeform)
             HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, w.qrec, qstride,
-                                         n, o,
-                                         offs, perm, f->eval_threads, s));
+                                         n, o, offs, perm, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
