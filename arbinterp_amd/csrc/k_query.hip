@@ -339,42 +339,71 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
 constexpr int KEYS_THREADS = 256;
 constexpr int KEYS_ITEMS = ARB_RS_THREADS * ARB_RS_ITEMS / KEYS_THREADS;
 constexpr int KEYS_BATCH = 4;
-static_assert(KEYS_ITEMS % KEYS_BATCH == 0, "rows per thread must be a multiple of the batch");
+static_assert(KEYS_ITEMS * KEYS_THREADS == ARB_RS_THREADS * ARB_RS_ITEMS, "a tile per workgroup");
+static_assert(KEYS_THREADS == 256, "a thread per digit");
+static_assert(KEYS_ITEMS % KEYS_BATCH == 0 && KEYS_ITEMS <= 32,
+              "rows per thread: a multiple of the batch, one bit each in a 32-bit mask");
 
-template <int D, typename T, typename TC, typename TO>
+// The row's record for the node-tile evaluation (3D): its local coordinates
+// (FP32 and corrections) and its local cell in the bucket.  *has = false for
+// a row that reduces to the sentinel (no record); *sent_row = true for such a
+// row, which the caller evaluates from the sentinel row afterwards.
+template <int D>
 __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBuckets& B,
-                                               const TC* __restrict__ sent, int32_t has_comps,
-                                               int32_t has_norm, const ArbOut& o, int64_t i,
-                                               const double* qp, int32_t cellbits,
-                                               float4* __restrict__ qrec, int32_t qstride) {
+                                               const ArbOut& o, int64_t i, const double* qp,
+                                               int32_t cellbits, float4* rec, bool* has,
+                                               bool* sent_row) {
     double loc[D];
     int64_t ia[D];
     bool oob;
     const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
+    *has = false;
+    *sent_row = false;
     if (o.cells) o.cells[i] = flat;
     if (oob && o.qmut) {
         for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
     }
     if (flat >= g.nc) {
         if (flat > g.nc) o.flag[0] = 1;
-        eval_point<D, T, TC, TO>(sent, has_comps, has_norm, loc, g, o, i);
+        *sent_row = true;
         return (uint32_t)B.nbt << cellbits;
     }
     int lc;
     const uint32_t b = (uint32_t)bucket_of<D>(flat, ia, g, B, &lc);
     if constexpr (D == 3) {
-        // the row's record for the node-tile evaluation: its local
-        // coordinates (FP32 and corrections) and local cell in the bucket
-        if (qrec) {
-            float lf[3];
-            const uint32_t lcq = loc_encode(loc, lf) | (uint32_t)lc;
-            qrec[i * qstride] = make_float4(lf[0], lf[1], lf[2], __uint_as_float(lcq));
-            // a 32-byte slot is written whole: whole-line stores, where a
-            // half-written line costs a partial write below the cache
-            if (qstride == 2) qrec[i * 2 + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
+        float lf[3];
+        const uint32_t lcq = loc_encode(loc, lf) | (uint32_t)lc;
+        *rec = make_float4(lf[0], lf[1], lf[2], __uint_as_float(lcq));
+        *has = true;
     }
     return (b << cellbits) | (cellbits ? (uint32_t)lc : 0u);
+}
+
+// Store the records of a wave's 64 consecutive rows (row0 .. row0 + 63; lane
+// l holds row row0 + l).  16-byte slots (qstride 1): one store per lane.
+// 32-byte slots (qstride 2): each slot is written whole, its second half
+// zero, by two adjacent lanes, so that every store instruction covers 1 KiB
+// of consecutive bytes (where a lane writing both halves of its own slot
+// would leave every line half-written by each instruction).  Every lane of
+// the wave must call it.
+__device__ __forceinline__ void store_recs(float4* __restrict__ qrec, int32_t qstride, int64_t row0,
+                                           int64_t n, float4 r, bool has) {
+    const int lane = threadIdx.x & 63;
+    if (qstride == 1) {
+        if (has) qrec[row0 + lane] = r;
+        return;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int src = (lane >> 1) + 32 * h;
+        const float x = __shfl(r.x, src, 64), y = __shfl(r.y, src, 64);
+        const float z = __shfl(r.z, src, 64), w = __shfl(r.w, src, 64);
+        const int hs = __shfl((int)has, src, 64);
+        const int64_t row = row0 + src;
+        if (hs && row < n)
+            qrec[row * 2 + (lane & 1)] =
+                (lane & 1) ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(x, y, z, w);
+    }
 }
 
 template <int D, typename T, typename TC, typename TO>
@@ -384,9 +413,10 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
        uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
        int64_t ntiles, float4* __restrict__ qrec, int32_t qstride) {
     __shared__ int32_t cnt[256];
-    cnt[threadIdx.x] = 0;                           // KEYS_THREADS == 256 == digits
+    cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * (KEYS_THREADS * KEYS_ITEMS) + threadIdx.x;
+    uint32_t sent_rows = 0;                         // bit j: row j reduces to the sentinel
     for (int j0 = 0; j0 < KEYS_ITEMS; j0 += KEYS_BATCH) {
         double qv[KEYS_BATCH][D];
 #pragma unroll
@@ -399,13 +429,27 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
 #pragma unroll
         for (int u = 0; u < KEYS_BATCH; ++u) {
             const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
+            float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+            bool has = false, sr = false;
             if (i < n) {
-                const uint32_t k = key_of_row<D, T, TC, TO>(g, B, sent, has_comps, has_norm, o,
-                                                            i, qv[u], cellbits, qrec, qstride);
+                const uint32_t k = key_of_row<D>(g, B, o, i, qv[u], cellbits, &r, &has, &sr);
                 key[i] = k;
                 atomicAdd(&cnt[k & 255u], 1);
             }
+            sent_rows |= (sr ? 1u : 0u) << (j0 + u);
+            if (qrec) store_recs(qrec, qstride, i - (threadIdx.x & 63), n, r, has);
         }
+    }
+    // rows outside the grid (rare): evaluated from the sentinel row, one at
+    // a time, their positions loaded again
+#pragma unroll 1
+    for (int u = 0; u < KEYS_ITEMS; ++u) {
+        if (!((sent_rows >> u) & 1u)) continue;
+        const int64_t i = base + (int64_t)u * KEYS_THREADS;
+        double loc[D];
+        bool oob;
+        arb_reduce<D>(q + i * ldq, g, loc, &oob);
+        eval_point<D, T, TC, TO>(sent, has_comps, has_norm, loc, g, o, i);
     }
     __syncthreads();
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];

@@ -26,6 +26,20 @@ constexpr int RS_ITEMS = ARB_RS_ITEMS;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;   // 8192
 constexpr int RS_WAVES = RS_THREADS / 64;
 
+// Tile of a scatter pass for workgroup b of ntiles.  Workgroups are handed
+// to the eight XCDs in turn (b -> XCD b % 8), each XCD with its own L2.  A
+// digit's run from tile t ends where the same digit's run from tile t + 1
+// begins, so the line they share is written by both: giving each XCD a
+// contiguous range of tiles, taken in order, has both halves of such a line
+// written through the same L2, close together in time, instead of two
+// partial lines from two L2s.
+constexpr int RS_XCDS = 8;
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
+    const int64_t x = b % RS_XCDS, k = b / RS_XCDS;
+    const int64_t q = ntiles / RS_XCDS, r = ntiles % RS_XCDS;
+    return x * q + (x < r ? x : r) + k;        // XCD x holds tiles [x q + min(x, r), +q + (x < r))
+}
+
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* __restrict__ hist,
              int64_t ntiles) {
@@ -119,9 +133,10 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
     __shared__ int32_t sval[RS_TILE];
     const bool dt = threadIdx.x < 256;
     const int dg0 = dt ? threadIdx.x : 0;
-    const int32_t tc = dt ? hist[(int64_t)dg0 * ntiles + blockIdx.x] : 0;
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    const int32_t tc = dt ? hist[(int64_t)dg0 * ntiles + tile] : 0;
     const int32_t gt = dt ? tot[dg0] : 0;
-    const int32_t hs = dt ? hscan[(int64_t)dg0 * ntiles + blockIdx.x] : 0;
+    const int32_t hs = dt ? hscan[(int64_t)dg0 * ntiles + tile] : 0;
     int32_t ttot, gtot;
     const int32_t tex = digits_exscan(tc, wsum[0], &ttot);
     const int32_t gex = digits_exscan(gt, wsum[1], &gtot);
@@ -131,7 +146,7 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
         gbase[dg0] = gex + hs;
     }
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t base = tile * RS_TILE;
 #pragma unroll
     for (int j = 0; j < RS_ITEMS; ++j) {
         const int64_t e = base + (int64_t)j * RS_THREADS + threadIdx.x;
@@ -173,7 +188,8 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
     const int32_t dtot = tot[dg0];
     if (dt) gbase[dg0] = dtot;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    const int64_t base = tile * RS_TILE + (int64_t)w * (64 * RS_ITEMS);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t key[RS_ITEMS];
     int32_t val[RS_ITEMS];
@@ -227,7 +243,7 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
     }
     if (dt) {
         toff[dg0] -= acc;
-        gbase[dg0] += hscan[(int64_t)dg0 * ntiles + blockIdx.x] - dtot;
+        gbase[dg0] += hscan[(int64_t)dg0 * ntiles + tile] - dtot;
     }
     __syncthreads();
     // local reorder: digit-major, element order inside each digit
@@ -246,7 +262,7 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
     // that starts a run of equal (key >> fshift) inside its digit's run
     // offers its position to first[key >> fshift] (the sorted order makes
     // the minimum over tiles the group's first position).
-    const int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t tbase = tile * RS_TILE;
     const int cnt = (int)((n - tbase) < RS_TILE ? (n - tbase) : RS_TILE);
     for (int i = threadIdx.x; i < cnt; i += RS_THREADS) {
         const uint32_t k = skey[i];
