@@ -1009,18 +1009,32 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                                  : (const void*)((const char*)f->table +
                                                  esz * (size_t)f->g.nc * f->ninterp *
                                                      ((size_t)1 << (2 * f->g.ndim)));
+            // rows grouped by bucket in any order inside it (the evaluation
+            // of a row does not depend on it): the grouping sort, whose
+            // second digit needs no stable ranking; with the cell in the key
+            // (rows ordered by cell inside a bucket) or keys past its range,
+            // the LSD radix sort
+            const bool grp = cellbits == 0 && arb_group_sort_ok(n, bits);
             HIPCHK(arb_launch_keys(f->g, BB, f->precision, out64, sentrow, f->has_comps,
                                    f->has_norm, q_dev, n, ldq, o, w.k[0], cellbits, w.hist,
-                                   nodeform ? w.qrec : nullptr, qstride, s));
+                                   grp ? bits - 8 : 0, nodeform ? w.qrec : nullptr, qstride, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[1], s));
-            int in_first = 1;
-            // the sort also leaves each bucket's first sorted position in
-            // w.offs (w.first as scratch)
-            HIPCHK(arb_launch_radix_sort(w.k[0], w.v[0], w.k[1], w.v[1], n, bits, w.hist,
-                                         w.hscan, 1, 1, w.offs, BB.nbt, cellbits, w.first,
-                                         &in_first, s));
+            if (grp) {
+                // each bucket's first position goes to w.offs (w.first: the
+                // per-bucket counters)
+                HIPCHK(arb_launch_group_sort(w.k[0], w.k[1], w.v[0], n, bits, w.hist, w.hscan,
+                                             w.offs, BB.nbt, w.first, s));
+                perm = w.v[0];
+            } else {
+                int in_first = 1;
+                // the sort also leaves each bucket's first sorted position in
+                // w.offs (w.first as scratch)
+                HIPCHK(arb_launch_radix_sort(w.k[0], w.v[0], w.k[1], w.v[1], n, bits, w.hist,
+                                             w.hscan, 1, 1, w.offs, BB.nbt, cellbits, w.first,
+                                             &in_first, s));
+                perm = w.v[in_first ? 0 : 1];
+            }
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
-            perm = w.v[in_first ? 0 : 1];
         }
         const int32_t* offs = w.offs;
         if (timing) HIPCHK(hipEventRecord(f->ev[3], s));

@@ -273,7 +273,8 @@ hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, i
 constexpr int ARB_RS_THREADS = 512;
 constexpr int ARB_RS_ITEMS = 16;
 // Keys (bucket, or bucket and cell) of n queries, and the tile histograms of
-// the keys' low 8 bits (hist[digit * ntiles + tile]).  The value of key i is
+// the keys' 8-bit digit (key >> hshift) & 255 (hist[digit * ntiles + tile]:
+// hshift 0 for the radix sort, bits - 8 for the grouping sort).  The value of key i is
 // the row index i, implicit (the radix sort's index_vals).  Rows outside the
 // box are evaluated here from the sentinel row `sent` (ninterp x K values).
 // With qrec (3D node-tile path) every row i inside the box also gets its
@@ -282,8 +283,8 @@ constexpr int ARB_RS_ITEMS = 16;
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* sent, int32_t has_comps, int32_t has_norm,
                            const double* q, int64_t n, int64_t ldq, const ArbOut& o,
-                           uint32_t* key, int32_t cellbits, int32_t* hist, void* qrec,
-                           int32_t qstride, hipStream_t s);
+                           uint32_t* key, int32_t cellbits, int32_t* hist, int32_t hshift,
+                           void* qrec, int32_t qstride, hipStream_t s);
 int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
@@ -293,6 +294,13 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
                                  int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
                                  int32_t* first, int* in_first, hipStream_t s);
+// Grouping of the rows by key for the bucketed evaluation (k_sort.hip): keys
+// below 2^bits with 9 <= bits <= 16 and n < 2^24 (arb_group_sort_ok); no
+// order inside a key.  kcnt: nbt + 1 ints of scratch.
+int arb_group_sort_ok(int64_t n, int bits);
+hipError_t arb_launch_group_sort(const uint32_t* keys, uint32_t* ent, int32_t* perm, int64_t n,
+                                 int bits, const int32_t* hist, int32_t* hscan, int32_t* offs,
+                                 int64_t nbt, int32_t* kcnt, hipStream_t s);
 hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int32_t* offs,
                               int32_t* perm, int64_t n, hipStream_t s);
 size_t arb_fused_lds(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t threads);

@@ -332,8 +332,9 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
 // or nbt for rows evaluated here from the sentinel row (the sort's value is
 // the row index, implicit).
 // One workgroup of KEYS_THREADS threads per radix tile (ARB_RS_THREADS x
-// ARB_RS_ITEMS rows): it also counts the low 8 bits of its keys in LDS and
-// writes the tile's histogram, which is the first radix pass's upsweep.
+// ARB_RS_ITEMS rows): it also counts the 8-bit digit (key >> hshift) & 255
+// of its keys in LDS and writes the tile's histogram, which is the first
+// sort pass's upsweep.
 // Positions are loaded KEYS_BATCH rows at a time so their loads are in flight
 // together.
 constexpr int KEYS_THREADS = 256;
@@ -410,7 +411,7 @@ template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(KEYS_THREADS)
 k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
        int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-       uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist,
+       uint32_t* __restrict__ key, int32_t cellbits, int32_t* __restrict__ hist, int32_t hshift,
        int64_t ntiles, float4* __restrict__ qrec, int32_t qstride) {
     __shared__ int32_t cnt[256];
     cnt[threadIdx.x] = 0;
@@ -434,7 +435,7 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
             if (i < n) {
                 const uint32_t k = key_of_row<D>(g, B, o, i, qv[u], cellbits, &r, &has, &sr);
                 key[i] = k;
-                atomicAdd(&cnt[k & 255u], 1);
+                atomicAdd(&cnt[(k >> hshift) & 255u], 1);
             }
             sent_rows |= (sr ? 1u : 0u) << (j0 + u);
             if (qrec) store_recs(qrec, qstride, i - (threadIdx.x & 63), n, r, has);
@@ -2150,14 +2151,14 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 hipError_t arb_launch_keys(const ArbGeom& g, const ArbBuckets& B, int32_t prec, int32_t out_f64,
                            const void* sent, int32_t has_comps, int32_t has_norm,
                            const double* q, int64_t n, int64_t ldq, const ArbOut& o,
-                           uint32_t* key, int32_t cellbits, int32_t* hist, void* qrec,
-                           int32_t qstride, hipStream_t s) {
+                           uint32_t* key, int32_t cellbits, int32_t* hist, int32_t hshift,
+                           void* qrec, int32_t qstride, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = arb_rs_ntiles(n);
 #define L_KEYS(DD, T, TC, TO)                                                                 \
     hipLaunchKernelGGL((k_keys<DD, T, TC, TO>), dim3((unsigned)ntiles), dim3(KEYS_THREADS), 0,   \
                        s, g, B, (const TC*)sent, has_comps, has_norm, q, n, ldq, o, key,         \
-                       cellbits, hist, ntiles, (float4*)qrec, qstride)
+                       cellbits, hist, hshift, ntiles, (float4*)qrec, qstride)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_KEYS);
     else ARB_DISPATCH(4, prec, out_f64, L_KEYS);
 #undef L_KEYS

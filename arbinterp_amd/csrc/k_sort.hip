@@ -60,10 +60,14 @@ k_rs_upsweep(const uint32_t* __restrict__ keys, int64_t n, int shift, int32_t* _
 // One workgroup per digit: hscan[d * ntiles + t] = sum of hist[d * ntiles + u]
 // over u < t, tot[d] = the sum over all tiles.  Chunks of RS_TILE counts,
 // RS_ITEMS consecutive ones per thread.
+// zero[0 .. zlen): cleared on the side (the grouping sort's per-key counts).
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_digit_scan(const int32_t* __restrict__ hist, int64_t ntiles, int32_t* __restrict__ hscan,
-                int32_t* __restrict__ tot) {
+                int32_t* __restrict__ tot, int32_t* __restrict__ zero, int64_t zlen) {
     __shared__ int32_t part[RS_THREADS];
+    for (int64_t i = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x; i < zlen;
+         i += (int64_t)gridDim.x * RS_THREADS)
+        zero[i] = 0;
     const int64_t base = (int64_t)blockIdx.x * ntiles;
     int32_t carry = 0;
     for (int64_t c0 = 0; c0 < ntiles; c0 += RS_TILE) {
@@ -115,16 +119,20 @@ __device__ __forceinline__ int32_t digits_exscan(int32_t v, int32_t* wsum, int32
     return before + x - (threadIdx.x < 256 ? v : 0);
 }
 
-// First pass of the LSD sort (the lowest digit; the values are the element
-// indices).  An LSD sort needs only its later passes to be stable, so each
-// pair's place inside its digit comes from an LDS atomic instead of the
+// First pass of the LSD sort (the lowest digit, shift 0; the values are the
+// element indices).  An LSD sort needs only its later passes to be stable, so
+// each pair's place inside its digit comes from an LDS atomic instead of the
 // stable wave-ballot ranking of k_rs_downsweep.  The tile's digit counts are
 // the first pass's histogram, made with the keys.
+// PACK (the grouping sort's first pass, digit = the high 8 bits, shift =
+// bits - 8): the output is one word per pair, the key's low `shift` bits
+// above the 24-bit element index, in kout; vout is not used.
+template <bool PACK>
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
            int32_t* __restrict__ vout, int64_t n, const int32_t* __restrict__ hist,
            const int32_t* __restrict__ hscan, const int32_t* __restrict__ tot,
-           int64_t ntiles) {
+           int64_t ntiles, int shift) {
     __shared__ int32_t cnt[256];                // next free place per digit in the tile
     __shared__ int32_t gbase[256];              // global start of the tile's part of each run
     __shared__ int32_t toff[256];               // tile-local start of each digit's run
@@ -152,7 +160,7 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
         const int64_t e = base + (int64_t)j * RS_THREADS + threadIdx.x;
         if (e < n) {
             const uint32_t k = __builtin_nontemporal_load(&kin[e]);
-            const int lp = atomicAdd(&cnt[k & 255u], 1);
+            const int lp = atomicAdd(&cnt[(k >> shift) & 255u], 1);
             // lp < RS_TILE: the tile holds tc[d] pairs of digit d, its run
             // starts at toff[d] and the runs tile [0, RS_TILE)
             skey[lp] = k;
@@ -161,13 +169,18 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
     }
     __syncthreads();
     const int cntt = (int)((n - base) < RS_TILE ? (n - base) : RS_TILE);
+    const uint32_t lmask = (1u << shift) - 1u;
     for (int i = threadIdx.x; i < cntt; i += RS_THREADS) {
         const uint32_t k = skey[i];
-        const uint32_t dg = k & 255u;
+        const uint32_t dg = (k >> shift) & 255u;
         const int64_t pos = (int64_t)gbase[dg] + (i - toff[dg]);
         if (pos < 0 || pos >= n) continue;    // unreachable: a digit's run stays inside [0, n)
-        vout[pos] = sval[i];
-        kout[pos] = k;
+        if constexpr (PACK) {
+            kout[pos] = ((k & lmask) << 24) | (uint32_t)sval[i];
+        } else {
+            vout[pos] = sval[i];
+            kout[pos] = k;
+        }
     }
 }
 
@@ -276,6 +289,149 @@ k_rs_downsweep(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin
         } else {
             kout[pos] = k;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grouping sort: keys below 2^bits (9 <= bits <= 16), n < 2^24 rows.  The
+// evaluation needs the rows grouped by key, in any order inside a group, so
+// the second digit need not keep the first pass's order: the first pass
+// (k_rs_pass1<true>) groups the rows by the key's high 8 bits into 256
+// segments, and each segment is then grouped by the remaining low bits on its
+// own, with no stable ranking:
+//   k_grp_count  each tile of a segment counts its low digits in LDS and adds
+//                them to the segment's per-key counts (global atomics, at most
+//                256 per tile);
+//   k_grp_starts per segment, the exclusive scan of its keys' counts from the
+//                segment start: each key's first position, written to offs
+//                (the bucket bounds) and to the counters;
+//   k_grp_place  each tile ranks its rows per low digit in LDS, takes the
+//                places of each digit's rows with one atomic on the key's
+//                counter, and writes the row indices there in digit order.
+// A segment of c rows has ceil(c / RS_TILE) tiles; the grids hold one
+// workgroup per tile of any segment (at most ntiles + 256), and each
+// workgroup finds its segment and tile from the 256 segment sizes.
+// ---------------------------------------------------------------------------
+struct SegTile {
+    int64_t lo, hi;     // the tile's entries [lo, hi)
+    int d;              // its segment (high digit)
+};
+
+// Block-wide; false (for every thread) when blockIdx.x is past the last tile.
+__device__ __forceinline__ bool seg_tile(const int32_t* __restrict__ tot, int32_t* sS, int32_t* sT,
+                                         int32_t (*wsum)[4], int* sd, SegTile* st) {
+    const bool dt = threadIdx.x < 256;
+    const int32_t c = dt ? tot[threadIdx.x] : 0;
+    const int32_t nt = (c + RS_TILE - 1) / RS_TILE;
+    int32_t ctot, ttot;
+    const int32_t cex = digits_exscan(c, wsum[0], &ctot);
+    const int32_t tex = digits_exscan(nt, wsum[1], &ttot);
+    if (dt) {
+        sS[threadIdx.x] = cex;
+        sT[threadIdx.x] = tex;
+    }
+    if (threadIdx.x == 0) {
+        sS[256] = ctot;
+        sT[256] = ttot;
+    }
+    __syncthreads();
+    const int32_t b = (int32_t)blockIdx.x;
+    if (b >= sT[256]) return false;
+    // exactly one segment holds tile b: sT[d] <= b < sT[d + 1] (empty
+    // segments have sT[d] == sT[d + 1])
+    if (dt && sT[threadIdx.x] <= b && b < sT[threadIdx.x + 1]) *sd = threadIdx.x;
+    __syncthreads();
+    const int d = *sd;
+    st->d = d;
+    st->lo = (int64_t)sS[d] + (int64_t)(b - sT[d]) * RS_TILE;
+    const int64_t end = sS[d + 1];
+    st->hi = st->lo + RS_TILE < end ? st->lo + RS_TILE : end;
+    return true;
+}
+
+__global__ void __launch_bounds__(RS_THREADS)
+k_grp_count(const uint32_t* __restrict__ ent, const int32_t* __restrict__ tot, int hs,
+            int32_t* __restrict__ kcnt, int64_t nbt) {
+    __shared__ int32_t sS[257], sT[257], wsum[2][4], cnt[256];
+    __shared__ int sd;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+    SegTile st;
+    if (!seg_tile(tot, sS, sT, wsum, &sd, &st)) return;
+    for (int64_t e = st.lo + threadIdx.x; e < st.hi; e += RS_THREADS)
+        atomicAdd(&cnt[ent[e] >> 24], 1);
+    __syncthreads();
+    if (threadIdx.x < (1 << hs)) {
+        const int64_t key = ((int64_t)st.d << hs) | threadIdx.x;
+        const int32_t c = cnt[threadIdx.x];
+        if (c > 0 && key <= nbt) atomicAdd(&kcnt[key], c);
+    }
+}
+
+// One workgroup of 256 threads per segment.
+__global__ void __launch_bounds__(256)
+k_grp_starts(const int32_t* __restrict__ tot, int hs, int32_t* __restrict__ kcnt,
+             int32_t* __restrict__ offs, int64_t nbt) {
+    __shared__ int32_t wsum[2][4];
+    __shared__ int32_t sbase;
+    int32_t total;
+    const int32_t sex = digits_exscan(tot[threadIdx.x], wsum[0], &total);
+    if (threadIdx.x == blockIdx.x) sbase = sex;  // the segment's start
+    const int64_t key = ((int64_t)blockIdx.x << hs) + threadIdx.x;
+    const bool in = threadIdx.x < (1 << hs) && key <= nbt;
+    const int32_t c = in ? kcnt[key] : 0;
+    int32_t ctot;
+    const int32_t lex = digits_exscan(c, wsum[1], &ctot);   // its barrier publishes sbase
+    if (in) {
+        const int32_t first = sbase + lex;
+        kcnt[key] = first;
+        offs[key] = first;
+    }
+}
+
+__global__ void __launch_bounds__(RS_THREADS)
+k_grp_place(const uint32_t* __restrict__ ent, const int32_t* __restrict__ tot, int hs,
+            int32_t* __restrict__ kcur, int32_t* __restrict__ perm, int64_t n, int64_t nbt) {
+    __shared__ int32_t sS[257], sT[257], wsum[3][4], cnt[256], toff[256], gbase[256];
+    __shared__ uint32_t sorted[RS_TILE];        // the tile's entries in low-digit order
+    __shared__ int sd;
+    if (threadIdx.x < 256) cnt[threadIdx.x] = 0;
+    SegTile st;
+    if (!seg_tile(tot, sS, sT, wsum, &sd, &st)) return;
+    uint32_t v[RS_ITEMS];
+    int32_t r[RS_ITEMS];
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = st.lo + (int64_t)j * RS_THREADS + threadIdx.x;
+        v[j] = 0;
+        r[j] = 0;
+        if (e < st.hi) {
+            v[j] = __builtin_nontemporal_load(&ent[e]);
+            r[j] = atomicAdd(&cnt[v[j] >> 24], 1);
+        }
+    }
+    __syncthreads();
+    const int32_t c = threadIdx.x < 256 ? cnt[threadIdx.x] : 0;
+    int32_t ttot;
+    const int32_t ex = digits_exscan(c, wsum[2], &ttot);
+    if (threadIdx.x < 256) {
+        toff[threadIdx.x] = ex;
+        const int64_t key = ((int64_t)st.d << hs) | threadIdx.x;
+        gbase[threadIdx.x] = (c > 0 && key <= nbt) ? atomicAdd(&kcur[key], c) : 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < RS_ITEMS; ++j) {
+        const int64_t e = st.lo + (int64_t)j * RS_THREADS + threadIdx.x;
+        if (e < st.hi) sorted[toff[v[j] >> 24] + r[j]] = v[j];
+    }
+    __syncthreads();
+    const int cntt = (int)(st.hi - st.lo);
+    for (int i = threadIdx.x; i < cntt; i += RS_THREADS) {
+        const uint32_t sv = sorted[i];
+        const uint32_t l = sv >> 24;
+        const int64_t pos = (int64_t)gbase[l] + (i - toff[l]);
+        if (pos < 0 || pos >= n) continue;    // unreachable: a key's places stay inside [0, n)
+        perm[pos] = (int32_t)(sv & 0xFFFFFFu);
     }
 }
 
@@ -456,11 +612,11 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
             hipLaunchKernelGGL(k_rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, n,
                                shift, hist, ntiles);
         hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
-                           tot);
+                           tot, (int32_t*)nullptr, (int64_t)0);
         const int32_t* vin = (shift == 0 && index_vals) ? nullptr : va;
         if (shift == 0 && index_vals && !last)
-            hipLaunchKernelGGL(k_rs_pass1, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka, kb,
-                               vb, n, hist, hscan, tot, ntiles);
+            hipLaunchKernelGGL(k_rs_pass1<false>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s,
+                               ka, kb, vb, n, hist, hscan, tot, ntiles, 0);
         else
             hipLaunchKernelGGL(k_rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, ka,
                                vin, kb, vb, n, shift, hscan, tot, ntiles, last ? first : nullptr,
@@ -477,3 +633,31 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
     return hipGetLastError();
 }
 
+int arb_group_sort_ok(int64_t n, int bits) {
+    return n > 0 && n < ((int64_t)1 << 24) && bits >= 9 && bits <= 16;
+}
+
+// Group the rows 0..n-1 by their keys (keys < 2^bits, key <= nbt; see
+// arb_group_sort_ok): perm lists the rows of key 0, then of key 1, ..., in
+// no set order inside a key, and offs[b] (b in [0, nbt]) is the first
+// position of key b.  hist: the tile histograms of the keys' high 8 bits
+// (key >> (bits - 8)), written with the keys; hscan as for the radix sort;
+// ent: n words; kcnt: nbt + 1 ints.
+hipError_t arb_launch_group_sort(const uint32_t* keys, uint32_t* ent, int32_t* perm, int64_t n,
+                                 int bits, const int32_t* hist, int32_t* hscan, int32_t* offs,
+                                 int64_t nbt, int32_t* kcnt, hipStream_t s) {
+    if (!arb_group_sort_ok(n, bits) || nbt >= ((int64_t)1 << bits)) return hipErrorInvalidValue;
+    const int64_t ntiles = arb_rs_ntiles(n);
+    int32_t* tot = hscan + 256 * ntiles;
+    const int hs = bits - 8;
+    hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
+                       tot, kcnt, nbt + 1);
+    hipLaunchKernelGGL(k_rs_pass1<true>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, keys,
+                       ent, (int32_t*)nullptr, n, hist, hscan, tot, ntiles, hs);
+    const unsigned gt = (unsigned)(ntiles + 256);    // tiles of all segments: at most this
+    hipLaunchKernelGGL(k_grp_count, dim3(gt), dim3(RS_THREADS), 0, s, ent, tot, hs, kcnt, nbt);
+    hipLaunchKernelGGL(k_grp_starts, dim3(256), dim3(256), 0, s, tot, hs, kcnt, offs, nbt);
+    hipLaunchKernelGGL(k_grp_place, dim3(gt), dim3(RS_THREADS), 0, s, ent, tot, hs, kcnt, perm, n,
+                       nbt);
+    return hipGetLastError();
+}
