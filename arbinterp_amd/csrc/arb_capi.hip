@@ -979,6 +979,10 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             }
         }
         const int32_t* perm = f->ws_perm;
+        // the evaluation's hot flag (a bucket holds more rows than one
+        // workgroup takes): the last int of w.first, set by the grouping
+        // sort, and set to 1 (look) by the other groupings
+        int32_t* hot = w.first + arb_rs_first_elems(BB.nbt) - 1;
         if (timing) HIPCHK(hipEventRecord(f->ev[0], s));
         if (flags & ARB_Q_GROUP_ATOMIC) {
             HIPCHK(arb_launch_fill_i32(f->ws_count, f->B.nbt, 0, s));
@@ -989,6 +993,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
             HIPCHK(arb_launch_scan(f->ws_count, f->ws_offs, f->B.nbt, s));
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
             HIPCHK(arb_launch_scatter(f->ws_bkt, f->ws_rank, f->ws_offs, f->ws_perm, n, s));
+            HIPCHK(arb_launch_fill_i32(hot, 1, 1, s));
         } else {
             // key = bucket (then the cell inside it with ARB_Q_SORT_CELL)
             int cellbits = 0;
@@ -1032,6 +1037,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                 HIPCHK(arb_launch_radix_sort(w.k[0], w.v[0], w.k[1], w.v[1], n, bits, w.hist,
                                              w.hscan, 1, 1, w.offs, BB.nbt, cellbits, w.first,
                                              &in_first, s));
+                HIPCHK(arb_launch_fill_i32(hot, 1, 1, s));
                 perm = w.v[in_first ? 0 : 1];
             }
             if (timing) HIPCHK(hipEventRecord(f->ev[2], s));
@@ -1044,19 +1050,19 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                                         perm, s));
         else if (nodeform)
             HIPCHK(arb_launch_eval_nodes(f->g, BB, f->ninterp, out64, f->tiles, w.qrec, qstride,
-                                         n, o, offs, perm, f->eval_threads, s));
+                                         n, o, offs, perm, hot, f->eval_threads, s));
         else if (fused)
             HIPCHK(arb_launch_eval_fused(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                          f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
-                                         perm, f->eval_threads, s));
+                                         perm, hot, f->eval_threads, s));
         else if (fused4)
             HIPCHK(arb_launch_eval_fused4(f->g, BB, f->precision, out64, f->vals, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o, offs,
-                                          perm, f->eval_threads, s));
+                                          perm, hot, f->eval_threads, s));
         else
             HIPCHK(arb_launch_eval_bucket(f->g, f->B, f->precision, out64, f->table, f->ninterp,
                                           f->has_comps, f->has_norm, q_dev, n, ldq, o,
-                                          offs, perm, f->eval_threads, s));
+                                          offs, perm, hot, f->eval_threads, s));
         if (timing) HIPCHK(hipEventRecord(f->ev[4], s));
         if (split)
             HIPCHK(arb_launch_unpack(f->g.ndim, f->precision == ARB_PREC_FP64 || out64,

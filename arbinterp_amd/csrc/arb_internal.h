@@ -268,6 +268,13 @@ hipError_t arb_launch_scan(const int32_t* count, int32_t* offs, int64_t nb, hipS
 hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, int32_t has_norm,
                              const void* rec, void* comps, void* norms, void* grads, int64_t n,
                              hipStream_t s);
+// Rows of one bucket a workgroup of the bucketed evaluation takes at most.
+// A bucket holding more (queries clustered in a few cells) is shared by the
+// workgroups nbt + o that cover its sorted positions past the first chunk,
+// o = position / chunk; the launchers size their grids nbt + ceil(n /
+// chunk).  hot[0] (set with the bucket bounds) is nonzero when some bucket
+// holds more than a chunk; when it is zero those workgroups end at once.
+constexpr int32_t ARB_EVAL_CHUNK = 2048;
 // Radix-sort tiles: ARB_RS_THREADS threads x ARB_RS_ITEMS pairs (8192; the
 // longer digit runs of the larger tile make the scattered writes wider).
 constexpr int ARB_RS_THREADS = 512;
@@ -289,14 +296,18 @@ int64_t arb_scan_tmp_elems(int64_t len);
 hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, int32_t* tmp,
                                 int32_t* tmp_scan, hipStream_t s);
 int64_t arb_rs_ntiles(int64_t n);
-int64_t arb_rs_first_elems(int64_t ngroups);   // scratch ints of arb_launch_radix_sort
+// scratch ints of arb_launch_radix_sort and arb_launch_group_sort; the last
+// one is the hot flag of the bucketed evaluation (ARB_EVAL_CHUNK)
+int64_t arb_rs_first_elems(int64_t ngroups);
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
                                  int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
                                  int32_t* first, int* in_first, hipStream_t s);
 // Grouping of the rows by key for the bucketed evaluation (k_sort.hip): keys
 // below 2^bits with 9 <= bits <= 16 and n < 2^24 (arb_group_sort_ok); no
-// order inside a key.  kcnt: nbt + 1 ints of scratch.
+// order inside a key.  kcnt: arb_rs_first_elems(nbt) ints of scratch, whose
+// last one becomes the hot flag (a key below nbt holds more than
+// ARB_EVAL_CHUNK rows).
 int arb_group_sort_ok(int64_t n, int bits);
 hipError_t arb_launch_group_sort(const uint32_t* keys, uint32_t* ent, int32_t* perm, int64_t n,
                                  int bits, const int32_t* hist, int32_t* hscan, int32_t* offs,
@@ -308,7 +319,8 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t out_f64, const double* vals, int32_t ninterp,
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                 const int32_t* perm, int32_t block_threads, hipStream_t s);
+                                 const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                 hipStream_t s);
 // 4D bucketed query without the table (coefficients formed per bucket with
 // the 4D table build's arithmetic); arb_fused4_lds is 0 where it does not
 // apply.
@@ -317,7 +329,8 @@ hipError_t arb_launch_eval_fused4(const ArbGeom& g, const ArbBuckets& B, int32_t
                                   int32_t out_f64, const double* vals, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
+                                  const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                  hipStream_t s);
 bool arb_nodes_ok(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp, int32_t has_comps,
                   int32_t has_norm);
 int64_t arb_tiles_bytes(const ArbBuckets& B, int32_t ninterp, int64_t* head_offset);
@@ -341,12 +354,13 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t out_f64, const void* tiles, const void* qrec,
                                  int32_t qstride, int64_t n,
                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
-                                 int32_t block_threads, hipStream_t s);
+                                 const int32_t* hot, int32_t block_threads, hipStream_t s);
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q,
                                   int64_t n, int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s);
+                                  const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                  hipStream_t s);
 hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                 int32_t out_f64, const void* table, int32_t ninterp,
                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,

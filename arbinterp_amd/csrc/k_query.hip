@@ -90,6 +90,21 @@ __device__ __forceinline__ void eval_point(const TC* a, int32_t has_comps, int32
     store_rec<D, TO>(rec, has_comps, has_norm, o, i);
 }
 
+// A row whose local coordinates hold a NaN (outside the box, or a NaN
+// position) evaluates to NaN in every output whatever the coefficients:
+// Horner's rule in a NaN coordinate gives NaN for the value and every
+// derivative.  Written here without evaluating; the record's padding stays
+// zero, as eval_point leaves it.
+template <int D, typename TO>
+__device__ __forceinline__ void store_nan_row(int32_t has_comps, int32_t has_norm,
+                                              const ArbOut& o, int64_t i) {
+    const int used = (has_comps ? 3 : 0) + (has_norm ? 1 + D : 0);
+    TO rec[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) rec[r] = r < used ? (TO)NAN : (TO)0;
+    store_rec<D, TO>(rec, has_comps, has_norm, o, i);
+}
+
 // Cell reduction of row i with the side outputs every path writes: queryInds,
 // the past-the-sentinel flag and the NaN row fill.  Returns the cell used for
 // the coefficients (clamped to the sentinel nc).
@@ -348,18 +363,20 @@ static_assert(KEYS_ITEMS % KEYS_BATCH == 0 && KEYS_ITEMS <= 32,
 // The row's record for the node-tile evaluation (3D): its local coordinates
 // (FP32 and corrections) and its local cell in the bucket.  *has = false for
 // a row that reduces to the sentinel (no record); *sent_row = true for such a
-// row, which the caller evaluates from the sentinel row afterwards.
+// row, which the caller evaluates from the sentinel row afterwards, and
+// *nan_loc = true when its local coordinates hold a NaN (its outputs are NaN).
 template <int D>
 __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBuckets& B,
                                                const ArbOut& o, int64_t i, const double* qp,
                                                int32_t cellbits, float4* rec, bool* has,
-                                               bool* sent_row) {
+                                               bool* sent_row, bool* nan_loc) {
     double loc[D];
     int64_t ia[D];
     bool oob;
     const int64_t flat = arb_reduce_axes<D>(qp, g, loc, ia, &oob);
     *has = false;
     *sent_row = false;
+    *nan_loc = false;
     if (o.cells) o.cells[i] = flat;
     if (oob && o.qmut) {
         for (int64_t c = 0; c < o.ldq; ++c) o.qmut[i * o.ldq + c] = NAN;
@@ -367,6 +384,10 @@ __device__ __forceinline__ uint32_t key_of_row(const ArbGeom& g, const ArbBucket
     if (flat >= g.nc) {
         if (flat > g.nc) o.flag[0] = 1;
         *sent_row = true;
+        bool nl = false;
+#pragma unroll
+        for (int a = 0; a < D; ++a) nl |= isnan(loc[a]);
+        *nan_loc = nl;
         return (uint32_t)B.nbt << cellbits;
     }
     int lc;
@@ -417,7 +438,7 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
     cnt[threadIdx.x] = 0;
     __syncthreads();
     const int64_t base = (int64_t)blockIdx.x * (KEYS_THREADS * KEYS_ITEMS) + threadIdx.x;
-    uint32_t sent_rows = 0;                         // bit j: row j reduces to the sentinel
+    uint32_t sent_rows = 0;     // bit j: row j reduces to the sentinel, finite local coordinates
     for (int j0 = 0; j0 < KEYS_ITEMS; j0 += KEYS_BATCH) {
         double qv[KEYS_BATCH][D];
 #pragma unroll
@@ -431,18 +452,22 @@ k_keys(ArbGeom g, ArbBuckets B, const TC* __restrict__ sent, int32_t has_comps,
         for (int u = 0; u < KEYS_BATCH; ++u) {
             const int64_t i = base + (int64_t)(j0 + u) * KEYS_THREADS;
             float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
-            bool has = false, sr = false;
+            bool has = false, sr = false, nl = false;
             if (i < n) {
-                const uint32_t k = key_of_row<D>(g, B, o, i, qv[u], cellbits, &r, &has, &sr);
+                const uint32_t k =
+                    key_of_row<D>(g, B, o, i, qv[u], cellbits, &r, &has, &sr, &nl);
                 key[i] = k;
                 atomicAdd(&cnt[(k >> hshift) & 255u], 1);
+                if (nl) store_nan_row<D, TO>(has_comps, has_norm, o, i);
             }
-            sent_rows |= (sr ? 1u : 0u) << (j0 + u);
+            sent_rows |= (sr && !nl ? 1u : 0u) << (j0 + u);
             if (qrec) store_recs(qrec, qstride, i - (threadIdx.x & 63), n, r, has);
         }
     }
-    // rows outside the grid (rare): evaluated from the sentinel row, one at
-    // a time, their positions loaded again
+    // rows that reduce to the sentinel with finite local coordinates (rare:
+    // the rows outside the box or NaN got their NaN outputs above): evaluated
+    // from the sentinel row one at a time, their positions loaded again
+    // (unchanged: only rows outside the box are NaN-filled)
 #pragma unroll 1
     for (int u = 0; u < KEYS_ITEMS; ++u) {
         if (!((sent_rows >> u) & 1u)) continue;
@@ -702,17 +727,56 @@ __device__ __forceinline__ void eval_row_lds(const ArbGeom& g, const ArbBuckets&
     eval_point<D, T, TC, TO>(a, has_comps, has_norm, loc, g, o, qi);
 }
 
+// Rows of the bucketed evaluation taken by this workgroup (uniform over it).
+// Workgroup w < nbt takes the first `chunk` rows of bucket w.  Workgroup
+// nbt + o takes, among the sorted positions [o chunk, (o + 1) chunk), the
+// rows past the first chunk of their bucket: those of one bucket at most,
+// the one holding position o chunk (a bucket that starts later inside the
+// range has its first chunk of rows there).  So every row of a bucket below
+// nbt is taken exactly once, and a bucket of many rows (queries clustered in
+// a few cells) is shared by as many workgroups.  hot[0] == 0 (no bucket
+// holds more than a chunk) ends the workgroups past nbt at once.
+__device__ __forceinline__ bool bucket_rows(const int32_t* __restrict__ offs, int64_t nbt,
+                                            const int32_t* __restrict__ hot, int32_t chunk,
+                                            int64_t* b, int32_t* start, int32_t* end) {
+    const int64_t w = blockIdx.x;
+    if (w < nbt) {
+        const int32_t s0 = offs[w], e0 = offs[w + 1];
+        *b = w;
+        *start = s0;
+        *end = e0 - s0 > chunk ? s0 + chunk : e0;
+        return s0 < e0;
+    }
+    if (hot[0] == 0) return false;
+    const int64_t p0 = (w - nbt) * (int64_t)chunk;
+    if (p0 >= offs[nbt]) return false;              // the rows outside the grid follow
+    int64_t lo = 0, hi = nbt;                       // offs[lo] <= p0 < offs[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (offs[mid] <= p0) lo = mid;
+        else hi = mid;
+    }
+    const int64_t s0 = (int64_t)offs[lo] + chunk, e0 = offs[lo + 1];
+    const int64_t s1 = p0 > s0 ? p0 : s0;
+    const int64_t e1 = p0 + chunk < e0 ? p0 + chunk : e0;
+    *b = lo;
+    *start = (int32_t)s1;
+    *end = (int32_t)e1;
+    return s1 < e1;
+}
+
 // One workgroup per bucket: stage the block's coefficients in LDS, then
 // evaluate the bucket's queries from LDS.
 template <int D, typename T, typename TC, typename TO>
 __global__ void __launch_bounds__(256)
 k_eval_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t has_comps,
               int32_t has_norm, const double* __restrict__ q, int64_t n, int64_t ldq, ArbOut o,
-              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
+              const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
+              const int32_t* __restrict__ hot) {
     extern __shared__ uint4 smem[];
-    const int64_t b = blockIdx.x;
-    const int32_t start = offs[b], end = offs[b + 1];
-    if (start == end) return;                       // uniform over the workgroup
+    int64_t b;
+    int32_t start, end;
+    if (!bucket_rows(offs, B.nbt, hot, ARB_EVAL_CHUNK, &b, &start, &end)) return;
     StageBase<D> sb;
     stage_base<D>(b, g, B, sb);
     uint32_t cs[D];
@@ -776,11 +840,11 @@ __global__ void __launch_bounds__(256)
 k_eval_fused3(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
               int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
               int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
-              const int32_t* __restrict__ perm) {
+              const int32_t* __restrict__ perm, const int32_t* __restrict__ hot) {
     extern __shared__ uint4 smem[];
-    const int64_t b = blockIdx.x;
-    const int32_t start = offs[b], end = offs[b + 1];
-    if (start == end) return;                       // uniform over the workgroup
+    int64_t b;
+    int32_t start, end;
+    if (!bucket_rows(offs, B.nbt, hot, ARB_EVAL_CHUNK, &b, &start, &end)) return;
     int64_t c0[3];
     {
         int64_t rem = b;
@@ -893,13 +957,13 @@ __global__ void __launch_bounds__(256)
 k_eval_fused4(ArbGeom g, ArbBuckets B, const double* __restrict__ vals, int32_t ninterp,
               int32_t has_comps, int32_t has_norm, const double* __restrict__ q, int64_t n,
               int64_t ldq, ArbOut o, const int32_t* __restrict__ offs,
-              const int32_t* __restrict__ perm) {
+              const int32_t* __restrict__ perm, const int32_t* __restrict__ hot) {
     using S = Fused4Shape<BX, BY, BZ, BT>;
     extern __shared__ uint4 smem[];
     __shared__ unsigned char cbad[S::NC];           // cells with a non-finite stencil
-    const int64_t b = blockIdx.x;
-    const int32_t start = offs[b], end = offs[b + 1];
-    if (start == end) return;                       // uniform over the workgroup
+    int64_t b;
+    int32_t start, end;
+    if (!bucket_rows(offs, B.nbt, hot, ARB_EVAL_CHUNK, &b, &start, &end)) return;
     int64_t c0[4];
     {
         int64_t rem = b;
@@ -1646,14 +1710,15 @@ template <typename TO, int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_eval_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
               const TileHead* __restrict__ heads, const float4* qrec, int32_t qstride, int64_t n,
-              ArbOut o, const int32_t* __restrict__ offs, const int32_t* __restrict__ perm) {
+              ArbOut o, const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
+              const int32_t* __restrict__ hot) {
     using S = TileShape<NI, BX, BY, BZ>;
     using RB = typename TileRec<NI>::bits;
     __shared__ typename TileRec<NI>::type tile[S::NZ * S::ZS];
     __shared__ TileHead h;
-    const int64_t b = blockIdx.x;
-    const int32_t start = offs[b], end = offs[b + 1];
-    if (start == end) return;                       // uniform over the workgroup
+    int64_t b;
+    int32_t start, end;
+    if (!bucket_rows(offs, B.nbt, hot, ARB_EVAL_CHUNK, &b, &start, &end)) return;
     int32_t s = start + threadIdx.x;
     int64_t qi = -1;
     float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1868,17 +1933,24 @@ hipError_t arb_launch_scatter(const int32_t* bkt, const int32_t* rank, const int
     return hipGetLastError();
 }
 
+// Workgroups of a bucketed evaluation: one per bucket, then one per chunk of
+// sorted positions for the rows of buckets past their first chunk.
+inline unsigned eval_grid(const ArbBuckets& B, int64_t n) {
+    return (unsigned)(B.nbt + (n + ARB_EVAL_CHUNK - 1) / ARB_EVAL_CHUNK);
+}
+
 hipError_t arb_launch_eval_bucket(const ArbGeom& g, const ArbBuckets& B, int32_t prec,
                                   int32_t out_f64, const void* table, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q,
                                   int64_t n, int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
+                                  const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                  hipStream_t s) {
     (void)ninterp;
     const size_t lds = (size_t)B.ncell * B.cellstride * 16;
-    dim3 grid((unsigned)B.nbt), block(block_threads);
+    dim3 grid(eval_grid(B, n)), block(block_threads);
 #define L_EVALB(DD, T, TC, TO)                                                              \
     hipLaunchKernelGGL((k_eval_bucket<DD, T, TC, TO>), grid, block, lds, s, g, B,             \
-                       (const TC*)table, has_comps, has_norm, q, n, ldq, o, offs, perm)
+                       (const TC*)table, has_comps, has_norm, q, n, ldq, o, offs, perm, hot)
     if (g.ndim == 3) ARB_DISPATCH(3, prec, out_f64, L_EVALB);
     else ARB_DISPATCH(4, prec, out_f64, L_EVALB);
 #undef L_EVALB
@@ -1920,14 +1992,15 @@ hipError_t arb_launch_eval_fused(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t out_f64, const double* vals, int32_t ninterp,
                                  int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                  int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                 const int32_t* perm, int32_t block_threads, hipStream_t s) {
+                                 const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                 hipStream_t s) {
     (void)block_threads;
     const size_t lds = arb_fused_lds(g, B, ninterp, arbs3::SEP_THREADS);
     if (lds == 0) return hipErrorInvalidValue;
-    dim3 grid((unsigned)B.nbt), block(arbs3::SEP_THREADS);
+    dim3 grid(eval_grid(B, n)), block(arbs3::SEP_THREADS);
 #define L_FUSED(T, TO, X, Y, Z)                                                               \
     hipLaunchKernelGGL((k_eval_fused3<T, TO, X, Y, Z>), grid, block, lds, s, g, B, vals,      \
-                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
+                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm, hot)
 #define ARB_FUSED_CASE(X, Y, Z)                                                               \
     if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
         if (prec == ARB_PREC_FP64) L_FUSED(double, double, X, Y, Z);                          \
@@ -1969,14 +2042,15 @@ hipError_t arb_launch_eval_fused4(const ArbGeom& g, const ArbBuckets& B, int32_t
                                   int32_t out_f64, const double* vals, int32_t ninterp,
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, const int32_t* offs,
-                                  const int32_t* perm, int32_t block_threads, hipStream_t s) {
+                                  const int32_t* perm, const int32_t* hot, int32_t block_threads,
+                                  hipStream_t s) {
     (void)block_threads;
     const size_t lds = arb_fused4_lds(g, B);
     if (lds == 0) return hipErrorInvalidValue;
-    dim3 grid((unsigned)B.nbt), block(arbs3::SEP_THREADS);
+    dim3 grid(eval_grid(B, n)), block(arbs3::SEP_THREADS);
 #define L_FUSED4(T, TO, X, Y, Z, TT)                                                          \
     hipLaunchKernelGGL((k_eval_fused4<T, TO, X, Y, Z, TT>), grid, block, lds, s, g, B, vals,  \
-                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm)
+                       ninterp, has_comps, has_norm, q, n, ldq, o, offs, perm, hot)
 #define ARB_FUSED4_CASE(X, Y, Z, TT)                                                          \
     if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z && B.b[3] == TT) {                          \
         if (prec == ARB_PREC_FP64) L_FUSED4(double, double, X, Y, Z, TT);                     \
@@ -2076,15 +2150,15 @@ hipError_t arb_launch_eval_nodes(const ArbGeom& g, const ArbBuckets& B, int32_t 
                                  int32_t out_f64, const void* tiles, const void* qrec,
                                  int32_t qstride, int64_t n,
                                  const ArbOut& o, const int32_t* offs, const int32_t* perm,
-                                 int32_t block_threads, hipStream_t s) {
+                                 const int32_t* hot, int32_t block_threads, hipStream_t s) {
     int64_t hoff = 0;
     arb_tiles_bytes(B, ninterp, &hoff);
     const TileHead* heads = (const TileHead*)((const char*)tiles + hoff);
-    dim3 grid((unsigned)B.nbt), block(block_threads);
+    dim3 grid(eval_grid(B, n)), block(block_threads);
 #define L_NODES(TO, NI, X, Y, Z)                                                              \
     hipLaunchKernelGGL((k_eval_tiles3<TO, NI, X, Y, Z>), grid, block, 0, s, g, B,             \
                        (const typename TileRec<NI>::type*)tiles, heads, (const float4*)qrec,  \
-                       qstride, n, o, offs, perm)
+                       qstride, n, o, offs, perm, hot)
 #define L_NODES_NI(TO, X, Y, Z)                                                               \
     if (ninterp == 4) L_NODES(TO, 4, X, Y, Z);                                                \
     else if (ninterp == 3) L_NODES(TO, 3, X, Y, Z);                                           \

@@ -367,10 +367,11 @@ k_grp_count(const uint32_t* __restrict__ ent, const int32_t* __restrict__ tot, i
     }
 }
 
-// One workgroup of 256 threads per segment.
+// One workgroup of 256 threads per segment.  hot[0] = 1 when a key below
+// nbt (a bucket; nbt is the rows outside the grid) holds more than chunk rows.
 __global__ void __launch_bounds__(256)
 k_grp_starts(const int32_t* __restrict__ tot, int hs, int32_t* __restrict__ kcnt,
-             int32_t* __restrict__ offs, int64_t nbt) {
+             int32_t* __restrict__ offs, int64_t nbt, int32_t* __restrict__ hot, int32_t chunk) {
     __shared__ int32_t wsum[2][4];
     __shared__ int32_t sbase;
     int32_t total;
@@ -385,6 +386,7 @@ k_grp_starts(const int32_t* __restrict__ tot, int hs, int32_t* __restrict__ kcnt
         const int32_t first = sbase + lex;
         kcnt[key] = first;
         offs[key] = first;
+        if (key < nbt && c > chunk) hot[0] = 1;
     }
 }
 
@@ -580,7 +582,7 @@ hipError_t arb_launch_scan_long(const int32_t* in, int32_t* out, int64_t len, in
 int64_t arb_rs_ntiles(int64_t n) { return (int64_t)cdiv(n, RS_TILE); }
 
 int64_t arb_rs_first_elems(int64_t ngroups) {
-    return ngroups + 1 + (int64_t)cdiv(ngroups + 1, SF_BLOCK);
+    return ngroups + 1 + (int64_t)cdiv(ngroups + 1, SF_BLOCK) + 1;
 }
 
 // Sort (k0, v0) by the low `bits` bits of the keys.  Buffers: k1/v1 the same
@@ -642,7 +644,7 @@ int arb_group_sort_ok(int64_t n, int bits) {
 // no set order inside a key, and offs[b] (b in [0, nbt]) is the first
 // position of key b.  hist: the tile histograms of the keys' high 8 bits
 // (key >> (bits - 8)), written with the keys; hscan as for the radix sort;
-// ent: n words; kcnt: nbt + 1 ints.
+// ent: n words; kcnt: arb_rs_first_elems(nbt) ints (the last: the hot flag).
 hipError_t arb_launch_group_sort(const uint32_t* keys, uint32_t* ent, int32_t* perm, int64_t n,
                                  int bits, const int32_t* hist, int32_t* hscan, int32_t* offs,
                                  int64_t nbt, int32_t* kcnt, hipStream_t s) {
@@ -650,13 +652,16 @@ hipError_t arb_launch_group_sort(const uint32_t* keys, uint32_t* ent, int32_t* p
     const int64_t ntiles = arb_rs_ntiles(n);
     int32_t* tot = hscan + 256 * ntiles;
     const int hs = bits - 8;
+    const int64_t klen = arb_rs_first_elems(nbt);
+    int32_t* hot = kcnt + klen - 1;
     hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
-                       tot, kcnt, nbt + 1);
+                       tot, kcnt, klen);
     hipLaunchKernelGGL(k_rs_pass1<true>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s, keys,
                        ent, (int32_t*)nullptr, n, hist, hscan, tot, ntiles, hs);
     const unsigned gt = (unsigned)(ntiles + 256);    // tiles of all segments: at most this
     hipLaunchKernelGGL(k_grp_count, dim3(gt), dim3(RS_THREADS), 0, s, ent, tot, hs, kcnt, nbt);
-    hipLaunchKernelGGL(k_grp_starts, dim3(256), dim3(256), 0, s, tot, hs, kcnt, offs, nbt);
+    hipLaunchKernelGGL(k_grp_starts, dim3(256), dim3(256), 0, s, tot, hs, kcnt, offs, nbt, hot,
+                       ARB_EVAL_CHUNK);
     hipLaunchKernelGGL(k_grp_place, dim3(gt), dim3(RS_THREADS), 0, s, ent, tot, hs, kcnt, perm, n,
                        nbt);
     return hipGetLastError();

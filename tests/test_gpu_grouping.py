@@ -8,7 +8,9 @@ with the cell in the key (ARB_Q_SORT_CELL), by the LSD radix sort.  Every
 grouping gives the same bits as the direct path, which evaluates each row
 where it stands (node tiles for mixed 3D fields, the coefficient table
 otherwise).  Pinned here on the distributions that stress the grouping:
-uniform rows; every row in one cell (one key: a segment of many tiles);
+uniform rows; every row in one cell (one key: a segment of many tiles,
+and a bucket shared by many evaluation workgroups, on every bucketed
+kernel);
 rows in a few neighbouring buckets; half the rows outside the box (the
 sentinel key, the largest); row counts at tile boundaries; more than 2^24
 rows (the radix sort), compared on the device.
@@ -91,6 +93,44 @@ def test_grouped_4d_gives_direct_bits(env):
                 if a is not None:
                     assert np.array_equal(a, b, equal_nan=True), prec
         t.close()
+
+
+@pytest.mark.parametrize("d,prec,kind", [
+    (3, "fp64", "table"), (3, "fp64", "on_the_fly"), (3, "mixed", "table"),
+    (3, "mixed", "nodes"), (4, "fp64", "table"), (4, "mixed", "table"),
+    (4, "fp64", "on_the_fly")])
+def test_hot_bucket_every_kernel(env, d, prec, kind):
+    """A bucket holding many more rows than one workgroup takes (every row
+    of the set in one cell, plus a uniform spread) is shared by several
+    workgroups on every bucketed kernel: staged table, on-the-fly (3D and
+    4D) and node tiles; the bits are the direct path's on the table."""
+    torch, G, NN, _, _, _ = env
+    if d == 3:
+        rows, _ = W.dipole3d(seed=9, n=(30, 26, 22), nq=1)
+        cls, kw = G.tricubic, {"mode": "both"}
+    else:
+        rows, _ = W.moving4d(seed=6, n=(14, 12, 12, 8), nq=1)
+        cls, kw = G.quadcubic, {"mode": "norm"}
+    ref_t = cls(rows.copy(), "quiet", precision=prec, **kw)
+    lo, hi = np.array(ref_t._lo), np.array(ref_t._hi)
+    rng = np.random.default_rng(17)
+    span = hi - lo
+    cell = lo + 0.43 * span + rng.uniform(0.0, 1e-3, size=(30_000, d)) * span
+    q = np.concatenate([cell, rng.uniform(lo, hi, size=(5_000, d))])
+    q = q[rng.permutation(len(q))]
+    base = NN.ARB_Q_TABLE if (d == 3 and prec == "mixed" and kind == "table") else 0
+    ref = _run(torch, ref_t, q, base | NN.ARB_Q_DIRECT)
+    t = cls(rows.copy(), "quiet", precision=prec, table_limit=1, **kw) \
+        if kind == "on_the_fly" else ref_t
+    got = _run(torch, t, q, base | NN.ARB_Q_SORTED)
+    if kind == "on_the_fly":
+        assert t.field_info().on_the_fly == 1
+    for a, b in zip(ref, got):
+        if a is not None:
+            assert np.array_equal(a, b, equal_nan=True), (d, prec, kind)
+    if t is not ref_t:
+        t.close()
+    ref_t.close()
 
 
 def test_more_rows_than_the_grouping_takes(env):
