@@ -62,7 +62,8 @@ def parse():
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run), then exit")
     ap.add_argument("--workload", default="dipole3d",
-                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield", "point"],
+                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield", "point",
+                             "clustered"],
                     help="dipole3d: the headline JSON line; the others print their own line")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the mixed-precision headline path (default flags, output arrays): "
@@ -572,8 +573,56 @@ def bench_point(args, torch):
                       "cpu_oracle_us_per_call": cpu_us, "cpu_sample_calls": m}))
 
 
+def bench_clustered(args, torch):
+    """128^3 dipole field, mode 'both', mixed precision, three FP32 arrays:
+    the step on row sets that load a few buckets heavily (1e7 rows, made on
+    the device): uniform; every row in one cell; in one bucket's span; in a
+    ball over about 250 buckets; half the rows outside the box.  Each on the
+    default grouping and on the LSD radix sort (ARB_Q_SORT_CELL, which also
+    orders the rows by cell inside a bucket), with the stage times."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd import _native as NN
+    from arbinterp_amd.ARBInterp import tricubic
+    rows, _ = W.dipole3d(seed=2, n=(args.grid,) * 3, nq=1)
+    t = tricubic(rows, "quiet", mode="both", precision="mixed")
+    f64 = dict(dtype=torch.float64, device="cuda")
+    lo, hi = torch.tensor(t._lo, **f64), torch.tensor(t._hi, **f64)
+    h = torch.tensor(list(t.field_info().h)[:3], **f64)
+    n = args.nq
+    g = torch.Generator(device="cuda")
+    g.manual_seed(1)
+
+    def u():
+        return torch.rand((n, 3), generator=g, **f64)
+    p0 = lo + 0.41 * (hi - lo)
+    sets = {
+        "uniform": lo + (hi - lo) * u(),
+        "one_cell": torch.floor((p0 - lo) / h) * h + lo + 0.05 * h + 0.9 * h * u(),
+        "one_bucket_span": p0 + u() * h * torch.tensor([4.0, 4.0, 2.0], **f64),
+        "ball_250_buckets": p0 + u() * h * 20.0,
+        "half_outside": torch.where((torch.arange(n, device="cuda") % 2 == 0)[:, None],
+                                    lo + (hi - lo) * u(), lo - 0.2 * (hi - lo) * u() - 0.01),
+    }
+    res = {}
+    out = None
+    for name, q in sets.items():
+        out = t.query_device(q, out=out)
+        res[name] = {}
+        for lab, fl in (("grouping", NN.ARB_Q_SORTED),
+                        ("radix_by_cell", NN.ARB_Q_SORTED | NN.ARB_Q_SORT_CELL)):
+            ms = time_steps(torch, lambda fl=fl: t.query_device(q, out=out, flags=fl),
+                            args.steps, args.warmup)
+            t.query_device(q, out=out, flags=fl | NN.ARB_Q_TIMING)
+            res[name][lab] = {"ms_per_step": ms,
+                              "stage_ms": [round(x, 5) for x in t.timings()]}
+            log("%-18s %-14s %.3f ms  %s" % (name, lab, ms, res[name][lab]["stage_ms"]))
+    print(json.dumps({"workload": "clustered rows, 128^3 dipole field, mode both, mixed, "
+                      "%d rows" % n, "stages": "keys, grouping, -, evaluation, split",
+                      "results": res}))
+
+
 OTHER = {"scalar3d": bench_scalar3d, "moving4d": bench_moving4d, "smooth3d": bench_smooth3d,
-         "multifield": bench_multifield, "point": bench_point}
+         "multifield": bench_multifield, "point": bench_point, "clustered": bench_clustered}
 
 
 if __name__ == "__main__":
