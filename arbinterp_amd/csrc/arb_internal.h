@@ -105,7 +105,7 @@ __device__ __forceinline__ int64_t arb_reduce(const double* __restrict__ q, cons
         double iu, fl;
         arb_reduce_axis(x[a], g, a, out, &iu, &fl, &loc[a]);
         bad |= isnan(iu);
-        int64_t ia = bad ? 0 : (int64_t)fl;
+        int64_t ia = bad ? 0 : (int64_t)(uint32_t)fl;   // 0 <= fl <= npos < 2^32
         flat += ia * mul;
         mul *= g.npos[a];
     }
@@ -133,7 +133,7 @@ __device__ __forceinline__ int64_t arb_reduce_axes(const double* __restrict__ q,
         double iu, fl;
         arb_reduce_axis(x[a], g, a, out, &iu, &fl, &loc[a]);
         bad |= isnan(iu);
-        ia[a] = bad ? 0 : (int64_t)fl;
+        ia[a] = bad ? 0 : (int64_t)(uint32_t)fl;        // 0 <= fl <= npos < 2^32
         flat += ia[a] * mul;
         mul *= g.npos[a];
     }

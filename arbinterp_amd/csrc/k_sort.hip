@@ -127,6 +127,19 @@ __device__ __forceinline__ int32_t digits_exscan(int32_t v, int32_t* wsum, int32
 // PACK (the grouping sort's first pass, digit = the high 8 bits, shift =
 // bits - 8): the output is one word per pair, the key's low `shift` bits
 // above the 24-bit element index, in kout; vout is not used.
+// The tile in LDS, in digit order: keys and values, or (PACK) the packed
+// words and their digits (40 KB instead of 64 KB: more tiles per CU).
+template <bool PACK>
+struct Pass1Tile {
+    uint32_t skey[RS_TILE];
+    int32_t sval[RS_TILE];
+};
+template <>
+struct Pass1Tile<true> {
+    uint32_t sword[RS_TILE];
+    uint8_t sdig[RS_TILE];
+};
+
 template <bool PACK>
 __global__ void __launch_bounds__(RS_THREADS)
 k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
@@ -137,8 +150,7 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
     __shared__ int32_t gbase[256];              // global start of the tile's part of each run
     __shared__ int32_t toff[256];               // tile-local start of each digit's run
     __shared__ int32_t wsum[2][4];
-    __shared__ uint32_t skey[RS_TILE];
-    __shared__ int32_t sval[RS_TILE];
+    __shared__ Pass1Tile<PACK> L;
     const bool dt = threadIdx.x < 256;
     const int dg0 = dt ? threadIdx.x : 0;
     const int64_t tile = xcd_tile(blockIdx.x, ntiles);
@@ -160,26 +172,32 @@ k_rs_pass1(const uint32_t* __restrict__ kin, uint32_t* __restrict__ kout,
         const int64_t e = base + (int64_t)j * RS_THREADS + threadIdx.x;
         if (e < n) {
             const uint32_t k = __builtin_nontemporal_load(&kin[e]);
-            const int lp = atomicAdd(&cnt[(k >> shift) & 255u], 1);
+            const uint32_t dg = (k >> shift) & 255u;
+            const int lp = atomicAdd(&cnt[dg], 1);
             // lp < RS_TILE: the tile holds tc[d] pairs of digit d, its run
             // starts at toff[d] and the runs tile [0, RS_TILE)
-            skey[lp] = k;
-            sval[lp] = (int32_t)e;
+            if constexpr (PACK) {
+                L.sword[lp] = ((k & ((1u << shift) - 1u)) << 24) | (uint32_t)e;
+                L.sdig[lp] = (uint8_t)dg;
+            } else {
+                L.skey[lp] = k;
+                L.sval[lp] = (int32_t)e;
+            }
         }
     }
     __syncthreads();
     const int cntt = (int)((n - base) < RS_TILE ? (n - base) : RS_TILE);
-    const uint32_t lmask = (1u << shift) - 1u;
     for (int i = threadIdx.x; i < cntt; i += RS_THREADS) {
-        const uint32_t k = skey[i];
-        const uint32_t dg = (k >> shift) & 255u;
+        uint32_t dg;
+        if constexpr (PACK) dg = L.sdig[i];
+        else dg = (L.skey[i] >> shift) & 255u;
         const int64_t pos = (int64_t)gbase[dg] + (i - toff[dg]);
         if (pos < 0 || pos >= n) continue;    // unreachable: a digit's run stays inside [0, n)
         if constexpr (PACK) {
-            kout[pos] = ((k & lmask) << 24) | (uint32_t)sval[i];
+            kout[pos] = L.sword[i];
         } else {
-            vout[pos] = sval[i];
-            kout[pos] = k;
+            vout[pos] = L.sval[i];
+            kout[pos] = L.skey[i];
         }
     }
 }
