@@ -274,7 +274,7 @@ hipError_t arb_launch_unpack(int32_t ndim, int32_t out_f64, int32_t has_comps, i
 // o = position / chunk; the launchers size their grids nbt + ceil(n /
 // chunk).  hot[0] (set with the bucket bounds) is nonzero when some bucket
 // holds more than a chunk; when it is zero those workgroups end at once.
-constexpr int32_t ARB_EVAL_CHUNK = 2048;
+constexpr int32_t ARB_EVAL_CHUNK = 512;
 // Radix-sort tiles: ARB_RS_THREADS threads x ARB_RS_ITEMS pairs (8192; the
 // longer digit runs of the larger tile make the scattered writes wider).
 constexpr int ARB_RS_THREADS = 512;
