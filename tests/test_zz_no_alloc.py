@@ -5,8 +5,9 @@ workspace carved from a fixed static device array, so its outputs must be
 bitwise those of arb_query (comps, norms, grads, cell indices, NaN rows
 written into the positions).  Checked through the C ABI on two calls in a
 row, the second with fewer rows (the array keeps the first call's contents),
-for the mixed 3D node-tile path, the FP64 3D table path, a mixed 4D field
-and FP64 3D and 4D fields without a table (coefficients on the fly); device
+for the mixed 3D node-tile path, the FP64 3D table path, a mixed 4D field,
+FP64 3D and 4D fields without a table (coefficients on the fly) and rows
+clustered in one cell (buckets shared by several workgroups); device
 memory must not change across the calls; bad sizes, refused
 flags and unprepared fields give ARB_EINVAL.
 """
@@ -60,14 +61,21 @@ def _same(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
-def _queries(t, nq, seed):
-    """Positions over the box widened by a cell per side (some rows outside)."""
+def _queries(t, nq, seed, clustered=False):
+    """Positions over the box widened by a cell per side (some rows outside);
+    clustered: four fifths of them inside one cell instead (a bucket of many
+    more rows than one workgroup of the evaluation takes)."""
     rng = np.random.default_rng(seed)
     d = t.NDIM
     lo = np.array([getattr(t, _ax + "IntMin") for _ax in "xyzt"[:d]])
     hi = np.array([getattr(t, _ax + "IntMax") for _ax in "xyzt"[:d]])
     h = (hi - lo) / np.array(t.nPos[:d], dtype=np.float64)
-    return rng.uniform(lo - h, hi + h, size=(nq, d))
+    q = rng.uniform(lo - h, hi + h, size=(nq, d))
+    if clustered:
+        k = nq * 4 // 5
+        q[:k] = lo + 0.37 * (hi - lo) + rng.uniform(0.1, 0.9, size=(k, d)) * h
+        q = q[rng.permutation(nq)]
+    return q
 
 
 CASES = [
@@ -85,12 +93,18 @@ CASES = [
      "tricubic", {"mode": "both", "table_limit": 1}, "fp64", 0, (300_000, 120_000)),
     ("fp64_4d_scalar_on_the_fly", lambda: W.moving4d(seed=10, n=(20, 18, 16, 10), nq=1)[0],
      "quadcubic", {"table_limit": 1}, "fp64", 0, (250_000, 90_000)),
+    # most rows in one cell: a bucket shared by many workgroups
+    ("mixed3d_both_clustered", lambda: W.dipole3d(seed=7, n=(40, 36, 32), nq=1)[0], "tricubic",
+     {"mode": "both"}, "mixed", 0, (300_000, 120_000), True),
+    ("fp64_3d_both_clustered", lambda: W.dipole3d(seed=8, n=(40, 36, 32), nq=1)[0], "tricubic",
+     {"mode": "both"}, "fp64", 0, (300_000, 120_000), True),
 ]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
 def test_noalloc_matches_query(torch, case):
-    name, make, cls, kw, prec, flags, sizes = case
+    name, make, cls, kw, prec, flags, sizes = case[:7]
+    clustered = len(case) > 7 and case[7]
     from arbinterp_amd import ARBInterp as G
     print("noalloc case", name, sizes, flush=True)
     t = getattr(G, cls)(make(), "quiet", precision=prec, **kw)
@@ -104,7 +118,8 @@ def test_noalloc_matches_query(torch, case):
     if (prec == "fp64" or flags & N.ARB_Q_TABLE or t.NDIM == 4) and not otf:
         N.check(L.arb_build_coeffs(t._h, _stream(torch)), "build")
     assert t.field_info().on_the_fly == (1 if otf else 0)
-    qs = [torch.from_numpy(_queries(t, n, 11 + k)).cuda() for k, n in enumerate(sizes)]
+    qs = [torch.from_numpy(_queries(t, n, 11 + k, clustered)).cuda()
+          for k, n in enumerate(sizes)]
     refs, q_refs = [], []
     for q in qs:
         qr = q.clone()
