@@ -76,6 +76,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's CPU model and the cores this process may run on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "cores_available": len(os.sched_getaffinity(0))}
+
+
 def time_steps(torch, fn, steps, warmup):
     for _ in range(warmup):
         fn()
@@ -265,7 +279,8 @@ def main():
                "kind": "port",
                "sample": "%d x 2^16 queries of the same workload through the numpy oracle "
                          "(reference-faithful vectorised restatement), table prebuilt "
-                         "(%.1f s cold fill, single thread)" % (args.cpu_chunks, cold)}
+                         "(%.1f s cold fill, single thread)" % (args.cpu_chunks, cold),
+               "host": host_cpu()}
         log("cpu baseline: %.3e q/s" % cpu["value"])
 
     per_m = res["mixed"]["per"]
