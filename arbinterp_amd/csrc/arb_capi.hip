@@ -210,14 +210,16 @@ struct arb_field {
     int32_t* rs_hscan = nullptr;   // their per-digit scans, then the digit totals
     int64_t rs_n = 0;
     // backward workspace
-    int32_t* bw_key = nullptr;     // per query: cell (-1 outside)
+    int32_t* bw_key = nullptr;     // per query: cell (nc outside)
     int32_t* bw_perm = nullptr;    // queries grouped per cell, index order
+    uint32_t* bw_k1 = nullptr;     // the stable radix sort's second key and
+    int32_t* bw_v1 = nullptr;      // value buffers (n each),
+    int32_t* bw_hist = nullptr;    // its tile histograms (256 per tile)
+    int32_t* bw_hscan = nullptr;   // and their scans (+ 256 digit totals)
     double* bw_rec = nullptr;      // 3D: per query u, v, w, gv, gg / h (8 doubles)
     int64_t bw_n = 0;
-    int32_t* bw_count = nullptr;   // nc + 1
+    int32_t* bw_count = nullptr;   // the radix sort's first positions
     int32_t* bw_offs = nullptr;    // nc + 1
-    int32_t* bw_cursor = nullptr;  // nc + 1
-    int32_t* bw_tmp = nullptr;     // scan temporaries
     double* bw_bbar = nullptr;     // node-adjoint scratch: 3D block tiles, 4D K x nc
     // particle push graph
     hipStream_t cap_stream = nullptr;
@@ -1036,7 +1038,7 @@ static int query_impl(arb_field* f, double* q_dev, int64_t n, int64_t ldq, void*
                 // w.offs (w.first as scratch)
                 HIPCHK(arb_launch_radix_sort(w.k[0], w.v[0], w.k[1], w.v[1], n, bits, w.hist,
                                              w.hscan, 1, 1, w.offs, BB.nbt, cellbits, w.first,
-                                             &in_first, s));
+                                             &in_first, 0, s));
                 HIPCHK(arb_launch_fill_i32(hot, 1, 1, s));
                 perm = w.v[in_first ? 0 : 1];
             }
@@ -1212,41 +1214,60 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             if (f->bw_key) (void)hipFree(f->bw_key);
             if (f->bw_perm) (void)hipFree(f->bw_perm);
             if (f->bw_rec) (void)hipFree(f->bw_rec);
-            f->bw_key = f->bw_perm = nullptr;
+            if (f->bw_k1) (void)hipFree(f->bw_k1);
+            if (f->bw_v1) (void)hipFree(f->bw_v1);
+            if (f->bw_hist) (void)hipFree(f->bw_hist);
+            if (f->bw_hscan) (void)hipFree(f->bw_hscan);
+            f->bw_key = f->bw_perm = f->bw_v1 = f->bw_hist = f->bw_hscan = nullptr;
+            f->bw_k1 = nullptr;
             f->bw_rec = nullptr;
             f->bw_n = 0;
-            HIPCHK(hipMalloc(&f->bw_key, sizeof(int32_t) * (n ? n : 1)));
-            HIPCHK(hipMalloc(&f->bw_perm, sizeof(int32_t) * (n ? n : 1)));
-            if (f->g.ndim == 3) HIPCHK(hipMalloc(&f->bw_rec, sizeof(double) * 8 * (n ? n : 1)));
+            const int64_t m = n ? n : 1;
+            const int64_t hl = 256 * arb_rs_ntiles(m);
+            HIPCHK(hipMalloc(&f->bw_key, sizeof(int32_t) * m));
+            HIPCHK(hipMalloc(&f->bw_perm, sizeof(int32_t) * m));
+            HIPCHK(hipMalloc(&f->bw_k1, sizeof(uint32_t) * m));
+            HIPCHK(hipMalloc(&f->bw_v1, sizeof(int32_t) * m));
+            HIPCHK(hipMalloc(&f->bw_hist, sizeof(int32_t) * hl));
+            HIPCHK(hipMalloc(&f->bw_hscan, sizeof(int32_t) * (hl + 256)));
+            if (f->g.ndim == 3) HIPCHK(hipMalloc(&f->bw_rec, sizeof(double) * 8 * m));
             f->bw_n = n;
         }
         if (!f->bw_bbar) {
-            const int64_t tl = arb_scan_tmp_elems(nc) + 1;
-            HIPCHK(hipMalloc(&f->bw_count, sizeof(int32_t) * (nc + 1)));
+            // the radix sort's first positions, then the cell bounds
+            HIPCHK(hipMalloc(&f->bw_count, sizeof(int32_t) * arb_rs_first_elems(nc)));
             HIPCHK(hipMalloc(&f->bw_offs, sizeof(int32_t) * (nc + 1)));
-            HIPCHK(hipMalloc(&f->bw_cursor, sizeof(int32_t) * (nc + 1)));
-            HIPCHK(hipMalloc(&f->bw_tmp, sizeof(int32_t) * 2 * tl));
             HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * arb_bwd_scratch(f->g)));
         }
-        HIPCHK(arb_launch_fill_i32(f->bw_count, nc + 1, 0, s));
-        HIPCHK(arb_launch_fill_i32(f->bw_cursor, nc + 1, 0, s));
     }
     if (n > 0 && nodes)
         HIPCHK(arb_launch_bwd_tiles(f->g, f->B, f->ninterp, f->tiles, q_dev, n, ldq, gv_dev,
-                                    gg_dev, dq_dev, dval_dev ? f->bw_key : nullptr, f->bw_count,
+                                    gg_dev, dq_dev, dval_dev ? f->bw_key : nullptr, nullptr,
                                     dval_dev ? f->bw_rec : nullptr, s));
     else if (n > 0)
         HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
                                      q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
-                                     dval_dev ? f->bw_key : nullptr, f->bw_count,
+                                     dval_dev ? f->bw_key : nullptr, nullptr,
                                      dval_dev ? f->bw_rec : nullptr, s));
     if (dval_dev) {
-        HIPCHK(arb_launch_scan_long(f->bw_count, f->bw_offs, nc, f->bw_tmp,
-                                    f->bw_tmp + arb_scan_tmp_elems(nc) + 1, s));
-        HIPCHK(arb_launch_seg_scatter(f->bw_key, f->bw_offs, f->bw_cursor, f->bw_perm, n, s));
-        HIPCHK(arb_launch_seg_sort(f->bw_offs, f->bw_perm, nc, s));
+        // the queries grouped per cell, in index order inside a cell (so
+        // every per-cell sum runs in a fixed order): a stable radix sort of
+        // the cell keys, O(n) whatever the distribution (a per-cell sort of
+        // the cell's queries was quadratic in their number)
+        const int32_t* perm = f->bw_perm;
+        if (n == 0) {
+            HIPCHK(arb_launch_fill_i32(f->bw_offs, nc + 1, 0, s));
+        } else {
+            int bits = 1;
+            while (((int64_t)1 << bits) <= nc) ++bits;      // keys 0..nc
+            int in_first = 1;
+            HIPCHK(arb_launch_radix_sort(reinterpret_cast<uint32_t*>(f->bw_key), f->bw_perm,
+                                         f->bw_k1, f->bw_v1, n, bits, f->bw_hist, f->bw_hscan, 0,
+                                         1, f->bw_offs, nc, 0, f->bw_count, &in_first, 1, s));
+            perm = in_first ? f->bw_perm : f->bw_v1;
+        }
         HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_rec,
-                                    f->bw_offs, f->bw_perm, f->bw_bbar, s));
+                                    f->bw_offs, perm, f->bw_bbar, s));
         HIPCHK(arb_launch_bwd_nodes(f->g, f->bw_bbar, dval_dev, s));
     }
     return ARB_OK;
@@ -1581,11 +1602,13 @@ void arb_field_destroy(arb_field* f) {
     if (f->rs_hscan) (void)hipFree(f->rs_hscan);
     if (f->bw_key) (void)hipFree(f->bw_key);
     if (f->bw_perm) (void)hipFree(f->bw_perm);
+    if (f->bw_k1) (void)hipFree(f->bw_k1);
+    if (f->bw_v1) (void)hipFree(f->bw_v1);
+    if (f->bw_hist) (void)hipFree(f->bw_hist);
+    if (f->bw_hscan) (void)hipFree(f->bw_hscan);
     if (f->bw_rec) (void)hipFree(f->bw_rec);
     if (f->bw_count) (void)hipFree(f->bw_count);
     if (f->bw_offs) (void)hipFree(f->bw_offs);
-    if (f->bw_cursor) (void)hipFree(f->bw_cursor);
-    if (f->bw_tmp) (void)hipFree(f->bw_tmp);
     if (f->bw_bbar) (void)hipFree(f->bw_bbar);
     if (f->ev_ready)
         for (auto& e : f->ev) (void)hipEventDestroy(e);

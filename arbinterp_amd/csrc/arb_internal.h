@@ -302,7 +302,7 @@ int64_t arb_rs_first_elems(int64_t ngroups);
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
                                  int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
-                                 int32_t* first, int* in_first, hipStream_t s);
+                                 int32_t* first, int* in_first, int stable, hipStream_t s);
 // Grouping of the rows by key for the bucketed evaluation (k_sort.hip): keys
 // below 2^bits with 9 <= bits <= 16 and n < 2^24 (arb_group_sort_ok); no
 // order inside a key.  kcnt: arb_rs_first_elems(nbt) ints of scratch, whose
@@ -370,9 +370,6 @@ hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* tab
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
                                  int32_t* key, int32_t* count, double* rec, hipStream_t s);
-hipError_t arb_launch_seg_scatter(const int32_t* key, const int32_t* offs, int32_t* cursor,
-                                  int32_t* perm, int64_t n, hipStream_t s);
-hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, hipStream_t s);
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
                                 const int32_t* offs, const int32_t* perm, double* bbar,

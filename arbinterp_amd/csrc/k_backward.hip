@@ -5,9 +5,9 @@
 //   d/dq   = gv grad U + Hess U . gg                  (per query, analytic)
 //   d/df_n = sum_i sum_j c_ij A_{j, l(n, cell_i)}    (U is linear in f)
 //            c_i = gv_i m(x_i) + sum_k gg_ik dm_k(x_i)/h_k
-// The node adjoint is deterministic: queries are grouped per cell (counting
-// sort), each cell's segment is put in query-index order, and the per-cell
-// 64 (256)-vector alpha_bar is summed in that order.  3D: blocks of 4^3
+// The node adjoint is deterministic: queries are grouped per cell in
+// query-index order (a stable radix sort of the cell keys, k_sort.hip), and
+// the per-cell 64 (256)-vector alpha_bar is summed in that order.  3D: blocks of 4^3
 // cells apply the transpose of the separable coefficient passes to their
 // alpha_bar, giving partial gradients on the block's 7^3 node tile, and
 // every node sums the tiles holding it in a fixed block order.  4D:
@@ -131,8 +131,8 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
     int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
     const bool ok = flat < g.nc;
     if (key) {
-        key[i] = ok ? (int32_t)flat : -1;
-        if (ok) atomicAdd(&count[flat], 1);
+        key[i] = ok ? (int32_t)flat : (int32_t)g.nc;  // rows outside: the group after the cells
+        if (ok && count) atomicAdd(&count[flat], 1);
     }
     if (rec && ok) {
         // the node adjoint's per-query record: local coordinates, gv, gg / h
@@ -157,31 +157,6 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
 #pragma unroll
         for (int y = 0; y < D; ++y) acc += (H[x][y] / (g.h[x] * g.h[y])) * gg[i * D + y];
         dq[i * D + x] = acc;
-    }
-}
-
-__global__ void k_seg_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ offs,
-                              int32_t* __restrict__ cursor, int32_t* __restrict__ perm, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t c = key[i];
-    if (c >= 0) perm[offs[c] + atomicAdd(&cursor[c], 1)] = (int32_t)i;
-}
-
-// Put every cell's segment in query-index order (segments are short).
-__global__ void k_seg_sort(const int32_t* __restrict__ offs, int32_t* __restrict__ perm,
-                           int64_t nc) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int32_t s0 = offs[c], s1 = offs[c + 1];
-    for (int32_t s = s0 + 1; s < s1; ++s) {
-        const int32_t v = perm[s];
-        int32_t t = s - 1;
-        while (t >= s0 && perm[t] > v) {
-            perm[t + 1] = perm[t];
-            --t;
-        }
-        perm[t + 1] = v;
     }
 }
 
@@ -469,18 +444,6 @@ hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* tab
             hipLaunchKernelGGL((k_bwd_points<4, float>), grid, block, 0, s, g, (const float*)table,
                                ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
     }
-    return hipGetLastError();
-}
-
-hipError_t arb_launch_seg_scatter(const int32_t* key, const int32_t* offs, int32_t* cursor,
-                                  int32_t* perm, int64_t n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seg_scatter, dim3(grid1(n, 256)), dim3(256), 0, s, key, offs, cursor, perm, n);
-    return hipGetLastError();
-}
-
-hipError_t arb_launch_seg_sort(const int32_t* offs, int32_t* perm, int64_t nc, hipStream_t s) {
-    hipLaunchKernelGGL(k_seg_sort, dim3(grid1(nc, 256)), dim3(256), 0, s, offs, perm, nc);
     return hipGetLastError();
 }
 

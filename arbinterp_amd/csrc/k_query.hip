@@ -1593,8 +1593,8 @@ k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restri
     const int64_t flat = arb_reduce_axes<3>(q + i * ldq, g, loc, ia, &oob);
     const bool ok = flat < g.nc;
     if (key) {
-        key[i] = ok ? (int32_t)flat : -1;
-        if (ok) atomicAdd(&count[flat], 1);
+        key[i] = ok ? (int32_t)flat : (int32_t)g.nc;  // rows outside: the group after the cells
+        if (ok && count) atomicAdd(&count[flat], 1);
     }
     if (rec && ok) {
         double4* r4 = reinterpret_cast<double4*>(rec + i * 8);

@@ -607,7 +607,9 @@ int64_t arb_rs_first_elems(int64_t ngroups) {
 // size as k0/v0; hist 256 * ntiles ints; hscan 256 * ntiles + 256 ints (the
 // per-tile digit prefixes, then the digit totals).  hist0_ready: hist already
 // holds the first pass's tile histograms (written with the keys).
-// index_vals: v0 is not read; the value of pair i is i.  The sorted values
+// index_vals: v0 is not read; the value of pair i is i.  stable: every pass
+// keeps the order of equal keys (with index_vals, the pairs of one key end in
+// index order); else the first pass ranks with LDS atomics.  The sorted values
 // end in v0 when the number of passes is even, else in v1; *in_first reports
 // which.  The sorted keys are not kept: instead offs[g] (g in [0, ngroups])
 // becomes the first sorted position whose key >> gshift is >= g, keys being
@@ -616,7 +618,7 @@ int64_t arb_rs_first_elems(int64_t ngroups) {
 hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_t* v1, int64_t n,
                                  int bits, int32_t* hist, int32_t* hscan, int hist0_ready,
                                  int index_vals, int32_t* offs, int64_t ngroups, int32_t gshift,
-                                 int32_t* first, int* in_first, hipStream_t s) {
+                                 int32_t* first, int* in_first, int stable, hipStream_t s) {
     const int64_t ntiles = arb_rs_ntiles(n);
     int32_t* tot = hscan + 256 * ntiles;
     uint32_t* ka = k0;
@@ -634,7 +636,7 @@ hipError_t arb_launch_radix_sort(uint32_t* k0, int32_t* v0, uint32_t* k1, int32_
         hipLaunchKernelGGL(k_rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, s, hist, ntiles, hscan,
                            tot, (int32_t*)nullptr, (int64_t)0);
         const int32_t* vin = (shift == 0 && index_vals) ? nullptr : va;
-        if (shift == 0 && index_vals && !last)
+        if (shift == 0 && index_vals && !last && !stable)
             hipLaunchKernelGGL(k_rs_pass1<false>, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, s,
                                ka, kb, vb, n, hist, hscan, tot, ntiles, 0);
         else

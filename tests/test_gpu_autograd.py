@@ -81,6 +81,46 @@ def test_backward_deterministic(mods):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_backward_clustered(mods, precision):
+    """Most queries in one cell (plus some outside the box): the node
+    adjoint groups the queries by cell in index order with a stable radix
+    sort, so a cell of tens of thousands of queries costs what it holds; the
+    gradients match the oracle and repeat bit for bit."""
+    torch, G, AG = mods
+    rows, q, w = W.smooth3d(seed=6, n=(22, 20, 18), nq=12_000)
+    t = G.tricubic(rows, "quiet", precision=precision)
+    lo, hi = np.array(t._lo), np.array(t._hi)
+    h = (hi - lo) / np.array(t.nPos[:3], dtype=np.float64)
+    rng = np.random.default_rng(9)
+    cell = lo + 0.41 * (hi - lo) + rng.uniform(0.1, 0.9, size=(48_000, 3)) * h
+    out = lo - rng.uniform(0.5, 2.0, size=(2_000, 3)) * h
+    qq = np.concatenate([cell, q, out])
+    perm = rng.permutation(len(qq))
+    qq = qq[perm]
+    ww = np.concatenate([rng.normal(size=len(cell)), w, rng.normal(size=len(out))])[perm]
+    fld = AG.DifferentiableField(t, dtype=torch.float64)
+    grads = []
+    for _ in range(2):
+        x = torch.from_numpy(qq).cuda().requires_grad_(True)
+        wt = torch.from_numpy(ww).cuda()
+        fld.values.grad = None
+        U, Gr = fld(x)
+        ok = torch.isfinite(U)
+        L = (wt[ok] * U[ok] + (Gr[ok] * Gr[ok]).sum(dim=1)).sum()
+        L.backward()
+        grads.append(fld.values.grad.clone())
+    assert torch.equal(grads[0], grads[1])
+    inside = np.isfinite(U.detach().cpu().numpy())
+    assert inside.sum() == len(cell) + len(q)
+    o = O.tricubic(rows, "quiet")
+    v, g = o.Query(qq[inside].copy())
+    _, dval_ref = o.backward(qq[inside], ww[inside], 2 * g)
+    e_v = _rel(grads[0].cpu().numpy(), dval_ref)
+    print("%s clustered: d/dvalues rel %.2e" % (precision, e_v))
+    assert e_v < (1e-10 if precision == "fp64" else 5e-5)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_full_size_fd_checks(mods, precision):
     """96^3 field, 2^20 FP32 positions: the gradients of the FP64 or mixed
     path against central finite differences of the FP64 interpolant on 256
