@@ -216,6 +216,7 @@ struct arb_field {
     int32_t* bw_v1 = nullptr;      // value buffers (n each),
     int32_t* bw_hist = nullptr;    // its tile histograms (256 per tile)
     int32_t* bw_hscan = nullptr;   // and their scans (+ 256 digit totals)
+    double* bw_part = nullptr;     // sums of chunks of cells holding many queries
     double* bw_rec = nullptr;      // 3D: per query u, v, w, gv, gg / h (8 doubles)
     int64_t bw_n = 0;
     int32_t* bw_count = nullptr;   // the radix sort's first positions
@@ -1218,8 +1219,10 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             if (f->bw_v1) (void)hipFree(f->bw_v1);
             if (f->bw_hist) (void)hipFree(f->bw_hist);
             if (f->bw_hscan) (void)hipFree(f->bw_hscan);
+            if (f->bw_part) (void)hipFree(f->bw_part);
             f->bw_key = f->bw_perm = f->bw_v1 = f->bw_hist = f->bw_hscan = nullptr;
             f->bw_k1 = nullptr;
+            f->bw_part = nullptr;
             f->bw_rec = nullptr;
             f->bw_n = 0;
             const int64_t m = n ? n : 1;
@@ -1230,6 +1233,7 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             HIPCHK(hipMalloc(&f->bw_v1, sizeof(int32_t) * m));
             HIPCHK(hipMalloc(&f->bw_hist, sizeof(int32_t) * hl));
             HIPCHK(hipMalloc(&f->bw_hscan, sizeof(int32_t) * (hl + 256)));
+            HIPCHK(hipMalloc(&f->bw_part, sizeof(double) * arb_bwd_part_elems(f->g, m)));
             if (f->g.ndim == 3) HIPCHK(hipMalloc(&f->bw_rec, sizeof(double) * 8 * m));
             f->bw_n = n;
         }
@@ -1267,7 +1271,7 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             perm = in_first ? f->bw_perm : f->bw_v1;
         }
         HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_rec,
-                                    f->bw_offs, perm, f->bw_bbar, s));
+                                    f->bw_offs, perm, n, f->bw_part, f->bw_bbar, s));
         HIPCHK(arb_launch_bwd_nodes(f->g, f->bw_bbar, dval_dev, s));
     }
     return ARB_OK;
@@ -1606,6 +1610,7 @@ void arb_field_destroy(arb_field* f) {
     if (f->bw_v1) (void)hipFree(f->bw_v1);
     if (f->bw_hist) (void)hipFree(f->bw_hist);
     if (f->bw_hscan) (void)hipFree(f->bw_hscan);
+    if (f->bw_part) (void)hipFree(f->bw_part);
     if (f->bw_rec) (void)hipFree(f->bw_rec);
     if (f->bw_count) (void)hipFree(f->bw_count);
     if (f->bw_offs) (void)hipFree(f->bw_offs);

@@ -370,10 +370,13 @@ hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* tab
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
                                  int32_t* key, int32_t* count, double* rec, hipStream_t s);
+// part: arb_bwd_part_elems(g, n) doubles of scratch (the sums of chunks of
+// cells holding many queries).
+int64_t arb_bwd_part_elems(const ArbGeom& g, int64_t n);
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
-                                const int32_t* offs, const int32_t* perm, double* bbar,
-                                hipStream_t s);
+                                const int32_t* offs, const int32_t* perm, int64_t n, double* part,
+                                double* bbar, hipStream_t s);
 hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s);
 // Doubles of node-adjoint scratch (bbar of arb_launch_bwd_cells / _nodes).
 int64_t arb_bwd_scratch(const ArbGeom& g);

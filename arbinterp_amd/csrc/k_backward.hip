@@ -237,10 +237,112 @@ constexpr int BN = BB + 3;                         // node tile edge
 constexpr int BCELLS = BB * BB * BB;
 constexpr int BTILE = BN * BN * BN;
 
+// A cell holding more than BWD_CHUNK queries is summed in pieces: the
+// queries at sorted positions [o C, (o + 1) C) of such a cell (C =
+// BWD_CHUNK, o C inside the cell) are summed by a chunk kernel into
+// part[o], in index order, and the cell's own sum takes the queries before
+// its first multiple of C in index order, then part[o] for each of its
+// chunks in order: a fixed order, so the result still repeats bit for bit,
+// and a cell of many queries (particles clustered in a few cells) is
+// spread over many threads instead of four.
+constexpr int32_t BWD_CHUNK = 1024;
+
+// The chunk starting at sorted position o C, if it belongs to a cell of
+// more than BWD_CHUNK queries: [s0, s1) and that cell; q, ldq: the query
+// positions (the cell of the query at o C, by the same reduction as the
+// keys).
+template <int D>
+__device__ __forceinline__ bool bwd_chunk(const ArbGeom& g, const double* __restrict__ q,
+                                          int64_t ldq, const int32_t* __restrict__ offs,
+                                          const int32_t* __restrict__ perm, int64_t o,
+                                          int32_t* s0, int32_t* s1) {
+    const int64_t p0 = o * BWD_CHUNK;
+    if (p0 >= offs[g.nc]) return false;              // the rows outside the box follow
+    const int64_t i = perm[p0];
+    double loc[D];
+    bool oob;
+    const int64_t c = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+    if (c < 0 || c >= g.nc) return false;            // cannot happen: p0 is a cell's row
+    const int32_t e = offs[c + 1];
+    if (e - offs[c] <= BWD_CHUNK) return false;      // summed whole by the cell kernel
+    *s0 = (int32_t)p0;
+    *s1 = p0 + BWD_CHUNK < e ? (int32_t)(p0 + BWD_CHUNK) : e;
+    return true;
+}
+
+// 3D: thread gq (of four) of a cell: its 16 entries of alpha_bar, summed over
+// the queries at sorted positions [s0, s1) in order, into acc[p][ez]
+// (entries 4 gq + p + 16 ez).
+__device__ __forceinline__ void abar_row3(const double4& r0, const double4& r1, int gq,
+                                          double (*acc)[4]);
+
+__device__ __forceinline__ void abar_rows3(const double* __restrict__ rec,
+                                           const int32_t* __restrict__ perm, int32_t s0,
+                                           int32_t s1, int gq, double (*acc)[4]) {
+    // two queries' records loaded together (their rows are scattered), then
+    // added in query order
+    int32_t s = s0;
+    for (; s + 2 <= s1; s += 2) {
+        const double4* r4a = reinterpret_cast<const double4*>(rec + (int64_t)perm[s] * 8);
+        const double4* r4b = reinterpret_cast<const double4*>(rec + (int64_t)perm[s + 1] * 8);
+        const double4 a0 = r4a[0], a1 = r4a[1], b0 = r4b[0], b1 = r4b[1];
+        abar_row3(a0, a1, gq, acc);
+        abar_row3(b0, b1, gq, acc);
+    }
+    for (; s < s1; ++s) {
+        const double4* r4 = reinterpret_cast<const double4*>(rec + (int64_t)perm[s] * 8);
+        abar_row3(r4[0], r4[1], gq, acc);
+    }
+}
+
+// One query's contribution (record r0 = u, v, w, gv; r1 = gg / h) to the 16
+// entries of thread gq.
+__device__ __forceinline__ void abar_row3(const double4& r0, const double4& r1, int gq,
+                                          double (*acc)[4]) {
+    {
+        double pu[4], du[4], pv[4], dv[4], pw[4], dw[4];
+        pow3(r0.x, pu, du);
+        pow3(r0.y, pv, dv);
+        pow3(r0.z, pw, dw);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const int exy = 4 * gq + p, ex = exy & 3, ey = exy >> 2;
+            const double a = sel4(pu, ex) * sel4(pv, ey);
+            const double au = sel4(du, ex) * sel4(pv, ey);
+            const double av = sel4(pu, ex) * sel4(dv, ey);
+            const double base = fma(r1.y, av, fma(r1.x, au, r0.w * a));
+            const double gwa = r1.z * a;
+#pragma unroll
+            for (int ez = 0; ez < 4; ++ez) acc[p][ez] += fma(gwa, dw[ez], base * pw[ez]);
+        }
+    }
+}
+
+// 3D chunk sums: four threads per chunk, part[o][entry] (64 entries).
+__global__ void __launch_bounds__(256)
+k_bwd_chunk3(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double* __restrict__ rec,
+             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm, int64_t nchunk,
+             double* __restrict__ part) {
+    const int64_t o = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
+    const int gq = threadIdx.x & 3;
+    int32_t s0, s1;
+    if (o >= nchunk || !bwd_chunk<3>(g, q, ldq, offs, perm, o, &s0, &s1)) return;
+    double acc[4][4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
+    abar_rows3(rec, perm, s0, s1, gq, acc);
+#pragma unroll
+    for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int ez = 0; ez < 4; ++ez) part[o * 64 + 4 * gq + p + 16 * ez] = acc[p][ez];
+}
+
 __global__ void __launch_bounds__(256)
 k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restrict__ offs,
              const int32_t* __restrict__ perm, int64_t nbx, int64_t nby,
-             double* __restrict__ tiles) {
+             const double* __restrict__ part, double* __restrict__ tiles) {
     // alpha_bar[cell][i] (padded); after the z pass the same memory holds
     // t1[z][y][cx][ex] (47 KB in all: three workgroups per CU)
     __shared__ double ab[BCELLS][65];
@@ -259,25 +361,21 @@ k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restric
             for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
         if (x < g.npos[0] && y < g.npos[1] && z < g.npos[2]) {
             const int64_t c = x + g.npos[0] * (y + g.npos[1] * z);
-            const int32_t s1 = offs[c + 1];
-            for (int32_t s = offs[c]; s < s1; ++s) {
-                const int64_t i = perm[s];
-                const double4* r4 = reinterpret_cast<const double4*>(rec + i * 8);
-                const double4 r0 = r4[0], r1 = r4[1];
-                double pu[4], du[4], pv[4], dv[4], pw[4], dw[4];
-                pow3(r0.x, pu, du);
-                pow3(r0.y, pv, dv);
-                pow3(r0.z, pw, dw);
+            const int32_t s0 = offs[c], s1 = offs[c + 1];
+            if (s1 - s0 <= BWD_CHUNK) {
+                abar_rows3(rec, perm, s0, s1, gq, acc);
+            } else {
+                // the queries before the cell's first chunk, then its chunks
+                const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
+                const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
+                abar_rows3(rec, perm, s0, h1, gq, acc);
+                // then the chunks' sums, in chunk order
+                for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
 #pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    const int exy = 4 * gq + p, ex = exy & 3, ey = exy >> 2;
-                    const double a = sel4(pu, ex) * sel4(pv, ey);
-                    const double au = sel4(du, ex) * sel4(pv, ey);
-                    const double av = sel4(pu, ex) * sel4(dv, ey);
-                    const double base = fma(r1.y, av, fma(r1.x, au, r0.w * a));
-                    const double gwa = r1.z * a;
+                    for (int p = 0; p < 4; ++p)
 #pragma unroll
-                    for (int ez = 0; ez < 4; ++ez) acc[p][ez] += fma(gwa, dw[ez], base * pw[ez]);
+                        for (int ez = 0; ez < 4; ++ez)
+                            acc[p][ez] += part[o * 64 + 4 * gq + p + 16 * ez];
                 }
             }
         }
@@ -360,12 +458,31 @@ k_bwd_nodes3(ArbGeom g, const double* __restrict__ tiles, int64_t nbx, int64_t n
 
 // 4D: one wave per cell: alpha_bar (256) summed over the cell's queries in
 // index order, then b_bar = A^T alpha_bar, written slot-major bbar[l][cell].
+// 4D chunk sums (cells of more than BWD_CHUNK queries): one wave per chunk,
+// part[o][entry] (256 entries).
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bwd_chunk(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double* __restrict__ gv,
+            const double* __restrict__ gg, const int32_t* __restrict__ offs,
+            const int32_t* __restrict__ perm, int64_t nchunk, double* __restrict__ part) {
+    constexpr int K = 1 << (2 * D);
+    constexpr int PER = K / 64;
+    const int lane = threadIdx.x & 63;
+    const int64_t o = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    int32_t s0, s1;
+    if (o >= nchunk || !bwd_chunk<D>(g, q, ldq, offs, perm, o, &s0, &s1)) return;
+    double acc[PER];
+    cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
+#pragma unroll
+    for (int r = 0; r < PER; ++r) part[o * K + lane + 64 * r] = acc[r];
+}
+
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
             const double* __restrict__ gv, const double* __restrict__ gg,
             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
-            double* __restrict__ bbar) {
+            const double* __restrict__ part, double* __restrict__ bbar) {
     constexpr int K = 1 << (2 * D);
     constexpr int PER = K / 64;
     __shared__ double abar[4][K];
@@ -374,7 +491,18 @@ k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ 
     const bool live = c < g.nc;
     const int32_t s0 = live ? offs[c] : 0, s1 = live ? offs[c + 1] : 0;
     double acc[PER];
-    cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
+    if (s1 - s0 <= BWD_CHUNK) {
+        cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
+    } else {
+        // the queries before the cell's first chunk, then its chunks
+        const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
+        const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
+        cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, h1, lane, 64, acc);
+        for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
+#pragma unroll
+            for (int r = 0; r < PER; ++r) acc[r] += part[o * K + lane + 64 * r];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < PER; ++r) abar[w][lane + 64 * r] = acc[r];
     __syncthreads();
@@ -461,18 +589,30 @@ int64_t arb_bwd_scratch(const ArbGeom& g) {
     return g.nc * 256;
 }
 
+int64_t arb_bwd_part_elems(const ArbGeom& g, int64_t n) {
+    const int64_t nchunk = (n + BWD_CHUNK - 1) / BWD_CHUNK;
+    return (nchunk ? nchunk : 1) * (g.ndim == 3 ? 64 : 256);
+}
+
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
-                                const int32_t* offs, const int32_t* perm, double* bbar,
-                                hipStream_t s) {
+                                const int32_t* offs, const int32_t* perm, int64_t n, double* part,
+                                double* bbar, hipStream_t s) {
+    const int64_t nchunk = (n + BWD_CHUNK - 1) / BWD_CHUNK;
     if (g.ndim == 3) {
+        if (nchunk > 0)
+            hipLaunchKernelGGL(k_bwd_chunk3, dim3(grid1(nchunk, 64)), dim3(256), 0, s, g, q, ldq,
+                               rec, offs, perm, nchunk, part);
         int64_t nb3[3];
         arb_bwd_blocks3(g, nb3);
         hipLaunchKernelGGL(k_bwd_block3, dim3((unsigned)(nb3[0] * nb3[1] * nb3[2])), dim3(256), 0,
-                           s, g, rec, offs, perm, nb3[0], nb3[1], bbar);
+                           s, g, rec, offs, perm, nb3[0], nb3[1], (const double*)part, bbar);
     } else {
+        if (nchunk > 0)
+            hipLaunchKernelGGL(k_bwd_chunk<4>, dim3(grid1(nchunk, 4)), dim3(256), 0, s, g, q, ldq,
+                               gv, gg, offs, perm, nchunk, part);
         hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, 4)), dim3(256), 0, s, g, A, q, ldq, gv,
-                           gg, offs, perm, bbar);
+                           gg, offs, perm, (const double*)part, bbar);
     }
     return hipGetLastError();
 }
