@@ -243,7 +243,7 @@ class _Interp:
 
     def timings(self):
         """Stage durations (ms) of the last query made with ARB_Q_TIMING:
-        [keys, radix sort (or bucket scan), scatter, evaluation, split into
+        [keys, grouping or radix sort (or bucket scan), scatter, evaluation, split into
         arrays]; the direct path reports its kernel as the evaluation."""
         ms = (ctypes.c_float * 5)()
         N.check(N.lib().arb_last_timings(self._h, ms))

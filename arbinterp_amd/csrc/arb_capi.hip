@@ -744,7 +744,7 @@ static int ensure_tiles(arb_field* f, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-call workspace of the bucketed query path: the radix sort's key and
+// Per-call workspace of the bucketed query path: the sort's key and
 // row-index buffers (two each), its tile histograms and their scans, and the
 // bucket bounds (`first` positions, then `offs`).  Carved from one base
 // address, each piece 256-byte aligned; with base == nullptr only the size

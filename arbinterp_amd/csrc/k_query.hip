@@ -237,9 +237,11 @@ k_query_direct_red32(ArbGeom g, const TC* __restrict__ table, int32_t ninterp,
 
 // ---------------------------------------------------------------------------
 // Cell-bucketed path.  Queries are grouped by the block of cells they fall in
-// (k_keys + the radix sort of k_sort.hip by default; with ARB_Q_GROUP_ATOMIC
-// a counting sort: atomic rank per bucket, one scan, one scatter of the query
-// index).  A workgroup then stages a block's coefficients in LDS with
+// (k_keys + the grouping of k_sort.hip by default: high key digit, then low
+// digit inside each segment, no order inside a bucket; its LSD radix sort
+// past 2^24 rows or 16 key bits, or with the cell in the key; with
+// ARB_Q_GROUP_ATOMIC a counting sort: atomic rank per bucket, one scan, one
+// scatter of the query index).  A workgroup then stages a block's coefficients in LDS with
 // contiguous 16-byte loads -- each cell's coefficients leave HBM once per
 // call -- and evaluates the bucket's queries from LDS.  Output rows are
 // written at the caller's row index, so no unsort pass is needed.  Results do
@@ -343,7 +345,8 @@ k_bucket(ArbGeom g, ArbBuckets B, const TC* __restrict__ table, int32_t ninterp,
     rank[i] = atomicAdd(&count[b], 1);
 }
 
-// Reduction + bucket key for the radix-sorted path (no atomics): key = bucket,
+// Reduction + bucket key for the sorted path (the grouping or the radix sort;
+// no atomics): key = bucket,
 // or nbt for rows evaluated here from the sentinel row (the sort's value is
 // the row index, implicit).
 // One workgroup of KEYS_THREADS threads per radix tile (ARB_RS_THREADS x

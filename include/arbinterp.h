@@ -85,7 +85,7 @@ extern "C" {
 #define ARB_Q_TIMING   64   /* record per-stage HIP events; the call then waits
                                for its own work (see arb_last_timings)           */
 #define ARB_Q_GROUP_ATOMIC 128 /* bucketed path: group queries with per-bucket
-                               atomic ranks instead of the radix sort            */
+                               atomic ranks instead of the sort                  */
 #define ARB_Q_PACKED  256   /* write each row's outputs as one aligned record of
                                arb_field_info.packed_width elements into
                                comps_dev: [comps(3)] [value, grad(d)], zero
@@ -213,7 +213,7 @@ int arb_query(arb_field* f, double* q_dev, int64_t n, int64_t ldq,
  * memory: code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc.  Same arguments, flags and results as arb_query (the same
  * kernels: outputs are bitwise those of arb_query), but no call allocates:
- *   - the bucketed path's per-call workspace (radix-sort keys and row
+ *   - the bucketed path's per-call workspace (sort keys and row
  *     indices, digit histograms, bucket bounds) is carved from one fixed
  *     device array of ARB_NOALLOC_SCRATCH_BYTES per GPU, held from the
  *     library's load until the process ends; separate output arrays are
@@ -315,8 +315,8 @@ int arb_push_rk4(arb_field* f, double* x_dev, double* v_dev, int64_t n, double d
 
 /* Stage durations (ms, HIP events on the call's stream) of the last arb_query
  * made with ARB_Q_TIMING, five entries: [0] reduction + bucket keys (the
- * whole kernel on the direct path, reported in [3]), [1] radix sort (bucket
- * scan with ARB_Q_GROUP_ATOMIC), [2] scatter (ARB_Q_GROUP_ATOMIC), [3]
+ * whole kernel on the direct path, reported in [3]), [1] grouping or radix sort
+ * (bucket scan with ARB_Q_GROUP_ATOMIC), [2] scatter (ARB_Q_GROUP_ATOMIC), [3]
  * evaluation, [4] split of packed records into separate output arrays (0
  * when not used).  No reference counterpart (the reference has no timers);
  * used by the benchmark. */
