@@ -66,9 +66,10 @@ def parse():
                              "clustered"],
                     help="dipole3d: the headline JSON line; the others print their own line")
     ap.add_argument("--profile-only", action="store_true",
-                    help="only the mixed-precision headline path (default flags, output arrays): "
-                         "--steps queries (for counter runs; no oracle, no CPU baseline, no JSON "
-                         "line)")
+                    help="only the headline path (default flags, output arrays): --steps "
+                         "queries (for counter runs; no oracle, no CPU baseline, no JSON line)")
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                    help="field precision of --profile-only (fp64: the FP64 table path)")
     return ap.parse_args()
 
 
@@ -127,7 +128,7 @@ def main():
         n = (args.grid,) * 3
         rows, q = W.dipole3d(seed=2, n=n, nq=args.nq)
         qd = torch.from_numpy(q).cuda()
-        t = tricubic(rows, "quiet", mode="both", precision="mixed")
+        t = tricubic(rows, "quiet", mode="both", precision=args.precision)
         out = t.query_device(qd, flags=args.flags)
         for _ in range(args.steps):
             t.query_device(qd, out=out, flags=args.flags)
