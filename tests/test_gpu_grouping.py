@@ -13,7 +13,8 @@ and a bucket shared by many evaluation workgroups, on every bucketed
 kernel);
 rows in a few neighbouring buckets; half the rows outside the box (the
 sentinel key, the largest); row counts at tile boundaries; more than 2^24
-rows (the radix sort), compared on the device.
+rows (the radix sort), compared on the device; keys of 9, 16 and 17 bits
+(either side of the grouping's limits).
 """
 import numpy as np
 import pytest
@@ -152,3 +153,34 @@ def test_more_rows_than_the_grouping_takes(env):
             assert torch.equal(a, b), n
         del q, ref, got
         torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("nodes,block,bits", [
+    ((24, 22, 20), (4, 4, 2), 9),       # 6 x 5 x 9 = 270 buckets: the grouping's narrowest keys
+    ((41, 41, 41), (1, 1, 1), 16),      # 38^3 = 54872 buckets: its widest
+    ((44, 44, 44), (1, 1, 1), 17)])     # 41^3 = 68921 buckets: the radix sort
+def test_key_widths_at_the_grouping_limits(env, nodes, block, bits):
+    """Bucket keys of 9 and 16 bits (the grouping's limits) and 17 bits (the
+    radix sort), forced by the cell block: the grouped rows give the direct
+    path's bits, FP64 and mixed, on uniform rows and on rows in one cell."""
+    torch, G, NN, _, _, _ = env
+    rows, _ = W.dipole3d(seed=13, n=nodes, nq=1)
+    for prec in ("mixed", "fp64"):
+        t = G.tricubic(rows.copy(), "quiet", mode="both", precision=prec)
+        t.set_tuning(block)
+        info = t.field_info()
+        assert [int(info.block[a]) for a in range(3)] == list(block)
+        nbt = int(np.prod([-(-info.npos[a] // block[a]) for a in range(3)]))
+        assert nbt.bit_length() == bits, (nbt, bits)     # keys 0..nbt
+        lo, hi = np.array(t._lo), np.array(t._hi)
+        rng = np.random.default_rng(bits)
+        span = hi - lo
+        sets = {"uniform": rng.uniform(lo, hi, size=(70_000, 3)),
+                "one_cell": lo + 0.61 * span + rng.uniform(0.0, 1.0, size=(20_000, 3)) * span / 400}
+        for name, q in sets.items():
+            ref = _run(torch, t, q, NN.ARB_Q_DIRECT)
+            got = _run(torch, t, q, NN.ARB_Q_SORTED)
+            for a, b in zip(ref, got):
+                if a is not None:
+                    assert np.array_equal(a, b, equal_nan=True), (prec, name, bits)
+        t.close()
