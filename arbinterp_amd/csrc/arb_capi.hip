@@ -1244,21 +1244,23 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
             HIPCHK(hipMalloc(&f->bw_bbar, sizeof(double) * arb_bwd_scratch(f->g)));
         }
     }
-    if (n > 0 && nodes)
-        HIPCHK(arb_launch_bwd_tiles(f->g, f->B, f->ninterp, f->tiles, q_dev, n, ldq, gv_dev,
-                                    gg_dev, dq_dev, dval_dev ? f->bw_key : nullptr, nullptr,
-                                    dval_dev ? f->bw_rec : nullptr, s));
-    else if (n > 0)
+    // The node adjoint groups the queries per cell in query-index order (so
+    // every per-cell sum runs in a fixed order): a stable radix sort of the
+    // cell keys, O(n) whatever the distribution (a per-cell sort of the
+    // cell's queries was quadratic in their number).  From the node tiles,
+    // d/dq runs in that grouped order (neighbouring threads then read the
+    // same tiles) and writes each query's record at its grouped position;
+    // from the table (a cell's coefficients one contiguous read), in query
+    // order, with the keys and records by row.
+    const bool grouped = dval_dev && nodes;
+    if (grouped && n > 0) HIPCHK(arb_launch_bwd_keys(f->g, q_dev, n, ldq, f->bw_key, s));
+    if (n > 0 && !nodes)
         HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
                                      q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
-                                     dval_dev ? f->bw_key : nullptr, nullptr,
+                                     dval_dev ? f->bw_key : nullptr,
                                      dval_dev ? f->bw_rec : nullptr, s));
+    const int32_t* perm = nullptr;
     if (dval_dev) {
-        // the queries grouped per cell, in index order inside a cell (so
-        // every per-cell sum runs in a fixed order): a stable radix sort of
-        // the cell keys, O(n) whatever the distribution (a per-cell sort of
-        // the cell's queries was quadratic in their number)
-        const int32_t* perm = f->bw_perm;
         if (n == 0) {
             HIPCHK(arb_launch_fill_i32(f->bw_offs, nc + 1, 0, s));
         } else {
@@ -1270,8 +1272,14 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
                                          1, f->bw_offs, nc, 0, f->bw_count, &in_first, 1, s));
             perm = in_first ? f->bw_perm : f->bw_v1;
         }
+    }
+    if (n > 0 && nodes)
+        HIPCHK(arb_launch_bwd_tiles(f->g, f->B, f->ninterp, f->tiles, q_dev, n, ldq, gv_dev,
+                                    gg_dev, dq_dev, perm, dval_dev ? f->bw_rec : nullptr, s));
+    if (dval_dev) {
         HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_rec,
-                                    f->bw_offs, perm, n, f->bw_part, f->bw_bbar, s));
+                                    grouped ? 1 : 0, f->bw_offs, perm, n, f->bw_part, f->bw_bbar,
+                                    s));
         HIPCHK(arb_launch_bwd_nodes(f->g, f->bw_bbar, dval_dev, s));
     }
     return ARB_OK;

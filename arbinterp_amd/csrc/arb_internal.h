@@ -341,11 +341,13 @@ hipError_t arb_launch_query_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t
                                   int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                   int64_t ldq, const ArbOut& o, hipStream_t s);
 // Backward of the last interpolant from the node tiles (mixed 3D; ninterp 4
-// or 1): d/dq, and the node adjoint's per-query cell key and record.
+// or 1): d/dq, and the node adjoint's per-query record.  perm (or nullptr):
+// the queries in grouped order; thread s then takes query perm[s] and writes
+// its record at position s.
 hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                 const void* tiles, const double* q, int64_t n, int64_t ldq,
-                                const double* gv, const double* gg, double* dq, int32_t* key,
-                                int32_t* count, double* rec, hipStream_t s);
+                                const double* gv, const double* gg, double* dq,
+                                const int32_t* perm, double* rec, hipStream_t s);
 // Node-tile evaluation of the bucketed path; qrec: the per-row records k_keys
 // wrote (16 bytes each at a stride of qstride x 16 bytes: FP32 local
 // coordinates, local cell in the bucket).  The records may live in the
@@ -366,17 +368,25 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
                                 int32_t has_comps, int32_t has_norm, const double* q, int64_t n,
                                 int64_t ldq, const ArbOut& o, const int32_t* offs,
                                 const int32_t* perm, hipStream_t s);
+// The node adjoint's cell key per query (flat cell, nc outside the box).
+hipError_t arb_launch_bwd_keys(const ArbGeom& g, const double* q, int64_t n, int64_t ldq,
+                               int32_t* key, hipStream_t s);
+// Backward of interpolant kn from the table, query i on thread i: d/dq,
+// and for the node adjoint (key, rec non-null) the query's cell key and (3D)
+// its record at row i.
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, int32_t* count, double* rec, hipStream_t s);
+                                 int32_t* key, double* rec, hipStream_t s);
 // part: arb_bwd_part_elems(g, n) doubles of scratch (the sums of chunks of
 // cells holding many queries).
 int64_t arb_bwd_part_elems(const ArbGeom& g, int64_t n);
+// rec_grouped: the 3D records sit at the queries' grouped positions (else by
+// row, reached through perm).
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
-                                const int32_t* offs, const int32_t* perm, int64_t n, double* part,
-                                double* bbar, hipStream_t s);
+                                int32_t rec_grouped, const int32_t* offs, const int32_t* perm,
+                                int64_t n, double* part, double* bbar, hipStream_t s);
 hipError_t arb_launch_bwd_nodes(const ArbGeom& g, const double* bbar, double* dval, hipStream_t s);
 // Doubles of node-adjoint scratch (bbar of arb_launch_bwd_cells / _nodes).
 int64_t arb_bwd_scratch(const ArbGeom& g);

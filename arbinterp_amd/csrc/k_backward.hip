@@ -7,7 +7,9 @@
 //            c_i = gv_i m(x_i) + sum_k gg_ik dm_k(x_i)/h_k
 // The node adjoint is deterministic: queries are grouped per cell in
 // query-index order (a stable radix sort of the cell keys, k_sort.hip), and
-// the per-cell 64 (256)-vector alpha_bar is summed in that order.  3D: blocks of 4^3
+// the per-cell 64 (256)-vector alpha_bar is summed in that order.  3D: the
+// per-query records are written by row (table path) or, from the node
+// tiles, at the queries' grouped positions (a cell's records consecutive).  3D: blocks of 4^3
 // cells apply the transpose of the separable coefficient passes to their
 // alpha_bar, giving partial gradients on the block's 7^3 node tile, and
 // every node sums the tiles holding it in a fixed block order.  4D:
@@ -116,13 +118,30 @@ __device__ __forceinline__ void hess4(const TC* __restrict__ a, double u, double
     }
 }
 
-// d/dq per query, and the cell key for the node adjoint.
+// Cell key of each query for the node adjoint's grouping: its flat cell, or
+// nc for rows outside the box (the group after the cells).
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bwd_keys(ArbGeom g, const double* __restrict__ q, int64_t n, int64_t ldq,
+           int32_t* __restrict__ key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double loc[D];
+    bool oob;
+    const int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
+    key[i] = flat < g.nc ? (int32_t)flat : (int32_t)g.nc;
+}
+
+// d/dq per query, thread i on query i (a cell's 64 (256) table coefficients
+// are one contiguous read, so query order costs nothing, and the queries'
+// own arrays are read and written in order), and, for the node adjoint, the
+// query's cell key and (3D) its record at row i.
 template <int D, typename TC>
 __global__ void __launch_bounds__(256)
 k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t kn,
              const double* __restrict__ q, int64_t n, int64_t ldq, const double* __restrict__ gv,
              const double* __restrict__ gg, double* __restrict__ dq, int32_t* __restrict__ key,
-             int32_t* __restrict__ count, double* __restrict__ rec) {
+             double* __restrict__ rec) {
     constexpr int K = 1 << (2 * D);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -130,10 +149,7 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
     bool oob;
     int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
     const bool ok = flat < g.nc;
-    if (key) {
-        key[i] = ok ? (int32_t)flat : (int32_t)g.nc;  // rows outside: the group after the cells
-        if (ok && count) atomicAdd(&count[flat], 1);
-    }
+    if (key) key[i] = ok ? (int32_t)flat : (int32_t)g.nc;  // rows outside: after the cells
     if (rec && ok) {
         // the node adjoint's per-query record: local coordinates, gv, gg / h
         double4* r4 = reinterpret_cast<double4*>(rec + i * 8);
@@ -277,20 +293,24 @@ __device__ __forceinline__ void abar_row3(const double4& r0, const double4& r1, 
                                           double (*acc)[4]);
 
 __device__ __forceinline__ void abar_rows3(const double* __restrict__ rec,
-                                           const int32_t* __restrict__ perm, int32_t s0,
+                                           const int32_t* __restrict__ rperm, int32_t s0,
                                            int32_t s1, int gq, double (*acc)[4]) {
-    // two queries' records loaded together (their rows are scattered), then
-    // added in query order
+    // the record of sorted position s: rec[s] (written at the grouped
+    // positions: a cell's records are consecutive) or rec[rperm[s]] (written
+    // by row); two loaded together, then added in query order
     int32_t s = s0;
     for (; s + 2 <= s1; s += 2) {
-        const double4* r4a = reinterpret_cast<const double4*>(rec + (int64_t)perm[s] * 8);
-        const double4* r4b = reinterpret_cast<const double4*>(rec + (int64_t)perm[s + 1] * 8);
+        const int64_t ra = rperm ? (int64_t)rperm[s] : s;
+        const int64_t rb = rperm ? (int64_t)rperm[s + 1] : s + 1;
+        const double4* r4a = reinterpret_cast<const double4*>(rec + ra * 8);
+        const double4* r4b = reinterpret_cast<const double4*>(rec + rb * 8);
         const double4 a0 = r4a[0], a1 = r4a[1], b0 = r4b[0], b1 = r4b[1];
         abar_row3(a0, a1, gq, acc);
         abar_row3(b0, b1, gq, acc);
     }
     for (; s < s1; ++s) {
-        const double4* r4 = reinterpret_cast<const double4*>(rec + (int64_t)perm[s] * 8);
+        const int64_t ra = rperm ? (int64_t)rperm[s] : s;
+        const double4* r4 = reinterpret_cast<const double4*>(rec + ra * 8);
         abar_row3(r4[0], r4[1], gq, acc);
     }
 }
@@ -321,8 +341,8 @@ __device__ __forceinline__ void abar_row3(const double4& r0, const double4& r1, 
 // 3D chunk sums: four threads per chunk, part[o][entry] (64 entries).
 __global__ void __launch_bounds__(256)
 k_bwd_chunk3(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double* __restrict__ rec,
-             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm, int64_t nchunk,
-             double* __restrict__ part) {
+             const int32_t* __restrict__ offs, const int32_t* __restrict__ perm,
+             const int32_t* __restrict__ rperm, int64_t nchunk, double* __restrict__ part) {
     const int64_t o = (int64_t)blockIdx.x * 64 + (threadIdx.x >> 2);
     const int gq = threadIdx.x & 3;
     int32_t s0, s1;
@@ -332,7 +352,7 @@ k_bwd_chunk3(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double*
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
-    abar_rows3(rec, perm, s0, s1, gq, acc);
+    abar_rows3(rec, rperm, s0, s1, gq, acc);
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -341,7 +361,7 @@ k_bwd_chunk3(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double*
 
 __global__ void __launch_bounds__(256)
 k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restrict__ offs,
-             const int32_t* __restrict__ perm, int64_t nbx, int64_t nby,
+             const int32_t* __restrict__ rperm, int64_t nbx, int64_t nby,
              const double* __restrict__ part, double* __restrict__ tiles) {
     // alpha_bar[cell][i] (padded); after the z pass the same memory holds
     // t1[z][y][cx][ex] (47 KB in all: three workgroups per CU)
@@ -363,12 +383,12 @@ k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restric
             const int64_t c = x + g.npos[0] * (y + g.npos[1] * z);
             const int32_t s0 = offs[c], s1 = offs[c + 1];
             if (s1 - s0 <= BWD_CHUNK) {
-                abar_rows3(rec, perm, s0, s1, gq, acc);
+                abar_rows3(rec, rperm, s0, s1, gq, acc);
             } else {
                 // the queries before the cell's first chunk, then its chunks
                 const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
                 const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
-                abar_rows3(rec, perm, s0, h1, gq, acc);
+                abar_rows3(rec, rperm, s0, h1, gq, acc);
                 // then the chunks' sums, in chunk order
                 for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
 #pragma unroll
@@ -551,26 +571,36 @@ inline unsigned grid1(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs);
 
 }  // namespace
 
+hipError_t arb_launch_bwd_keys(const ArbGeom& g, const double* q, int64_t n, int64_t ldq,
+                               int32_t* key, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (g.ndim == 3)
+        hipLaunchKernelGGL(k_bwd_keys<3>, dim3(grid1(n, 256)), dim3(256), 0, s, g, q, n, ldq, key);
+    else
+        hipLaunchKernelGGL(k_bwd_keys<4>, dim3(grid1(n, 256)), dim3(256), 0, s, g, q, n, ldq, key);
+    return hipGetLastError();
+}
+
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, int32_t* count, double* rec, hipStream_t s) {
+                                 int32_t* key, double* rec, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 grid(grid1(n, 256)), block(256);
     if (g.ndim == 3) {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<3, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<3, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
     } else {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<4, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<4, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, count, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
     }
     return hipGetLastError();
 }
@@ -596,17 +626,18 @@ int64_t arb_bwd_part_elems(const ArbGeom& g, int64_t n) {
 
 hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double* q, int64_t ldq,
                                 const double* gv, const double* gg, const double* rec,
-                                const int32_t* offs, const int32_t* perm, int64_t n, double* part,
-                                double* bbar, hipStream_t s) {
+                                int32_t rec_grouped, const int32_t* offs, const int32_t* perm,
+                                int64_t n, double* part, double* bbar, hipStream_t s) {
+    const int32_t* rperm = rec_grouped ? nullptr : perm;
     const int64_t nchunk = (n + BWD_CHUNK - 1) / BWD_CHUNK;
     if (g.ndim == 3) {
         if (nchunk > 0)
             hipLaunchKernelGGL(k_bwd_chunk3, dim3(grid1(nchunk, 64)), dim3(256), 0, s, g, q, ldq,
-                               rec, offs, perm, nchunk, part);
+                               rec, offs, perm, rperm, nchunk, part);
         int64_t nb3[3];
         arb_bwd_blocks3(g, nb3);
         hipLaunchKernelGGL(k_bwd_block3, dim3((unsigned)(nb3[0] * nb3[1] * nb3[2])), dim3(256), 0,
-                           s, g, rec, offs, perm, nb3[0], nb3[1], (const double*)part, bbar);
+                           s, g, rec, offs, rperm, nb3[0], nb3[1], (const double*)part, bbar);
     } else {
         if (nchunk > 0)
             hipLaunchKernelGGL(k_bwd_chunk<4>, dim3(grid1(nchunk, 4)), dim3(256), 0, s, g, q, ldq,

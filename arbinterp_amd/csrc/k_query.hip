@@ -1577,30 +1577,29 @@ __device__ __forceinline__ void hermite_weights2(float u, float* ddw) {
 // x^3, y^3, z^3 terms, phi''(u) = 12u - 6) and the FP32 contractions of the
 // residual records with the Hermite weights w, w' and w''.  The model
 // carries the large, cancelling part of the Hessian in FP64, so only the
-// residuals' rounding reaches it.  Also writes the node adjoint's per-query
-// cell key and record (as k_bwd_points does).
+// residuals' rounding reaches it.  With perm (the queries grouped by cell),
+// thread s takes the query perm[s] and writes the node adjoint's record of
+// that query at position s (as k_bwd_points does); without, query s.
 template <int NI, int BX, int BY, int BZ>
 __global__ void __launch_bounds__(256)
 k_bwd_tiles3(ArbGeom g, ArbBuckets B, const typename TileRec<NI>::type* __restrict__ tiles,
              const TileHead* __restrict__ heads, const double* __restrict__ q, int64_t n,
              int64_t ldq, const double* __restrict__ gv, const double* __restrict__ gg,
-             double* __restrict__ dq, int32_t* __restrict__ key, int32_t* __restrict__ count,
+             double* __restrict__ dq, const int32_t* __restrict__ perm,
              double* __restrict__ rec) {
     using S = TileShape<NI, BX, BY, BZ>;
     using R = typename TileRec<NI>::type;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm ? (int64_t)perm[s] : s;
+    if (i < 0 || i >= n) return;                    // unreachable: perm holds row indices
     double loc[3];
     int64_t ia[3];
     bool oob;
     const int64_t flat = arb_reduce_axes<3>(q + i * ldq, g, loc, ia, &oob);
     const bool ok = flat < g.nc;
-    if (key) {
-        key[i] = ok ? (int32_t)flat : (int32_t)g.nc;  // rows outside: the group after the cells
-        if (ok && count) atomicAdd(&count[flat], 1);
-    }
     if (rec && ok) {
-        double4* r4 = reinterpret_cast<double4*>(rec + i * 8);
+        double4* r4 = reinterpret_cast<double4*>(rec + s * 8);
         r4[0] = make_double4(loc[0], loc[1], loc[2], gv[i]);
         r4[1] = make_double4(gg[i * 3] / g.h[0], gg[i * 3 + 1] / g.h[1], gg[i * 3 + 2] / g.h[2], 0.0);
     }
@@ -2125,8 +2124,8 @@ hipError_t arb_launch_build_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t
 
 hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t ninterp,
                                 const void* tiles, const double* q, int64_t n, int64_t ldq,
-                                const double* gv, const double* gg, double* dq, int32_t* key,
-                                int32_t* count, double* rec, hipStream_t s) {
+                                const double* gv, const double* gg, double* dq,
+                                const int32_t* perm, double* rec, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     if (ninterp != 4 && ninterp != 1) return hipErrorInvalidValue;   // modes with a gradient
     int64_t hoff = 0;
@@ -2136,7 +2135,7 @@ hipError_t arb_launch_bwd_tiles(const ArbGeom& g, const ArbBuckets& B, int32_t n
 #define L_BWD(NI, X, Y, Z)                                                                    \
     hipLaunchKernelGGL((k_bwd_tiles3<NI, X, Y, Z>), grid, block, 0, s, g, B,                  \
                        (const typename TileRec<NI>::type*)tiles, heads, q, n, ldq, gv, gg, dq, \
-                       key, count, rec)
+                       perm, rec)
 #define ARB_BWD_CASE(X, Y, Z)                                                                 \
     if (B.b[0] == X && B.b[1] == Y && B.b[2] == Z) {                                          \
         if (ninterp == 4) L_BWD(4, X, Y, Z);                                                  \
