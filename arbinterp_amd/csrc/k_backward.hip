@@ -221,7 +221,7 @@ __device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __rest
 }
 
 // 3D: a workgroup takes a block of BB^3 cells.  Four threads per cell form
-// alpha_bar into LDS from the queries' records (k_bwd_points: u, v, w, gv,
+// alpha_bar into LDS from the queries' records (u, v, w, gv,
 // gg / h): thread g of a cell sums, over the cell's queries in index order,
 // the entries ex + 4 ey + 16 ez with ex + 4 ey in 4g..4g+3,
 //   gv u^ex v^ey w^ez + gk_u (u^ex)' v^ey w^ez + gk_v u^ex (v^ey)' w^ez
