@@ -292,21 +292,24 @@ __device__ __forceinline__ bool bwd_chunk(const ArbGeom& g, const double* __rest
 __device__ __forceinline__ void abar_row3(const double4& r0, const double4& r1, int gq,
                                           double (*acc)[4]);
 
+template <int NB>
 __device__ __forceinline__ void abar_rows3(const double* __restrict__ rec,
                                            const int32_t* __restrict__ rperm, int32_t s0,
                                            int32_t s1, int gq, double (*acc)[4]) {
     // the record of sorted position s: rec[s] (written at the grouped
     // positions: a cell's records are consecutive) or rec[rperm[s]] (written
-    // by row); two loaded together, then added in query order
+    // by row); NB (1 or 2) loaded together, then added in query order
     int32_t s = s0;
-    for (; s + 2 <= s1; s += 2) {
-        const int64_t ra = rperm ? (int64_t)rperm[s] : s;
-        const int64_t rb = rperm ? (int64_t)rperm[s + 1] : s + 1;
-        const double4* r4a = reinterpret_cast<const double4*>(rec + ra * 8);
-        const double4* r4b = reinterpret_cast<const double4*>(rec + rb * 8);
-        const double4 a0 = r4a[0], a1 = r4a[1], b0 = r4b[0], b1 = r4b[1];
-        abar_row3(a0, a1, gq, acc);
-        abar_row3(b0, b1, gq, acc);
+    if constexpr (NB == 2) {
+        for (; s + 2 <= s1; s += 2) {
+            const int64_t ra = rperm ? (int64_t)rperm[s] : s;
+            const int64_t rb = rperm ? (int64_t)rperm[s + 1] : s + 1;
+            const double4* r4a = reinterpret_cast<const double4*>(rec + ra * 8);
+            const double4* r4b = reinterpret_cast<const double4*>(rec + rb * 8);
+            const double4 a0 = r4a[0], a1 = r4a[1], b0 = r4b[0], b1 = r4b[1];
+            abar_row3(a0, a1, gq, acc);
+            abar_row3(b0, b1, gq, acc);
+        }
     }
     for (; s < s1; ++s) {
         const int64_t ra = rperm ? (int64_t)rperm[s] : s;
@@ -352,29 +355,28 @@ k_bwd_chunk3(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double*
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int ez = 0; ez < 4; ++ez) acc[p][ez] = 0.0;
-    abar_rows3(rec, rperm, s0, s1, gq, acc);
+    abar_rows3<2>(rec, rperm, s0, s1, gq, acc);
 #pragma unroll
     for (int p = 0; p < 4; ++p)
 #pragma unroll
         for (int ez = 0; ez < 4; ++ez) part[o * 64 + 4 * gq + p + 16 * ez] = acc[p][ez];
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restrict__ offs,
              const int32_t* __restrict__ rperm, int64_t nbx, int64_t nby,
              const double* __restrict__ part, double* __restrict__ tiles) {
-    // alpha_bar[cell][i] (padded); after the z pass the same memory holds
-    // t1[z][y][cx][ex] (47 KB in all: three workgroups per CU)
-    __shared__ double ab[BCELLS][65];
+    // the z pass's output, then the y pass's (20.6 KB in all)
     __shared__ double t2[BN][BB][BB][16];           // [z][cy][cx][ex + 4 ey]
-    static_assert(BN * BN * BB * 4 <= BCELLS * 65, "t1 must fit in alpha_bar's memory");
-    double (*t1)[BN][BB][4] = reinterpret_cast<double (*)[BN][BB][4]>(&ab[0][0]);
+    __shared__ double t1[BN][BN][BB][4];            // [z][y][cx][ex]
+    for (int t = threadIdx.x; t < BN * BB * BB * 16; t += 256) (&t2[0][0][0][0])[t] = 0.0;
     const int64_t blk = blockIdx.x;
     const int64_t bx = blk % nbx, by = (blk / nbx) % nby, bz = blk / (nbx * nby);
     {
         const int cl = threadIdx.x >> 2, gq = threadIdx.x & 3;
-        const int64_t x = bx * BB + (cl & 3), y = by * BB + ((cl >> 2) & 3), z = bz * BB + (cl >> 4);
-        double acc[4][4];                          // [pair][ez]
+        const int cx = cl & 3, cy = (cl >> 2) & 3, cz = cl >> 4;
+        const int64_t x = bx * BB + cx, y = by * BB + cy, z = bz * BB + cz;
+        double acc[4][4];                          // [pair][ez]: alpha_bar[4 gq + p + 16 ez]
 #pragma unroll
         for (int p = 0; p < 4; ++p)
 #pragma unroll
@@ -383,12 +385,12 @@ k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restric
             const int64_t c = x + g.npos[0] * (y + g.npos[1] * z);
             const int32_t s0 = offs[c], s1 = offs[c + 1];
             if (s1 - s0 <= BWD_CHUNK) {
-                abar_rows3(rec, rperm, s0, s1, gq, acc);
+                abar_rows3<1>(rec, rperm, s0, s1, gq, acc);
             } else {
                 // the queries before the cell's first chunk, then its chunks
                 const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
                 const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
-                abar_rows3(rec, rperm, s0, h1, gq, acc);
+                abar_rows3<1>(rec, rperm, s0, h1, gq, acc);
                 // then the chunks' sums, in chunk order
                 for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
 #pragma unroll
@@ -399,27 +401,22 @@ k_bwd_block3(ArbGeom g, const double* __restrict__ rec, const int32_t* __restric
                 }
             }
         }
+        // z: t2[z][cy][cx][exy] = sum_{cz + oz = z} sum_ez M[ez][oz] alpha_bar[cell][exy + 16 ez],
+        // from the registers: one wave (one cz) at a time, cz descending, so
+        // every entry adds its terms in ascending oz from 0.0
+        __syncthreads();                           // t2 cleared
+        for (int ph = BB - 1; ph >= 0; --ph) {
+            if (cz == ph) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p)
+                for (int p = 0; p < 4; ++p)
 #pragma unroll
-            for (int ez = 0; ez < 4; ++ez) ab[cl][4 * gq + p + 16 * ez] = acc[p][ez];
-    }
-    __syncthreads();
-    // z: t2[z][cy][cx][exy] = sum_{cz + oz = z} sum_ez M[ez][oz] ab[cell][exy + 16 ez]
-    for (int t = threadIdx.x; t < BN * BB * BB * 16; t += 256) {
-        const int exy = t & 15, cx = (t >> 4) & 3, cy = (t >> 6) & 3, z = t >> 8;
-        double sum = 0.0;
-#pragma unroll
-        for (int oz = 0; oz < 4; ++oz) {
-            const int cz = z - oz;
-            if (cz >= 0 && cz < BB) {
-                const double* a = &ab[cx + BB * (cy + BB * cz)][exy];
-                sum += mcol(oz, a[0], a[16], a[32], a[48]);
+                    for (int oz = 0; oz < 4; ++oz)
+                        t2[cz + oz][cy][cx][4 * gq + p] +=
+                            mcol(oz, acc[p][0], acc[p][1], acc[p][2], acc[p][3]);
             }
+            __syncthreads();
         }
-        t2[z][cy][cx][exy] = sum;
     }
-    __syncthreads();
     // y: t1[z][y][cx][ex] = sum_{cy + oy = y} sum_ey M[ey][oy] t2[z][cy][cx][ex + 4 ey]
     for (int t = threadIdx.x; t < BN * BN * BB * 4; t += 256) {
         const int ex = t & 3, cx = (t >> 2) & 3, r = t >> 4, y = r % BN, z = r / BN;
