@@ -185,3 +185,46 @@ def test_full_size_fd_checks(mods, precision):
     e_node = _rel(gv[nodes], fdn)
     print("FD (%s): positions rel %.2e, nodes rel %.2e" % (precision, e_pos, e_node))
     assert e_pos < FD_TOL and e_node < FD_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_backward_4d_matches_oracle(mods, precision):
+    """Quadcubic field (moving Gaussians, 12 x 11 x 10 x 8 nodes): uniform
+    positions, 3000 in one cell (more than one chunk of the per-cell sum) and
+    500 outside the box.  d/dq and d/dvalues against the oracle's FP64
+    adjoint; the node gradients repeat bit for bit."""
+    torch, G, AG = mods
+    rows, q = W.moving4d(seed=8, n=(12, 11, 10, 8), nq=6_000)
+    t = G.quadcubic(rows.copy(), "quiet", precision=precision)
+    lo, hi = np.array(t._lo), np.array(t._hi)
+    h = (hi - lo) / np.array(t.nPos[:4], dtype=np.float64)
+    rng = np.random.default_rng(12)
+    cell = lo + 0.37 * (hi - lo) + rng.uniform(0.1, 0.9, size=(3_000, 4)) * h
+    out = hi + rng.uniform(0.5, 2.0, size=(500, 4)) * h
+    qq = np.concatenate([q, cell, out])
+    qq = qq[rng.permutation(len(qq))]
+    ww = rng.normal(size=len(qq))
+    fld = AG.DifferentiableField(t, dtype=torch.float64)
+    grads = []
+    for _ in range(2):
+        x = torch.from_numpy(qq).cuda().requires_grad_(True)
+        wt = torch.from_numpy(ww).cuda()
+        fld.values.grad = None
+        U, Gr = fld(x)
+        ok = torch.isfinite(U)
+        L = (wt[ok] * U[ok] + (Gr[ok] * Gr[ok]).sum(dim=1)).sum()
+        L.backward()
+        grads.append((x.grad[ok].cpu().numpy(), fld.values.grad.clone()))
+    assert torch.equal(grads[0][1], grads[1][1])
+    inside = np.isfinite(U.detach().cpu().numpy())
+    assert inside.sum() == len(q) + len(cell)
+    o = O.quadcubic(rows.copy(), "quiet")
+    v, g = o.Query(qq[inside].copy())
+    dq_ref, dval_ref = o.backward(qq[inside], ww[inside], 2 * g)
+    # mixed 4D reads the FP32 coefficient table: its rounding, seen through
+    # this metric (floor 1e-3 of the largest component), bounds the error
+    tol = 1e-10 if precision == "fp64" else 5e-5
+    e_q = _rel(grads[0][0], dq_ref)
+    e_v = _rel(grads[0][1].cpu().numpy(), dval_ref)
+    print("4D %s: d/dq rel %.2e  d/dvalues rel %.2e" % (precision, e_q, e_v))
+    assert e_q < tol and e_v < tol
