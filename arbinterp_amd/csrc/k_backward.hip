@@ -473,8 +473,10 @@ k_bwd_nodes3(ArbGeom g, const double* __restrict__ tiles, int64_t nbx, int64_t n
     dval[nidx] = sum;
 }
 
-// 4D: one wave per cell: alpha_bar (256) summed over the cell's queries in
-// index order, then b_bar = A^T alpha_bar, written slot-major bbar[l][cell].
+// 4D: a workgroup per 32 cells: one wave per cell at a time sums alpha_bar
+// (256) over the cell's queries in index order, then b_bar = A^T alpha_bar
+// for the 32 cells together (each element of A read once per workgroup),
+// written slot-major bbar[l][cell].
 // 4D chunk sums (cells of more than BWD_CHUNK queries): one wave per chunk,
 // part[o][entry] (256 entries).
 template <int D>
@@ -494,6 +496,8 @@ k_bwd_chunk(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double* 
     for (int r = 0; r < PER; ++r) part[o * K + lane + 64 * r] = acc[r];
 }
 
+constexpr int BWD_CELLS4 = 32;                    // cells per workgroup, 4D
+
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ q, int64_t ldq,
@@ -502,35 +506,59 @@ k_bwd_cells(ArbGeom g, const double* __restrict__ A, const double* __restrict__ 
             const double* __restrict__ part, double* __restrict__ bbar) {
     constexpr int K = 1 << (2 * D);
     constexpr int PER = K / 64;
-    __shared__ double abar[4][K];
+    constexpr int CB = BWD_CELLS4;
+    static_assert(K == 256, "one thread per entry l of b_bar");
+    __shared__ double abar[CB][K];
+    __shared__ int32_t any;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t c = (int64_t)blockIdx.x * 4 + w;
-    const bool live = c < g.nc;
-    const int32_t s0 = live ? offs[c] : 0, s1 = live ? offs[c + 1] : 0;
-    double acc[PER];
-    if (s1 - s0 <= BWD_CHUNK) {
-        cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
-    } else {
-        // the queries before the cell's first chunk, then its chunks
-        const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
-        const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
-        cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, h1, lane, 64, acc);
-        for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
+    const int64_t c0 = (int64_t)blockIdx.x * CB;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    // alpha_bar of the workgroup's cells: wave w forms cells w, w + 4, ...
+    for (int k = w; k < CB; k += 4) {
+        const int64_t c = c0 + k;
+        const bool live = c < g.nc;
+        const int32_t s0 = live ? offs[c] : 0, s1 = live ? offs[c + 1] : 0;
+        double acc[PER];
+        if (s1 - s0 <= BWD_CHUNK) {
+            cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, s1, lane, 64, acc);
+        } else {
+            // the queries before the cell's first chunk, then its chunks
+            const int64_t ofirst = ((int64_t)s0 + BWD_CHUNK - 1) / BWD_CHUNK;
+            const int32_t h1 = ofirst * BWD_CHUNK < s1 ? (int32_t)(ofirst * BWD_CHUNK) : s1;
+            cell_abar<D, PER>(g, q, ldq, gv, gg, perm, s0, h1, lane, 64, acc);
+            for (int64_t o = ofirst; o * BWD_CHUNK < s1; ++o) {
 #pragma unroll
-            for (int r = 0; r < PER; ++r) acc[r] += part[o * K + lane + 64 * r];
+                for (int r = 0; r < PER; ++r) acc[r] += part[o * K + lane + 64 * r];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < PER; ++r) abar[k][lane + 64 * r] = acc[r];
+        if (s0 != s1 && lane == 0) any = 1;
+    }
+    __syncthreads();
+    // b_bar[l][c] = sum_j A[j][l] alpha_bar[c][j], j ascending from 0, for
+    // the CB cells at once: thread l reads each A[j][l] once (a row of A is
+    // one coalesced read) and alpha_bar[c][j] is an LDS broadcast.  A cell
+    // without queries gets 0.
+    const int l = threadIdx.x;
+    double b[CB];
+#pragma unroll
+    for (int k = 0; k < CB; ++k) b[k] = 0.0;
+    if (any) {
+        for (int j = 0; j < K; ++j) {
+            const double a = A[(int64_t)j * K + l];
+#pragma unroll
+            for (int k = 0; k < CB; ++k) b[k] += a * abar[k][j];
         }
     }
 #pragma unroll
-    for (int r = 0; r < PER; ++r) abar[w][lane + 64 * r] = acc[r];
-    __syncthreads();
-    if (!live) return;
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int l = lane + 64 * r;
-        double b = 0.0;
-        if (s0 != s1)
-            for (int j = 0; j < K; ++j) b += A[(int64_t)j * K + l] * abar[w][j];
-        bbar[(int64_t)l * g.nc + c] = b;
+    for (int k = 0; k < CB; ++k) {
+        const int64_t c = c0 + k;
+        if (c < g.nc) {
+            const int32_t s0 = offs[c], s1 = offs[c + 1];
+            bbar[(int64_t)l * g.nc + c] = s0 != s1 ? b[k] : 0.0;
+        }
     }
 }
 
@@ -639,8 +667,8 @@ hipError_t arb_launch_bwd_cells(const ArbGeom& g, const double* A, const double*
         if (nchunk > 0)
             hipLaunchKernelGGL(k_bwd_chunk<4>, dim3(grid1(nchunk, 4)), dim3(256), 0, s, g, q, ldq,
                                gv, gg, offs, perm, nchunk, part);
-        hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, 4)), dim3(256), 0, s, g, A, q, ldq, gv,
-                           gg, offs, perm, (const double*)part, bbar);
+        hipLaunchKernelGGL(k_bwd_cells<4>, dim3(grid1(g.nc, BWD_CELLS4)), dim3(256), 0, s, g, A,
+                           q, ldq, gv, gg, offs, perm, (const double*)part, bbar);
     }
     return hipGetLastError();
 }
