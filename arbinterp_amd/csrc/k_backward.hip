@@ -176,16 +176,23 @@ k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t k
     }
 }
 
-// alpha_bar[j] of one cell for the entries j = j0 + stride*r (r < R): the sum
-// over the cell's queries, in query-index order, of gv m_j + sum_x gk_x dm_j/dx.
+// alpha_bar[j] of one 4D cell for the entries j = j0 + 64 r (r < 4; j0 a
+// lane): the sum over the cell's queries, in query-index order, of
+// gv m_j + sum_x gk_x dm_j/dx.  With j = e0 + 4 e1 + 16 e2 + 64 e3, lane j0
+// fixes e0, e1, e2 and r is e3, so m_j = P b0_t[r] with P = u^e0 v^e1 w^e2
+// shared by the lane's four entries:
+//   t_j = (gv P + gk_u P_u + gk_v P_v + gk_w P_w) b0_t[r] + gk_t P b1_t[r].
 template <int D, int R>
 __device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __restrict__ q,
                                           int64_t ldq, const double* __restrict__ gv,
                                           const double* __restrict__ gg, const int32_t* perm,
                                           int32_t s0, int32_t s1, int j0, int stride,
                                           double* acc) {
+    static_assert(D == 4 && R == 4, "4D: lane j0 holds the entries j0 + 64 r");
+    (void)stride;
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    const int e0 = j0 & 3, e1 = (j0 >> 2) & 3, e2 = (j0 >> 4) & 3;
     for (int32_t s = s0; s < s1; ++s) {
         const int64_t i = perm[s];
         double loc[D];
@@ -197,26 +204,15 @@ __device__ __forceinline__ void cell_abar(const ArbGeom& g, const double* __rest
         double gk[D];
 #pragma unroll
         for (int x = 0; x < D; ++x) gk[x] = gg[i * D + x] / g.h[x];
+        const double a0 = sel4(B.b0[0], e0), a1 = sel4(B.b0[1], e1), a2 = sel4(B.b0[2], e2);
+        const double d0 = sel4(B.b1[0], e0), d1 = sel4(B.b1[1], e1), d2 = sel4(B.b1[2], e2);
+        const double a01 = a0 * a1;
+        const double P = a01 * a2;
+        const double Pu = d0 * a1 * a2, Pv = a0 * d1 * a2, Pw = a01 * d2;
+        const double c = v * P + gk[0] * Pu + gk[1] * Pv + gk[2] * Pw;
+        const double e = gk[3] * P;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int j = j0 + stride * r;
-            int e[D];
-#pragma unroll
-            for (int x = 0; x < D; ++x) e[x] = (j >> (2 * x)) & 3;
-            double m = 1.0;
-#pragma unroll
-            for (int x = 0; x < D; ++x) m *= sel4(B.b0[x], e[x]);
-            double t = v * m;
-#pragma unroll
-            for (int x = 0; x < D; ++x) {
-                double dm = sel4(B.b1[x], e[x]);
-#pragma unroll
-                for (int y = 0; y < D; ++y)
-                    if (y != x) dm *= sel4(B.b0[y], e[y]);
-                t += gk[x] * dm;
-            }
-            acc[r] += t;
-        }
+        for (int r = 0; r < R; ++r) acc[r] += c * B.b0[3][r] + e * B.b1[3][r];
     }
 }
 
@@ -473,9 +469,9 @@ k_bwd_nodes3(ArbGeom g, const double* __restrict__ tiles, int64_t nbx, int64_t n
     dval[nidx] = sum;
 }
 
-// 4D: a workgroup per 32 cells: one wave per cell at a time sums alpha_bar
+// 4D: a workgroup per 16 cells: one wave per cell at a time sums alpha_bar
 // (256) over the cell's queries in index order, then b_bar = A^T alpha_bar
-// for the 32 cells together (each element of A read once per workgroup),
+// for the 16 cells together (each element of A read once per workgroup),
 // written slot-major bbar[l][cell].
 // 4D chunk sums (cells of more than BWD_CHUNK queries): one wave per chunk,
 // part[o][entry] (256 entries).
@@ -496,7 +492,7 @@ k_bwd_chunk(ArbGeom g, const double* __restrict__ q, int64_t ldq, const double* 
     for (int r = 0; r < PER; ++r) part[o * K + lane + 64 * r] = acc[r];
 }
 
-constexpr int BWD_CELLS4 = 32;                    // cells per workgroup, 4D
+constexpr int BWD_CELLS4 = 16;                    // cells per workgroup, 4D
 
 template <int D>
 __global__ void __launch_bounds__(256)
