@@ -62,8 +62,8 @@ def parse():
                     help="only run a plain device-to-device copy of MB MiB, 5 times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiler run), then exit")
     ap.add_argument("--workload", default="dipole3d",
-                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "multifield", "point",
-                             "clustered"],
+                    choices=["dipole3d", "scalar3d", "moving4d", "smooth3d", "autograd4d",
+                             "multifield", "point", "clustered"],
                     help="dipole3d: the headline JSON line; the others print their own line")
     ap.add_argument("--profile-only", action="store_true",
                     help="only the headline path (default flags, output arrays): --steps "
@@ -505,6 +505,36 @@ def bench_smooth3d(args, torch):
     print(json.dumps({"workload": "autograd 96^3 scalar, 2^20 FP32 positions", "results": res}))
 
 
+def bench_autograd4d(args, torch):
+    """Quadcubic moving Gaussians (48^3 x 16 nodes), 2e6 FP32 positions:
+    forward alone, and forward + backward of L = sum w U + sum |grad U|^2
+    through the autograd function."""
+    from arbinterp_amd import workloads as W
+    from arbinterp_amd.ARBInterp import quadcubic
+    from arbinterp_amd import autograd as AG
+    rows, q = W.moving4d(seed=3)
+    w = np.random.default_rng(1).normal(size=len(q)).astype(np.float32)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        t = quadcubic(rows.copy(), "quiet", precision=prec)
+        fld = AG.DifferentiableField(t, dtype=torch.float32)
+        x = torch.from_numpy(q.astype(np.float32)).cuda().requires_grad_(True)
+        wt = torch.from_numpy(w).cuda()
+
+        def step(bwd=True):
+            x.grad = None
+            fld.values.grad = None
+            U, G = fld(x)
+            L = (wt * U + (G * G).sum(dim=1)).sum()
+            if bwd:
+                L.backward()
+        fwd = _events_ms(torch, lambda: step(False), args.steps, args.warmup)
+        both = _events_ms(torch, step, args.steps, args.warmup)
+        res[prec] = {"ms_per_forward": fwd, "ms_per_forward_backward": both, "queries": len(q)}
+    print(json.dumps({"workload": "autograd 48^3 x 16 quadcubic, 2e6 FP32 positions",
+                      "results": res}))
+
+
 def bench_multifield(args, torch):
     """64 uneven scalar fields in one persistent launch, then the RK4 push of
     1e6 particles for 1000 steps (dt = 1e-3) in field 0."""
@@ -638,6 +668,7 @@ def bench_clustered(args, torch):
 
 
 OTHER = {"scalar3d": bench_scalar3d, "moving4d": bench_moving4d, "smooth3d": bench_smooth3d,
+         "autograd4d": bench_autograd4d,
          "multifield": bench_multifield, "point": bench_point, "clustered": bench_clustered}
 
 
