@@ -1252,11 +1252,15 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     // same tiles) and writes each query's record at its grouped position;
     // from the table (a cell's coefficients one contiguous read), in query
     // order, with the keys and records by row.
+    // (4D, whose cells hold 2 KB of coefficients each, runs d/dq in the
+    // grouped order too; it has no records)
     const bool grouped = dval_dev && nodes;
-    if (grouped && n > 0) HIPCHK(arb_launch_bwd_keys(f->g, q_dev, n, ldq, f->bw_key, s));
-    if (n > 0 && !nodes)
+    const bool grouped4 = dval_dev && !nodes && f->g.ndim == 4;
+    if ((grouped || grouped4) && n > 0)
+        HIPCHK(arb_launch_bwd_keys(f->g, q_dev, n, ldq, f->bw_key, s));
+    if (n > 0 && !nodes && !grouped4)
         HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
-                                     q_dev, n, ldq, gv_dev, gg_dev, dq_dev,
+                                     q_dev, n, ldq, gv_dev, gg_dev, dq_dev, nullptr,
                                      dval_dev ? f->bw_key : nullptr,
                                      dval_dev ? f->bw_rec : nullptr, s));
     const int32_t* perm = nullptr;
@@ -1276,6 +1280,10 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     if (n > 0 && nodes)
         HIPCHK(arb_launch_bwd_tiles(f->g, f->B, f->ninterp, f->tiles, q_dev, n, ldq, gv_dev,
                                     gg_dev, dq_dev, perm, dval_dev ? f->bw_rec : nullptr, s));
+    if (n > 0 && grouped4)
+        HIPCHK(arb_launch_bwd_points(f->g, f->precision, f->table, f->ninterp, f->ninterp - 1,
+                                     q_dev, n, ldq, gv_dev, gg_dev, dq_dev, perm, nullptr,
+                                     nullptr, s));
     if (dval_dev) {
         HIPCHK(arb_launch_bwd_cells(f->g, f->A, q_dev, ldq, gv_dev, gg_dev, f->bw_rec,
                                     grouped ? 1 : 0, f->bw_offs, perm, n, f->bw_part, f->bw_bbar,

@@ -371,13 +371,15 @@ hipError_t arb_launch_eval_perm(const ArbGeom& g, const ArbBuckets& B, int32_t p
 // The node adjoint's cell key per query (flat cell, nc outside the box).
 hipError_t arb_launch_bwd_keys(const ArbGeom& g, const double* q, int64_t n, int64_t ldq,
                                int32_t* key, hipStream_t s);
-// Backward of interpolant kn from the table, query i on thread i: d/dq,
-// and for the node adjoint (key, rec non-null) the query's cell key and (3D)
-// its record at row i.
+// Backward of interpolant kn from the table: d/dq, and for the node adjoint
+// (key, rec non-null) the query's cell key and (3D) its record at row i.
+// perm (or nullptr): the queries in grouped order; thread s then takes
+// query perm[s].
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, double* rec, hipStream_t s);
+                                 const int32_t* perm, int32_t* key, double* rec,
+                                 hipStream_t s);
 // part: arb_bwd_part_elems(g, n) doubles of scratch (the sums of chunks of
 // cells holding many queries).
 int64_t arb_bwd_part_elems(const ArbGeom& g, int64_t n);

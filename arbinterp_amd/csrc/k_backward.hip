@@ -132,19 +132,25 @@ k_bwd_keys(ArbGeom g, const double* __restrict__ q, int64_t n, int64_t ldq,
     key[i] = flat < g.nc ? (int32_t)flat : (int32_t)g.nc;
 }
 
-// d/dq per query, thread i on query i (a cell's 64 (256) table coefficients
-// are one contiguous read, so query order costs nothing, and the queries'
-// own arrays are read and written in order), and, for the node adjoint, the
-// query's cell key and (3D) its record at row i.
+// d/dq per query, and, for the node adjoint, the query's cell key and (3D)
+// its record at row i.  Without perm, thread i takes query i: 3D reads a
+// cell's 64 table coefficients as one contiguous 512-byte run, so query
+// order costs nothing and the queries' own arrays are read and written in
+// order.  With perm (the queries grouped by cell; 4D, whose 256
+// coefficients of a cell are 2 KB), thread s takes query perm[s], so
+// neighbouring threads read the same cell's coefficients.
 template <int D, typename TC>
 __global__ void __launch_bounds__(256)
 k_bwd_points(ArbGeom g, const TC* __restrict__ table, int32_t ninterp, int32_t kn,
              const double* __restrict__ q, int64_t n, int64_t ldq, const double* __restrict__ gv,
-             const double* __restrict__ gg, double* __restrict__ dq, int32_t* __restrict__ key,
+             const double* __restrict__ gg, double* __restrict__ dq,
+             const int32_t* __restrict__ perm, int32_t* __restrict__ key,
              double* __restrict__ rec) {
     constexpr int K = 1 << (2 * D);
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t i = perm ? (int64_t)perm[s] : s;
+    if (i < 0 || i >= n) return;                    // unreachable: perm holds row indices
     double loc[D];
     bool oob;
     int64_t flat = arb_reduce<D>(q + i * ldq, g, loc, &oob);
@@ -605,23 +611,24 @@ hipError_t arb_launch_bwd_keys(const ArbGeom& g, const double* q, int64_t n, int
 hipError_t arb_launch_bwd_points(const ArbGeom& g, int32_t prec, const void* table,
                                  int32_t ninterp, int32_t kn, const double* q, int64_t n,
                                  int64_t ldq, const double* gv, const double* gg, double* dq,
-                                 int32_t* key, double* rec, hipStream_t s) {
+                                 const int32_t* perm, int32_t* key, double* rec,
+                                 hipStream_t s) {
     if (n <= 0) return hipSuccess;
     dim3 grid(grid1(n, 256)), block(256);
     if (g.ndim == 3) {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<3, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, perm, key, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<3, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, perm, key, rec);
     } else {
         if (prec == ARB_PREC_FP64)
             hipLaunchKernelGGL((k_bwd_points<4, double>), grid, block, 0, s, g, (const double*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, perm, key, rec);
         else
             hipLaunchKernelGGL((k_bwd_points<4, float>), grid, block, 0, s, g, (const float*)table,
-                               ninterp, kn, q, n, ldq, gv, gg, dq, key, rec);
+                               ninterp, kn, q, n, ldq, gv, gg, dq, perm, key, rec);
     }
     return hipGetLastError();
 }
