@@ -1249,11 +1249,11 @@ int arb_query_backward(arb_field* f, const double* q_dev, int64_t n, int64_t ldq
     // cell keys, O(n) whatever the distribution (a per-cell sort of the
     // cell's queries was quadratic in their number).  From the node tiles,
     // d/dq runs in that grouped order (neighbouring threads then read the
-    // same tiles) and writes each query's record at its grouped position;
-    // from the table (a cell's coefficients one contiguous read), in query
-    // order, with the keys and records by row.
-    // (4D, whose cells hold 2 KB of coefficients each, runs d/dq in the
-    // grouped order too; it has no records)
+    // same tiles) and writes each query's record at its grouped position.
+    // From the 3D table (a cell's 64 coefficients one contiguous 512-byte
+    // read), in query order, with the keys and records by row.  From the
+    // 4D table (2 KB per cell), in the grouped order again; 4D has no
+    // records.
     const bool grouped = dval_dev && nodes;
     const bool grouped4 = dval_dev && !nodes && f->g.ndim == 4;
     if ((grouped || grouped4) && n > 0)
